@@ -1,0 +1,150 @@
+/*
+ * ptbgpu.h - C ABI of the MI355X density-matrix engine (libptbgpu.so).
+ *
+ * Every integer is int64_t because the reference is compiled with -i8
+ * (source/Makefile:37): a Fortran `integer` on the caller's side is 8 bytes.
+ * Matrices are the reference's packed upper storage: element (i,j), i <= j,
+ * 1-based, at lin(i,j) = i + j(j-1)/2 (source/lin.f:5-11,
+ * source/solve.f:149-200), length ndim*(ndim+1)/2, FP64.
+ *
+ * Entry points without the _dev suffix take host arrays (the true drop-in:
+ * the engine copies in and out).  The _dev variants take device pointers on
+ * the context's device (inputs already resident in HBM); xyz, aoat and all
+ * scalar outputs stay on the host.
+ *
+ * Threading: one context per host thread; calls on one context are not
+ * re-entrant.  Each context owns one HIP stream (default priority) and a
+ * device workspace that grows on demand and is kept across calls; caller
+ * pointers are never retained after a call returns.
+ *
+ * Every call returns PTBGPU_OK (0) or one of the status codes below;
+ * ptbgpu_last_error() gives a message for the last failure on a context.
+ */
+#ifndef PTBGPU_H
+#define PTBGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ptbgpu_status {
+    PTBGPU_OK = 0,
+    PTBGPU_E_ARG = 1,      /* invalid argument (size, null pointer, range) */
+    PTBGPU_E_MODE = 2,     /* unsupported submatrix_columns / submatrix_mode:
+                              the reference stops (source/sm.f90:159-161,249-251,312-314) */
+    PTBGPU_E_NOT_SPD = 3,  /* an overlap submatrix is not positive definite
+                              (dsygvd INFO>0; the reference ignores it, source/sm.f90:299) */
+    PTBGPU_E_NAN = 4,      /* NaN in a result (source/purification.f90:179-181,491-492) */
+    PTBGPU_E_KMEANS = 5,   /* k-means++ fault (source/sm.f90:478-481) */
+    PTBGPU_E_HIP = 6,      /* HIP runtime error */
+    PTBGPU_E_NOMEM = 7,    /* device allocation failed */
+    PTBGPU_E_NOCONV = 8    /* eigensolver or iteration did not converge */
+};
+
+/* arithmetic of the per-submatrix solve */
+enum ptbgpu_precision {
+    PTBGPU_FP64 = 0,   /* plain FP64 drop-in */
+    PTBGPU_MIXED = 1   /* low-precision MFMA solve + FP64 refinement */
+};
+
+typedef struct ptbgpu_ctx ptbgpu_ctx;
+
+typedef struct {
+    int64_t precision;    /* enum ptbgpu_precision */
+    int64_t strict;       /* 1: a non-SPD overlap block fails the call with
+                             PTBGPU_E_NOT_SPD; 0: count it in the report and go on */
+    int64_t verbose;      /* 1: print the reference's progress lines */
+    int64_t refine_steps; /* MIXED only: FP64 refinement steps, 0 = default */
+} ptbgpu_opts;
+
+/* Phase slots of ptbgpu_report.t_phase (labels of source/sm.f90:76-386). */
+enum ptbgpu_phase {
+    PTBGPU_T_ALLOC = 0,        /* 'alloc' (device upload / workspace) */
+    PTBGPU_T_COMBINATION = 1,  /* 'submatrix combination' */
+    PTBGPU_T_STATS = 2,        /* 'submatrix stats' (pattern + index sets) */
+    PTBGPU_T_CONSTRUCTION = 3, /* 'submatrix construction' (gather) */
+    PTBGPU_T_EIGEN = 4,        /* 'submatrix eigendecomp' */
+    PTBGPU_T_FE = 5,           /* 'submatrix fe' */
+    PTBGPU_T_MUITER = 6,       /* 'mu iter' (occ + Nel bisection) */
+    PTBGPU_T_FINAL = 7,        /* 'submatrix final iter' (dmat + scatter) */
+    PTBGPU_T_DEALLOC = 8,      /* 'submatrix dealloc' (download) */
+    PTBGPU_T_TOTAL = 9,
+    PTBGPU_T_NSLOT = 12
+};
+
+typedef struct {
+    double mu;            /* final chemical potential */
+    double nel;           /* sum(fe*occ) at mu */
+    int64_t muiter;       /* bisection passes (the printed muiter count) */
+    int64_t nsm;          /* number of submatrices */
+    int64_t sum_m;        /* sum of submatrix dimensions */
+    int64_t max_m;        /* largest submatrix dimension */
+    double sum_m3;        /* sum of m^3 (work measure) */
+    int64_t not_spd;      /* overlap blocks that failed Cholesky */
+    double t_phase[PTBGPU_T_NSLOT]; /* seconds, enum ptbgpu_phase */
+} ptbgpu_report;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Create a context on HIP device `device`. flags: reserved, pass 0. */
+int ptbgpu_init(ptbgpu_ctx **ctx, int64_t device, int64_t flags);
+int ptbgpu_finalize(ptbgpu_ctx *ctx);
+const char *ptbgpu_strerror(int64_t status);
+const char *ptbgpu_last_error(const ptbgpu_ctx *ctx);
+/* Block until all work on the context's stream is done. */
+int ptbgpu_sync(ptbgpu_ctx *ctx);
+/* The context's HIP stream (hipStream_t), for callers that time or order work. */
+int ptbgpu_stream(ptbgpu_ctx *ctx, void **stream);
+
+/* ---- filter / analyse ----------------------------------------------------
+ * Replaces filter_matrix_sym (source/pgtb.f90:1532-1548): A(k) = 0 where
+ * abs(A(k)) < filter (strict), in place over the packed matrix.  counts, if
+ * not NULL, receives the analyse_matrix_sym figure (source/pgtb.f90:1549-1579)
+ * "nonzero elements" = number of strictly positive entries of the full
+ * symmetric matrix (diagonal once, off-diagonal twice) before [0] and after [1]
+ * filtering. */
+int ptbgpu_filter_sym(ptbgpu_ctx *ctx, int64_t ndim, double *A, double filter,
+                      int64_t counts[2]);
+int ptbgpu_filter_sym_dev(ptbgpu_ctx *ctx, int64_t ndim, double *dA, double filter,
+                          int64_t counts[2]);
+/* analyse_matrix_sym alone (source/pgtb.f90:1549-1579): positive-entry count. */
+int ptbgpu_analyse_sym(ptbgpu_ctx *ctx, int64_t ndim, const double *A, int64_t *npos);
+int ptbgpu_analyse_sym_dev(ptbgpu_ctx *ctx, int64_t ndim, const double *dA, int64_t *npos);
+
+/* ---- submatrix density build ---------------------------------------------
+ * Replaces sm_stupid_simple(ndim,nelref,Hvec,Svec,Pvec,n,xyz,aoat,
+ * submatrix_columns,submatrix_mode) (source/sm.f90:34-409).
+ *   xyz(3,n) bohr, aoat(ndim) 1-based atom of each AO (host arrays);
+ *   submatrix_columns: 1 single_columns, 2 atoms_columns, 3 full_matrix,
+ *                      4 kmeans_r_heuristic (source/purification_settings.f90:9-12);
+ *   submatrix_mode: 2 submatrix_sygv (1 = submatrix_sign -> PTBGPU_E_MODE, as
+ *                   the reference stops there).
+ * P is fully overwritten.  opts and rep may be NULL (defaults: FP64, compat). */
+int ptbgpu_sm_density(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const double *H,
+                      const double *S, double *P, int64_t n, const double *xyz,
+                      const int64_t *aoat, int64_t submatrix_columns,
+                      int64_t submatrix_mode, const ptbgpu_opts *opts,
+                      ptbgpu_report *rep);
+int ptbgpu_sm_density_dev(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const double *dH,
+                          const double *dS, double *dP, int64_t n, const double *xyz,
+                          const int64_t *aoat, int64_t submatrix_columns,
+                          int64_t submatrix_mode, const ptbgpu_opts *opts,
+                          ptbgpu_report *rep);
+
+/* ---- density checks (source/metrics.f90) ---------------------------------
+ * get_nel_packed (:202-230) = tr(P S); get_band_structure_E_packed (:253-277)
+ * = tr(P H); idempotent_packed (:316-360) deviation = |sum(0.5 P S P - P)|
+ * over the full matrix, as the reference writes it (a sum, not a norm). */
+int ptbgpu_get_nel(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const double *S,
+                   double *nel);
+int ptbgpu_band_energy(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const double *H,
+                       double *energy);
+int ptbgpu_idempotency(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const double *S,
+                       double *deviation);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTBGPU_H */

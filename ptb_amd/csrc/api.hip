@@ -1,0 +1,187 @@
+// C ABI entry points: context management, workspace, filter/analyse.
+#include <cstdarg>
+#include <cstring>
+
+#include "common.hpp"
+
+namespace ptb {
+
+int fail(ptbgpu_ctx *ctx, int status, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->last_error = buf;
+    return status;
+}
+
+void *ws(ptbgpu_ctx *ctx, const char *name, size_t bytes) {
+    DevBuf &b = ctx->bufs[name];
+    if (b.bytes >= bytes && b.ptr) return b.ptr;
+    if (b.ptr) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    if (hipMalloc(&b.ptr, want) != hipSuccess) {
+        (void)hipGetLastError();
+        b.ptr = nullptr;
+        fail(ctx, PTBGPU_E_NOMEM, "device allocation of %zu bytes for '%s' failed", want, name);
+        return nullptr;
+    }
+    b.bytes = want;
+    return b.ptr;
+}
+
+void *hws(ptbgpu_ctx *ctx, const char *name, size_t bytes) {
+    DevBuf &b = ctx->host_bufs[name];
+    if (b.bytes >= bytes && b.ptr) return b.ptr;
+    if (b.ptr) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipHostFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    if (hipHostMalloc(&b.ptr, want, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        b.ptr = nullptr;
+        fail(ctx, PTBGPU_E_NOMEM, "pinned host allocation of %zu bytes for '%s' failed", want,
+             name);
+        return nullptr;
+    }
+    b.bytes = want;
+    return b.ptr;
+}
+
+}  // namespace ptb
+
+extern "C" {
+
+const char *ptbgpu_strerror(int64_t status) {
+    switch (status) {
+        case PTBGPU_OK: return "ok";
+        case PTBGPU_E_ARG: return "invalid argument";
+        case PTBGPU_E_MODE: return "unsupported submatrix mode";
+        case PTBGPU_E_NOT_SPD: return "overlap submatrix is not positive definite";
+        case PTBGPU_E_NAN: return "NaN encountered";
+        case PTBGPU_E_KMEANS: return "k-means clustering failed";
+        case PTBGPU_E_HIP: return "HIP runtime error";
+        case PTBGPU_E_NOMEM: return "device allocation failed";
+        case PTBGPU_E_NOCONV: return "iteration did not converge";
+        default: return "unknown status";
+    }
+}
+
+const char *ptbgpu_last_error(const ptbgpu_ctx *ctx) {
+    return ctx ? ctx->last_error.c_str() : "no context";
+}
+
+int ptbgpu_init(ptbgpu_ctx **out, int64_t device, int64_t flags) {
+    (void)flags;
+    if (!out) return PTBGPU_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PTBGPU_E_HIP;
+    if (device < 0 || device >= ndev) return PTBGPU_E_ARG;
+    auto *ctx = new ptbgpu_ctx();
+    ctx->device = (int)device;
+    PTB_HIP(ctx, hipSetDevice(ctx->device));
+    PTB_HIP(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    PTB_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    ctx->num_cus = prop.multiProcessorCount;
+    *out = ctx;
+    return PTBGPU_OK;
+}
+
+int ptbgpu_finalize(ptbgpu_ctx *ctx) {
+    if (!ctx) return PTBGPU_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->bufs)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    for (auto &kv : ctx->host_bufs)
+        if (kv.second.ptr) (void)hipHostFree(kv.second.ptr);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PTBGPU_OK;
+}
+
+int ptbgpu_sync(ptbgpu_ctx *ctx) {
+    if (!ctx) return PTBGPU_E_ARG;
+    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PTBGPU_OK;
+}
+
+int ptbgpu_stream(ptbgpu_ctx *ctx, void **stream) {
+    if (!ctx || !stream) return PTBGPU_E_ARG;
+    *stream = (void *)ctx->stream;
+    return PTBGPU_OK;
+}
+
+int ptbgpu_filter_sym_dev(ptbgpu_ctx *ctx, int64_t ndim, double *dA, double filter,
+                          int64_t counts[2]) {
+    if (!ctx) return PTBGPU_E_ARG;
+    if (ndim < 0 || (ndim > 0 && !dA)) return ptb::fail(ctx, PTBGPU_E_ARG, "bad ndim/pointer");
+    if (ndim == 0) {
+        if (counts) counts[0] = counts[1] = 0;
+        return PTBGPU_OK;
+    }
+    PTB_HIP(ctx, hipSetDevice(ctx->device));
+    return ptb::filter_count(ctx, dA, ndim, filter, true, counts);
+}
+
+int ptbgpu_analyse_sym_dev(ptbgpu_ctx *ctx, int64_t ndim, const double *dA, int64_t *npos) {
+    if (!ctx) return PTBGPU_E_ARG;
+    if (ndim < 0 || (ndim > 0 && !dA)) return ptb::fail(ctx, PTBGPU_E_ARG, "bad ndim/pointer");
+    int64_t c[2] = {0, 0};
+    if (ndim > 0) {
+        PTB_HIP(ctx, hipSetDevice(ctx->device));
+        PTB_TRY(ptb::filter_count(ctx, const_cast<double *>(dA), ndim, 0.0, false, c));
+    }
+    if (npos) *npos = c[0];
+    return PTBGPU_OK;
+}
+
+int ptbgpu_filter_sym(ptbgpu_ctx *ctx, int64_t ndim, double *A, double filter,
+                      int64_t counts[2]) {
+    if (!ctx) return PTBGPU_E_ARG;
+    if (ndim < 0 || (ndim > 0 && !A)) return ptb::fail(ctx, PTBGPU_E_ARG, "bad ndim/pointer");
+    if (ndim == 0) {
+        if (counts) counts[0] = counts[1] = 0;
+        return PTBGPU_OK;
+    }
+    PTB_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t bytes = sizeof(double) * (size_t)(ndim * (ndim + 1) / 2);
+    auto *d = static_cast<double *>(ptb::ws(ctx, "io_a", bytes));
+    if (!d) return PTBGPU_E_NOMEM;
+    PTB_HIP(ctx, hipMemcpyAsync(d, A, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PTB_TRY(ptb::filter_count(ctx, d, ndim, filter, true, counts));
+    PTB_HIP(ctx, hipMemcpyAsync(A, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PTBGPU_OK;
+}
+
+int ptbgpu_analyse_sym(ptbgpu_ctx *ctx, int64_t ndim, const double *A, int64_t *npos) {
+    if (!ctx) return PTBGPU_E_ARG;
+    if (ndim < 0 || (ndim > 0 && !A)) return ptb::fail(ctx, PTBGPU_E_ARG, "bad ndim/pointer");
+    if (ndim == 0) {
+        if (npos) *npos = 0;
+        return PTBGPU_OK;
+    }
+    PTB_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t bytes = sizeof(double) * (size_t)(ndim * (ndim + 1) / 2);
+    auto *d = static_cast<double *>(ptb::ws(ctx, "io_a", bytes));
+    if (!d) return PTBGPU_E_NOMEM;
+    PTB_HIP(ctx, hipMemcpyAsync(d, A, bytes, hipMemcpyHostToDevice, ctx->stream));
+    int64_t c[2];
+    PTB_TRY(ptb::filter_count(ctx, d, ndim, 0.0, false, c));
+    if (npos) *npos = c[0];
+    return PTBGPU_OK;
+}
+
+}  // extern "C"

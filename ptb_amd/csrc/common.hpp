@@ -1,0 +1,101 @@
+// Shared definitions of the engine: context, device workspace, error macros,
+// wavefront reductions.  gfx950 only (64-wide wavefronts).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ptbgpu.h"
+
+#define PTB_WAVE 64
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct ptbgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    std::map<std::string, DevBuf> bufs;      // named device workspace, grows on demand
+    std::map<std::string, DevBuf> host_bufs; // pinned host staging
+    int num_cus = 0;
+};
+
+namespace ptb {
+
+// Record an error on the context and return its status.
+int fail(ptbgpu_ctx *ctx, int status, const char *fmt, ...);
+
+// Named device workspace of at least `bytes` (contents undefined after growth).
+void *ws(ptbgpu_ctx *ctx, const char *name, size_t bytes);
+// Named pinned host buffer of at least `bytes`.
+void *hws(ptbgpu_ctx *ctx, const char *name, size_t bytes);
+
+}  // namespace ptb
+
+#define PTB_HIP(ctx, call)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ptb::fail((ctx), PTBGPU_E_HIP, "%s failed: %s (%s:%d)", #call,         \
+                             hipGetErrorString(e_), __FILE__, __LINE__);                   \
+    } while (0)
+
+#define PTB_CHECK_LAUNCH(ctx) PTB_HIP(ctx, hipGetLastError())
+
+#define PTB_TRY(expr)                  \
+    do {                               \
+        int s_ = (expr);               \
+        if (s_ != PTBGPU_OK) return s_; \
+    } while (0)
+
+// ---- wavefront reductions (64 lanes) -------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = PTB_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, PTB_WAVE);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int off = PTB_WAVE / 2; off > 0; off >>= 1) {
+        T o = __shfl_xor(v, off, PTB_WAVE);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Block-wide sum; every thread gets the result. `scratch` holds >= 16 T.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *scratch) {
+    const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
+    const int nw = (blockDim.x + PTB_WAVE - 1) / PTB_WAVE;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) scratch[wid] = v;
+    __syncthreads();
+    T t = 0;
+    for (int w = 0; w < nw; ++w) t += scratch[w];
+    __syncthreads();
+    return t;
+}
+
+// packed-upper index of (i,j), 0-based, i <= j
+__host__ __device__ __forceinline__ int64_t pk(int64_t i, int64_t j) {
+    return i + j * (j + 1) / 2;
+}
+
+// ---- kernels exported between translation units ---------------------------
+namespace ptb {
+int filter_count(ptbgpu_ctx *ctx, double *dA, int64_t ndim, double filter, bool do_filter,
+                 int64_t counts[2]);
+}
