@@ -1,0 +1,219 @@
+"""Python mirror of the reference's hot-path interface over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference routines:
+
+* ``filter_matrix_sym(A, n, filter)``      source/pgtb.f90:1532-1548 (in place)
+* ``analyse_matrix_sym(A, n)``             source/pgtb.f90:1549-1579
+* ``sm_stupid_simple(ndim, nelref, Hvec, Svec, Pvec, n, xyz, aoat,
+  submatrix_columns, submatrix_mode)``     source/sm.f90:34-409 (Pvec overwritten)
+* ``get_nel``, ``band_structure``, ``idempotent``, ``check_density``
+                                           source/metrics.f90
+
+Where the reference stops the process (unsupported mode, k-means fault, NaN),
+the engine returns a status code and these wrappers raise ``PtbGpuError``.
+Arrays are numpy (host, the drop-in path) or torch CUDA tensors for the
+``*_dev`` variants (inputs resident in HBM).  There is no CPU fallback: a
+missing libptbgpu.so raises at load time.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+I64 = ctypes.c_int64
+F64 = ctypes.c_double
+VP = ctypes.c_void_p
+
+PRECISION = {"fp64": 0, "mixed": 1}
+COLUMNS = {"single_columns": 1, "atoms_columns": 2, "full_matrix": 3, "kmeans_r_heuristic": 4}
+
+PHASES = ["alloc", "submatrix combination", "submatrix stats", "submatrix construction",
+          "submatrix eigendecomp", "submatrix fe", "mu iter", "submatrix final iter",
+          "submatrix dealloc", "total"]
+
+
+class PtbGpuError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"ptbgpu status {status}: {message}")
+        self.status = status
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("precision", I64), ("strict", I64), ("verbose", I64), ("refine_steps", I64)]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [("mu", F64), ("nel", F64), ("muiter", I64), ("nsm", I64), ("sum_m", I64),
+                ("max_m", I64), ("sum_m3", F64), ("not_spd", I64), ("t_phase", F64 * 12)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "t_phase"}
+        d["t_phase"] = {PHASES[i]: self.t_phase[i] for i in range(len(PHASES))}
+        return d
+
+
+def _lib():
+    lib = _native.engine_lib()
+    if getattr(lib, "_ptb_typed", False):
+        return lib
+    lib.ptbgpu_strerror.restype = ctypes.c_char_p
+    lib.ptbgpu_strerror.argtypes = [I64]
+    lib.ptbgpu_last_error.restype = ctypes.c_char_p
+    lib.ptbgpu_last_error.argtypes = [VP]
+    lib.ptbgpu_init.argtypes = [ctypes.POINTER(VP), I64, I64]
+    for name in ("ptbgpu_finalize", "ptbgpu_sync"):
+        getattr(lib, name).argtypes = [VP]
+    lib.ptbgpu_stream.argtypes = [VP, ctypes.POINTER(VP)]
+    lib.ptbgpu_filter_sym.argtypes = [VP, I64, VP, F64, VP]
+    lib.ptbgpu_filter_sym_dev.argtypes = [VP, I64, VP, F64, VP]
+    lib.ptbgpu_analyse_sym.argtypes = [VP, I64, VP, VP]
+    lib.ptbgpu_analyse_sym_dev.argtypes = [VP, I64, VP, VP]
+    sm_args = [VP, I64, I64, VP, VP, VP, I64, VP, VP, I64, I64, ctypes.POINTER(Opts),
+               ctypes.POINTER(Report)]
+    for name in ("ptbgpu_sm_density", "ptbgpu_sm_density_dev"):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = sm_args
+    for name in ("ptbgpu_get_nel", "ptbgpu_band_energy", "ptbgpu_idempotency"):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = [VP, I64, VP, VP, ctypes.POINTER(F64)]
+    lib._ptb_typed = True
+    return lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(VP)
+    return VP(a.data_ptr())   # torch tensor
+
+
+def _f64(a, n, name):
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError(f"{name} must be a contiguous float64 numpy array")
+    if a.size < n:
+        raise ValueError(f"{name} has {a.size} elements, needs {n}")
+    return a
+
+
+class Engine:
+    """One engine context (one HIP stream) on one device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib()
+        self.ctx = VP()
+        st = self.lib.ptbgpu_init(ctypes.byref(self.ctx), device, 0)
+        if st != 0:
+            raise PtbGpuError(st, self.lib.ptbgpu_strerror(st).decode())
+        self.device = device
+
+    def close(self):
+        if self.ctx:
+            self.lib.ptbgpu_finalize(self.ctx)
+            self.ctx = VP()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int):
+        if st != 0:
+            msg = self.lib.ptbgpu_last_error(self.ctx).decode() or \
+                self.lib.ptbgpu_strerror(st).decode()
+            raise PtbGpuError(st, msg)
+
+    def stream_handle(self) -> int:
+        s = VP()
+        self._check(self.lib.ptbgpu_stream(self.ctx, ctypes.byref(s)))
+        return s.value or 0
+
+    def sync(self):
+        self._check(self.lib.ptbgpu_sync(self.ctx))
+
+    # ---- filter / analyse -------------------------------------------------
+    def filter_matrix_sym(self, A: np.ndarray, n: int, filter_: float):
+        """In-place filter of a packed matrix; returns positive counts
+        (before, after) as analyse_matrix_sym prints them."""
+        _f64(A, n * (n + 1) // 2, "A")
+        c = np.zeros(2, dtype=np.int64)
+        self._check(self.lib.ptbgpu_filter_sym(self.ctx, n, _ptr(A), filter_, _ptr(c)))
+        return int(c[0]), int(c[1])
+
+    def filter_matrix_sym_dev(self, A, n: int, filter_: float):
+        c = np.zeros(2, dtype=np.int64)
+        self._check(self.lib.ptbgpu_filter_sym_dev(self.ctx, n, _ptr(A), filter_, _ptr(c)))
+        return int(c[0]), int(c[1])
+
+    def analyse_matrix_sym(self, A, n: int) -> dict:
+        c = np.zeros(1, dtype=np.int64)
+        if isinstance(A, np.ndarray):
+            _f64(A, n * (n + 1) // 2, "A")
+            self._check(self.lib.ptbgpu_analyse_sym(self.ctx, n, _ptr(A), _ptr(c)))
+        else:
+            self._check(self.lib.ptbgpu_analyse_sym_dev(self.ctx, n, _ptr(A), _ptr(c)))
+        tot = int(c[0])
+        return {"dimension": n, "nonzero": tot, "sparsity": tot / float(n) ** 2 if n else 0.0,
+                "average": tot // n if n else 0}
+
+    # ---- submatrix density build -----------------------------------------
+    def _opts(self, precision, strict, verbose, refine_steps):
+        return Opts(PRECISION[precision] if isinstance(precision, str) else int(precision),
+                    1 if strict else 0, 1 if verbose else 0, int(refine_steps))
+
+    def sm_stupid_simple(self, ndim, nelref, Hvec, Svec, Pvec, n, xyz, aoat,
+                         submatrix_columns, submatrix_mode, precision="fp64", strict=False,
+                         verbose=False, refine_steps=0) -> dict:
+        """Drop-in for sm_stupid_simple on host arrays; Pvec is overwritten."""
+        npk = ndim * (ndim + 1) // 2
+        _f64(Hvec, npk, "Hvec")
+        _f64(Svec, npk, "Svec")
+        _f64(Pvec, npk, "Pvec")
+        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(n, 3)
+        a = np.ascontiguousarray(aoat, dtype=np.int64)
+        if a.size != ndim:
+            raise ValueError("aoat must have ndim entries")
+        rep = Report()
+        opts = self._opts(precision, strict, verbose, refine_steps)
+        self._check(self.lib.ptbgpu_sm_density(
+            self.ctx, ndim, nelref, _ptr(Hvec), _ptr(Svec), _ptr(Pvec), n, _ptr(x), _ptr(a),
+            submatrix_columns, submatrix_mode, ctypes.byref(opts), ctypes.byref(rep)))
+        return rep.as_dict()
+
+    def sm_stupid_simple_dev(self, ndim, nelref, dH, dS, dP, n, xyz, aoat,
+                             submatrix_columns, submatrix_mode, precision="fp64",
+                             strict=False, verbose=False, refine_steps=0) -> dict:
+        """Same with device-resident packed H, S, P (torch CUDA tensors)."""
+        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(n, 3)
+        a = np.ascontiguousarray(aoat, dtype=np.int64)
+        rep = Report()
+        opts = self._opts(precision, strict, verbose, refine_steps)
+        self._check(self.lib.ptbgpu_sm_density_dev(
+            self.ctx, ndim, nelref, _ptr(dH), _ptr(dS), _ptr(dP), n, _ptr(x), _ptr(a),
+            submatrix_columns, submatrix_mode, ctypes.byref(opts), ctypes.byref(rep)))
+        return rep.as_dict()
+
+    # ---- metrics -----------------------------------------------------------
+    def _scalar(self, fn, ndim, A, B):
+        out = F64(0.0)
+        self._check(fn(self.ctx, ndim, _ptr(A), _ptr(B), ctypes.byref(out)))
+        return out.value
+
+    def get_nel(self, ndim, P, S) -> float:
+        return self._scalar(self.lib.ptbgpu_get_nel, ndim, P, S)
+
+    def band_structure(self, ndim, P, H) -> float:
+        return self._scalar(self.lib.ptbgpu_band_energy, ndim, P, H)
+
+    def idempotency_deviation(self, ndim, P, S) -> float:
+        return self._scalar(self.lib.ptbgpu_idempotency, ndim, P, S)
