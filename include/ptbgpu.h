@@ -113,6 +113,34 @@ int ptbgpu_filter_sym_dev(ptbgpu_ctx *ctx, int64_t ndim, double *dA, double filt
 int ptbgpu_analyse_sym(ptbgpu_ctx *ctx, int64_t ndim, const double *A, int64_t *npos);
 int ptbgpu_analyse_sym_dev(ptbgpu_ctx *ctx, int64_t ndim, const double *dA, int64_t *npos);
 
+/* ---- submatrix density build ---------------------------------------------
+ * Replaces sm_stupid_simple(ndim,nelref,Hvec,Svec,Pvec,n,xyz,aoat,
+ * submatrix_columns,submatrix_mode) (source/sm.f90:34-409).
+ *   xyz(3,n) bohr, aoat(ndim) 1-based atom of each AO (host arrays);
+ *   submatrix_columns: 1 single_columns, 2 atoms_columns, 3 full_matrix,
+ *                      4 kmeans_r_heuristic (source/purification_settings.f90:9-12);
+ *   submatrix_mode: 2 submatrix_sygv (1 = submatrix_sign -> PTBGPU_E_MODE, as
+ *                   the reference stops there).
+ * P is fully overwritten.  opts and rep may be NULL (defaults: FP64, compat). */
+int ptbgpu_sm_density(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const double *H,
+                      const double *S, double *P, int64_t n, const double *xyz,
+                      const int64_t *aoat, int64_t submatrix_columns,
+                      int64_t submatrix_mode, const ptbgpu_opts *opts,
+                      ptbgpu_report *rep);
+int ptbgpu_sm_density_dev(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const double *dH,
+                          const double *dS, double *dP, int64_t n, const double *xyz,
+                          const int64_t *aoat, int64_t submatrix_columns,
+                          int64_t submatrix_mode, const ptbgpu_opts *opts,
+                          ptbgpu_report *rep);
+
+/* Per-submatrix dimensions, eigenvalues and fe weights of the last build on
+ * this context (the quantities the reference prints per submatrix and sums in
+ * the mu bisection, source/sm.f90:298-310,340).  Arrays sized from the report:
+ * m[nsm], eig[sum_m], fe[sum_m]; eigenvalues per submatrix are in no particular
+ * order but eig[i] and fe[i] belong together.  Any pointer may be NULL. */
+int ptbgpu_sm_spectra(ptbgpu_ctx *ctx, int64_t nsm, int64_t sum_m, int64_t *m, double *eig,
+                      double *fe);
+
 /* ---- density checks (source/metrics.f90) ---------------------------------
  * get_nel_packed (:202-230) = tr(P S); get_band_structure_E_packed (:253-277)
  * = tr(P H); idempotent_packed (:316-360) deviation = |sum(0.5 P S P - P)|

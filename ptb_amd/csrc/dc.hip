@@ -1,0 +1,394 @@
+// Divide-and-conquer eigensolver for the symmetric tridiagonal T of every
+// submatrix (Cuppen's method with Gu-Eisenstat / Loewner eigenvectors; the
+// algorithm of LAPACK dstedc, which dsygvd uses).
+//
+// Tree: each submatrix's index range is halved recursively until a piece has at
+// most 64 rows.  A split at sp couples rows sp-1 and sp through beta = e[sp-1]:
+//   T = diag(T1 - |beta| e_last e_last^T, T2 - |beta| e_1 e_1^T) + |beta| u u^T,
+//   u = [e_last ; sgn(beta) e_1].
+// Leaves: implicit QL with Wilkinson-type shifts (tql2), one wavefront per leaf,
+// lane r owning row r of the leaf's eigenvector block in LDS.
+// Merge of (Q1,D1),(Q2,D2): z = [Q1(last,:); sgn Q2(first,:)] / sqrt 2,
+// rho = 2|beta|; sort D; deflate (small rho|z_i| or close pairs rotated together);
+// solve the secular equation 1 + rho sum w_i^2/(d_i - lam) = 0 by bisection on
+// lam - d_origin (origin = nearer pole) so that d_i - lam is formed accurately;
+// Loewner vector zhat; U(i,j) = zhat_i/(d_i - lam_j) normalised; the merged
+// eigenvectors are Qnd U (a GEMM launched for all nodes of a tree level; this
+// kernel writes its descriptor).  Eigenvalues are kept in the same column order
+// as the eigenvectors (not sorted).
+#include <cfloat>
+
+#include "sm_internal.hpp"
+
+namespace {
+
+constexpr int kLeaf = 64;
+constexpr int kMT = 256;
+constexpr double kEps = DBL_EPSILON;
+
+__global__ __launch_bounds__(64) void k_dc_leaf(const SubDesc *__restrict__ desc,
+                                                const DcNode *__restrict__ leaves,
+                                                double *__restrict__ d_all,
+                                                const double *__restrict__ e_all,
+                                                double *__restrict__ Za) {
+    __shared__ double z[kLeaf][kLeaf + 1];
+    __shared__ double dl[kLeaf], el[kLeaf];
+    const DcNode nd = leaves[blockIdx.x];
+    const SubDesc ds = desc[nd.s];
+    const int n = nd.hi - nd.lo, lane = threadIdx.x;
+    const double *e = e_all + ds.mapoff;
+    double *d = d_all + ds.mapoff;
+    if (lane < n) {
+        double dv = d[nd.lo + lane];
+        if (lane == 0 && nd.lo > 0) dv -= fabs(e[nd.lo - 1]);
+        if (lane == n - 1 && nd.hi < ds.m) dv -= fabs(e[nd.hi - 1]);
+        dl[lane] = dv;
+        el[lane] = (lane < n - 1) ? e[nd.lo + lane] : 0.0;
+    }
+    for (int c = 0; c < kLeaf; ++c) z[lane][c] = (lane == c) ? 1.0 : 0.0;
+    __syncthreads();
+    // every lane runs the scalar recurrence on the shared d/e (same values);
+    // lane r applies each rotation to row r of z
+    for (int l = 0; l < n; ++l) {
+        for (int iter = 0; iter < 60; ++iter) {
+            int mm = l;
+            for (; mm < n - 1; ++mm) {
+                const double dd = fabs(dl[mm]) + fabs(dl[mm + 1]);
+                if (fabs(el[mm]) <= kEps * dd) break;
+            }
+            if (mm == l) break;
+            double g = (dl[l + 1] - dl[l]) / (2.0 * el[l]);
+            double r = hypot(g, 1.0);
+            g = dl[mm] - dl[l] + el[l] / (g + copysign(r, g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            bool under = false;
+            for (int i = mm - 1; i >= l; --i) {
+                const double f = s * el[i], b = c * el[i];
+                r = hypot(f, g);
+                __syncthreads();
+                if (lane == 0) el[i + 1] = r;
+                if (r == 0.0) {
+                    if (lane == 0) {
+                        dl[i + 1] -= p;
+                        el[mm] = 0.0;
+                    }
+                    __syncthreads();
+                    under = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = dl[i + 1] - p;
+                r = (dl[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                if (lane == 0) dl[i + 1] = g + p;
+                g = c * r - b;
+                if (lane < n) {
+                    const double zi = z[lane][i], zi1 = z[lane][i + 1];
+                    z[lane][i + 1] = s * zi + c * zi1;
+                    z[lane][i] = c * zi - s * zi1;
+                }
+                __syncthreads();
+            }
+            if (under) continue;
+            __syncthreads();
+            if (lane == 0) {
+                dl[l] -= p;
+                el[l] = g;
+                el[mm] = 0.0;
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    if (lane < n) d[nd.lo + lane] = dl[lane];
+    double *Z = Za + ds.off;
+    for (int c = 0; c < n; ++c)
+        if (lane < n) Z[(int64_t)(nd.lo + c) * ds.ld + nd.lo + lane] = z[lane][c];
+}
+
+// bitonic sort of (key, idx) pairs in LDS, ascending, ties by idx
+__device__ void bitonic_sort(double *key, int *idx, int npow) {
+    for (int size = 2; size <= npow; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (int t = threadIdx.x; t < npow / 2; t += kMT) {
+                const int lo = 2 * stride * (t / stride) + (t % stride);
+                const int hi = lo + stride;
+                const bool up = ((lo & size) == 0);
+                const double ka = key[lo], kb = key[hi];
+                const int ia = idx[lo], ib = idx[hi];
+                const bool gt = (ka > kb) || (ka == kb && ia > ib);
+                if (gt == up) {
+                    key[lo] = kb;
+                    key[hi] = ka;
+                    idx[lo] = ib;
+                    idx[hi] = ia;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ double block_max(double v, double *red) {
+    v = wave_max(v);
+    const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double t = red[0];
+    for (int w = 1; w < kMT / PTB_WAVE; ++w) t = fmax(t, red[w]);
+    __syncthreads();
+    return t;
+}
+
+// One merge node per workgroup.  Scratch: W node block receives the sorted,
+// rotated, compacted eigenvector columns [nondeflated | deflated]; X node block
+// receives U (K x K); rot holds the deflation rotations (4 doubles each).
+__global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ desc,
+                                                  const DcNode *__restrict__ nodes, int npow,
+                                                  const double *__restrict__ e_all,
+                                                  double *__restrict__ D_all,
+                                                  double *__restrict__ Za, double *__restrict__ Wa,
+                                                  double *__restrict__ Xa,
+                                                  double *__restrict__ rot_all,
+                                                  GemmDesc *__restrict__ gd) {
+    extern __shared__ double shm[];
+    const DcNode nd = nodes[blockIdx.x];
+    const SubDesc ds = desc[nd.s];
+    const int n = nd.hi - nd.lo, n1 = nd.sp - nd.lo, ld = ds.ld;
+    double *Dk = shm;               // npow sorted keys
+    double *z = Dk + npow;          // n (sorted order after step 3)
+    double *dl = z + npow;          // K non-deflated poles
+    double *w = dl + npow;          // K weights
+    double *tau = w + npow;         // K root offsets
+    double *zh = tau + npow;        // K Loewner vector
+    int *ix = reinterpret_cast<int *>(zh + npow);  // npow sort permutation
+    int *slot = ix + npow;          // n output slot of sorted position
+    int *org = slot + npow;         // K origin pole of each root
+    __shared__ double red[kMT / PTB_WAVE];
+    __shared__ int sK, sNr;
+    __shared__ double srho;
+
+    const double beta = e_all[ds.mapoff + nd.sp - 1];
+    const double rho = 2.0 * fabs(beta);
+    const double sgn = beta >= 0.0 ? 1.0 : -1.0;
+    double *Z = Za + ds.off, *W = Wa + ds.off, *X = Xa + ds.off;
+    double *D = D_all + ds.mapoff;
+    double *rot = rot_all + 4 * (int64_t)(ds.mapoff + nd.lo);
+    GemmDesc *g = gd + blockIdx.x;
+
+    if (rho == 0.0) {  // decoupled: children are already eigenpairs of T
+        if (threadIdx.x == 0) {
+            g->M = 0;
+            g->N = 0;
+            g->K = 0;
+        }
+        return;
+    }
+    const double rs2 = 1.0 / sqrt(2.0);
+    double *zraw = tau;  // temporary: raw z by column before sorting
+    for (int i = threadIdx.x; i < npow; i += kMT) {
+        if (i < n) {
+            Dk[i] = D[nd.lo + i];
+            ix[i] = i;
+            const int col = nd.lo + i;
+            const double zv = (i < n1) ? Z[(int64_t)col * ld + nd.sp - 1]
+                                       : sgn * Z[(int64_t)col * ld + nd.sp];
+            zraw[i] = zv * rs2;
+        } else {
+            Dk[i] = DBL_MAX;
+            ix[i] = i;
+        }
+    }
+    bitonic_sort(Dk, ix, npow);
+    for (int j = threadIdx.x; j < n; j += kMT) z[j] = zraw[ix[j]];
+    __syncthreads();
+    double mx = 0.0;
+    for (int j = threadIdx.x; j < n; j += kMT) mx = fmax(mx, fmax(fabs(Dk[j]), fabs(z[j])));
+    const double tol = 8.0 * kEps * block_max(mx, red);
+
+    // deflation scan (sequential; rotations recorded, applied to columns later)
+    if (threadIdx.x == 0) {
+        int K = 0, nd_ = 0, nr = 0, pj = -1;
+        // slot: >= 0 final slot for non-deflated (assigned after counting)
+        // first pass marks: slot[j] = -1 deflated, -2 non-deflated
+        for (int j = 0; j < n; ++j) {
+            if (rho * fabs(z[j]) <= tol) {
+                slot[j] = -1;
+                continue;
+            }
+            if (pj >= 0) {
+                double s = z[pj], c = z[j];
+                const double tv = hypot(c, s);
+                const double t = Dk[j] - Dk[pj];
+                c /= tv;
+                s = -s / tv;
+                if (fabs(t * c * s) <= tol) {
+                    z[j] = tv;
+                    z[pj] = 0.0;
+                    rot[4 * nr + 0] = (double)pj;
+                    rot[4 * nr + 1] = (double)j;
+                    rot[4 * nr + 2] = c;
+                    rot[4 * nr + 3] = s;
+                    ++nr;
+                    const double t1 = Dk[pj] * c * c + Dk[j] * s * s;
+                    Dk[j] = Dk[pj] * s * s + Dk[j] * c * c;
+                    Dk[pj] = t1;
+                    slot[pj] = -1;
+                    pj = j;
+                    continue;
+                }
+                slot[pj] = -2;
+            }
+            pj = j;
+        }
+        if (pj >= 0) slot[pj] = -2;
+        for (int j = 0; j < n; ++j)
+            if (slot[j] == -2) {
+                dl[K] = Dk[j];
+                w[K] = z[j];
+                slot[j] = K++;
+            }
+        nd_ = K;
+        for (int j = 0; j < n; ++j)
+            if (slot[j] == -1) slot[j] = nd_++;
+        sK = K;
+        sNr = nr;
+        srho = rho;
+    }
+    __syncthreads();
+    const int K = sK, nr = sNr;
+    // sorted, compacted eigenvector columns into W: W(:, slot[j]) = Z(:, lo+ix[j])
+    for (int l = threadIdx.x; l < n * n; l += kMT) {
+        const int r = l % n, j = l / n;
+        W[(int64_t)(nd.lo + slot[j]) * ld + nd.lo + r] = Z[(int64_t)(nd.lo + ix[j]) * ld + nd.lo + r];
+    }
+    __syncthreads();
+    // deflation rotations, in order, row-parallel
+    for (int r = threadIdx.x; r < n; r += kMT) {
+        for (int q = 0; q < nr; ++q) {
+            const int cp = nd.lo + slot[(int)rot[4 * q]], cj = nd.lo + slot[(int)rot[4 * q + 1]];
+            const double c = rot[4 * q + 2], s = rot[4 * q + 3];
+            double *pp = W + (int64_t)cp * ld + nd.lo + r, *pjj = W + (int64_t)cj * ld + nd.lo + r;
+            const double qp = *pp, qj = *pjj;
+            *pp = c * qp + s * qj;
+            *pjj = c * qj - s * qp;
+        }
+    }
+    __syncthreads();
+    // deflated columns and eigenvalues go straight to their final place
+    for (int l = threadIdx.x; l < n * (n - K); l += kMT) {
+        const int r = l % n, c = K + l / n;
+        Z[(int64_t)(nd.lo + c) * ld + nd.lo + r] = W[(int64_t)(nd.lo + c) * ld + nd.lo + r];
+    }
+    for (int j = threadIdx.x; j < n; j += kMT)
+        if (slot[j] >= K) D[nd.lo + slot[j]] = Dk[j];
+    __syncthreads();
+    // secular roots, one thread per root
+    for (int j = threadIdx.x; j < K; j += kMT) {
+        int o;
+        double lo, hi;
+        if (j < K - 1) {
+            const double mid = 0.5 * (dl[j + 1] - dl[j]);
+            double f = 1.0;
+            for (int i = 0; i < K; ++i) f += rho * w[i] * w[i] / ((dl[i] - dl[j]) - mid);
+            if (f >= 0.0) {
+                o = j;
+                lo = 0.0;
+                hi = mid;
+            } else {
+                o = j + 1;
+                lo = -mid;
+                hi = 0.0;
+            }
+        } else {
+            o = j;
+            double s2 = 0.0;
+            for (int i = 0; i < K; ++i) s2 += w[i] * w[i];
+            lo = 0.0;
+            hi = rho * s2;
+        }
+        const double dorg = dl[o];
+        for (int it = 0; it < 400; ++it) {
+            const double tm = 0.5 * (lo + hi);
+            if (tm == lo || tm == hi) break;
+            double f = 1.0;
+            for (int i = 0; i < K; ++i) f += rho * w[i] * w[i] / ((dl[i] - dorg) - tm);
+            if (f < 0.0) lo = tm;
+            else hi = tm;
+        }
+        org[j] = o;
+        tau[j] = 0.5 * (lo + hi);
+    }
+    __syncthreads();
+    // Loewner vector: rho zhat_i^2 = (lam_i - d_i) prod_{j != i} (lam_j - d_i)/(d_j - d_i)
+    for (int i = threadIdx.x; i < K; i += kMT) {
+        const double di = dl[i];
+        double prod = -((di - dl[org[i]]) - tau[i]);
+        for (int j = 0; j < K; ++j) {
+            if (j == i) continue;
+            const double dlt = (di - dl[org[j]]) - tau[j];  // d_i - lam_j
+            prod *= (-dlt) / (dl[j] - di);
+        }
+        zh[i] = copysign(sqrt(fmax(prod, 0.0) / rho), w[i]);
+    }
+    __syncthreads();
+    // U(:, j) = zhat ./ (d - lam_j), normalised; eigenvalues lam_j
+    for (int j = threadIdx.x; j < K; j += kMT) {
+        const double dorg = dl[org[j]], tj = tau[j];
+        double nrm = 0.0;
+        for (int i = 0; i < K; ++i) {
+            const double u = zh[i] / ((dl[i] - dorg) - tj);
+            nrm += u * u;
+        }
+        const double inv = 1.0 / sqrt(nrm);
+        double *Uc = X + (int64_t)(nd.lo + j) * ld + nd.lo;
+        for (int i = 0; i < K; ++i) Uc[i] = inv * zh[i] / ((dl[i] - dorg) - tj);
+        D[nd.lo + j] = dorg + tj;
+    }
+    if (threadIdx.x == 0) {
+        g->A = W + (int64_t)nd.lo * ld + nd.lo;
+        g->B = X + (int64_t)nd.lo * ld + nd.lo;
+        g->C = Z + (int64_t)nd.lo * ld + nd.lo;
+        g->M = n;
+        g->N = K;
+        g->K = K;
+        g->lda = ld;
+        g->ldb = ld;
+        g->ldc = ld;
+        g->alpha = 1.0;
+        g->beta = 0.0;
+    }
+    (void)srho;
+}
+
+}  // namespace
+
+namespace ptb {
+
+int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int nleaves,
+              double *d, const double *e, double *Z) {
+    if (nleaves <= 0) return PTBGPU_OK;
+    hipLaunchKernelGGL(k_dc_leaf, dim3(nleaves), dim3(64), 0, ctx->stream, ddesc, dleaves, d, e, Z);
+    PTB_CHECK_LAUNCH(ctx);
+    return PTBGPU_OK;
+}
+
+int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, int count,
+                   int max_n, const double *e, double *D, double *Z, double *W, double *X,
+                   double *rot, GemmDesc *dgemm) {
+    if (count <= 0) return PTBGPU_OK;
+    int npow = 1;
+    while (npow < max_n) npow <<= 1;
+    const size_t shmem = (size_t)npow * (6 * sizeof(double) + 3 * sizeof(int));
+    if (shmem > 150 * 1024)
+        return fail(ctx, PTBGPU_E_ARG, "merge of %d rows exceeds the LDS budget", max_n);
+    hipLaunchKernelGGL(k_dc_merge, dim3(count), dim3(kMT), shmem, ctx->stream, ddesc, dnodes, npow,
+                       e, D, Z, W, X, rot, dgemm);
+    PTB_CHECK_LAUNCH(ctx);
+    return gemm_vbatch(ctx, false, false, dgemm, count, max_n, max_n);
+}
+
+}  // namespace ptb

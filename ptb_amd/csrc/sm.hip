@@ -1,0 +1,406 @@
+// Orchestration of the submatrix density build: the C ABI entry points that
+// replace sm_stupid_simple (source/sm.f90:34-409).
+//
+// Phases (reference timer labels in brackets):
+//   [submatrix combination]  column groups from aoat (host, closed forms)
+//   [submatrix stats]        pattern bitmap, per-group row union, map (device)
+//   [submatrix construction] gather of dense Hsm, Ssm
+//   [submatrix eigendecomp]  Cholesky Ssm = L L^T, K = L^-1 Hsm L^-T,
+//                            Householder tridiagonalisation, divide & conquer,
+//                            back-transformation Y = Q Z
+//   [submatrix fe]           fe weights, U = L^-T Y
+//   [mu iter]                bisection of the chemical potential
+//   [submatrix final iter]   owned-column density, scatter into packed P
+// Every phase is batched over all submatrices: one launch (or one per tree level
+// / reflector block) covers the whole batch.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "sm_internal.hpp"
+
+namespace {
+
+constexpr int kLeafMax = 64;
+constexpr int kNB = 32;  // reflector block of the back-transformation
+
+struct Timer {
+    hipEvent_t ev[PTBGPU_T_NSLOT + 1];
+    bool ok = false;
+    void init(hipStream_t s) {
+        ok = true;
+        for (auto &e : ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+        (void)s;
+    }
+    void mark(int slot, hipStream_t s) {
+        if (ok) (void)hipEventRecord(ev[slot], s);
+    }
+    float between(int a, int b) {
+        float ms = 0.f;
+        if (ok) (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
+        return ms;
+    }
+    ~Timer() {
+        if (ok)
+            for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+// column groups, reference modes 1-3 (source/sm.f90:92-128)
+int column_groups(int64_t mode, int64_t ndim, const int64_t *aoat, std::vector<int> &gptr,
+                  std::vector<int> &gcols) {
+    gptr.clear();
+    gcols.resize(ndim);
+    for (int64_t i = 0; i < ndim; ++i) gcols[i] = (int)i;
+    gptr.push_back(0);
+    if (mode == 1) {
+        for (int64_t i = 1; i <= ndim; ++i) gptr.push_back((int)i);
+    } else if (mode == 2) {
+        for (int64_t i = 1; i < ndim; ++i)
+            if (aoat[i] != aoat[i - 1]) gptr.push_back((int)i);
+        gptr.push_back((int)ndim);
+    } else if (mode == 3) {
+        gptr.push_back((int)ndim);
+    } else {
+        return PTBGPU_E_MODE;
+    }
+    return PTBGPU_OK;
+}
+
+void build_tree(int s, int lo, int hi, std::vector<DcNode> &leaves,
+                std::vector<std::vector<DcNode>> &levels, int &height) {
+    if (hi - lo <= kLeafMax) {
+        leaves.push_back({s, lo, hi, hi});
+        height = 0;
+        return;
+    }
+    const int sp = lo + (hi - lo) / 2;
+    int h1, h2;
+    build_tree(s, lo, sp, leaves, levels, h1);
+    build_tree(s, sp, hi, leaves, levels, h2);
+    height = 1 + std::max(h1, h2);
+    if ((int)levels.size() < height) levels.resize(height);
+    levels[height - 1].push_back({s, lo, sp, hi});
+}
+
+template <typename T>
+int upload(ptbgpu_ctx *ctx, const char *name, const std::vector<T> &v, T **out) {
+    *out = static_cast<T *>(ptb::ws(ctx, name, sizeof(T) * std::max<size_t>(v.size(), 1)));
+    if (!*out) return PTBGPU_E_NOMEM;
+    if (!v.empty())
+        PTB_HIP(ctx, hipMemcpyAsync(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice,
+                                    ctx->stream));
+    return PTBGPU_OK;
+}
+
+int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const double *dH,
+                      const double *dS, double *dP, const int64_t *aoat, int64_t cols,
+                      int64_t mode, const ptbgpu_opts &opts, ptbgpu_report &rep) {
+    hipStream_t st = ctx->stream;
+    Timer tm;
+    tm.init(st);
+    tm.mark(0, st);
+    if (mode != 2)
+        return ptb::fail(ctx, PTBGPU_E_MODE, "submatrix_mode %lld not implemented "
+                         "(the reference stops, source/sm.f90:249-251,312-314)", (long long)mode);
+    if (opts.precision != PTBGPU_FP64 && opts.precision != PTBGPU_MIXED)
+        return ptb::fail(ctx, PTBGPU_E_ARG, "unknown precision %lld", (long long)opts.precision);
+    // ---- combination
+    std::vector<int> gptr, gcols;
+    if (column_groups(cols, ndim, aoat, gptr, gcols) != PTBGPU_OK)
+        return ptb::fail(ctx, PTBGPU_E_MODE, "submatrix_columns %lld not implemented", (long long)cols);
+    const int nsm = (int)gptr.size() - 1;
+    int *dgptr, *dgcols;
+    PTB_TRY(upload(ctx, "sm_gptr", gptr, &dgptr));
+    PTB_TRY(upload(ctx, "sm_gcols", gcols, &dgcols));
+    tm.mark(1, st);
+    // ---- stats: pattern, unions, dimensions
+    const int64_t words = (ndim + 31) / 32;
+    auto *bits = static_cast<uint32_t *>(ptb::ws(ctx, "sm_bits", sizeof(uint32_t) * ndim * words));
+    auto *uni = static_cast<uint32_t *>(ptb::ws(ctx, "sm_union", sizeof(uint32_t) * nsm * words));
+    auto *dcount = static_cast<int *>(ptb::ws(ctx, "sm_count", sizeof(int) * nsm));
+    if (!bits || !uni || !dcount) return PTBGPU_E_NOMEM;
+    PTB_TRY(ptb::pattern_bits(ctx, dH, dS, ndim, words, bits));
+    PTB_TRY(ptb::group_union(ctx, bits, words, dgptr, dgcols, nsm, uni, dcount));
+    std::vector<int> mcount(nsm);
+    PTB_HIP(ctx, hipMemcpyAsync(mcount.data(), dcount, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
+    PTB_HIP(ctx, hipStreamSynchronize(st));
+    std::vector<SubDesc> desc(nsm);
+    int64_t arena = 0, rows = 0;
+    int max_m = 0, max_own = 0;
+    double sum_m3 = 0.0;
+    for (int s = 0; s < nsm; ++s) {
+        SubDesc &d = desc[s];
+        d.m = mcount[s];
+        d.ld = (d.m + 3) & ~3;
+        d.off = arena;
+        d.mapoff = (int)rows;
+        d.gbeg = gptr[s];
+        d.gend = gptr[s + 1];
+        d.flags = 0;
+        arena += (int64_t)d.ld * d.m;
+        rows += d.m;
+        max_m = std::max(max_m, d.m);
+        max_own = std::max(max_own, d.gend - d.gbeg);
+        sum_m3 += (double)d.m * d.m * d.m;
+    }
+    if (max_m > 2048)
+        return ptb::fail(ctx, PTBGPU_E_ARG, "submatrix dimension %d exceeds 2048", max_m);
+    SubDesc *ddesc;
+    PTB_TRY(upload(ctx, "sm_desc", desc, &ddesc));
+    auto *dmap = static_cast<int *>(ptb::ws(ctx, "sm_map", sizeof(int) * rows));
+    auto *doloc = static_cast<int *>(ptb::ws(ctx, "sm_oloc", sizeof(int) * ndim));
+    if (!dmap || !doloc) return PTBGPU_E_NOMEM;
+    PTB_TRY(ptb::group_map(ctx, uni, words, ddesc, nsm, dmap, dgptr, dgcols, doloc));
+    tm.mark(2, st);
+    // ---- construction
+    const size_t abytes = sizeof(double) * (size_t)arena;
+    auto *A = static_cast<double *>(ptb::ws(ctx, "sm_A", abytes));
+    auto *B = static_cast<double *>(ptb::ws(ctx, "sm_B", abytes));
+    auto *Z = static_cast<double *>(ptb::ws(ctx, "sm_Z", abytes));
+    auto *W = static_cast<double *>(ptb::ws(ctx, "sm_W", abytes));
+    auto *X = static_cast<double *>(ptb::ws(ctx, "sm_X", abytes));
+    auto *vec = static_cast<double *>(ptb::ws(ctx, "sm_vec", sizeof(double) * rows * 9 + 4096));
+    auto *flags = static_cast<int *>(ptb::ws(ctx, "sm_flags", sizeof(int) * nsm));
+    auto *Lown = static_cast<double *>(ptb::ws(ctx, "sm_lown", sizeof(double) * ndim * max_m));
+    auto *SY = static_cast<double *>(ptb::ws(ctx, "sm_sy", sizeof(double) * ndim * max_m));
+    auto *Tb = static_cast<double *>(ptb::ws(ctx, "sm_T", sizeof(double) * nsm * kNB * kNB));
+    if (!A || !B || !Z || !W || !X || !vec || !flags || !Lown || !SY || !Tb) return PTBGPU_E_NOMEM;
+    double *dvec = vec, *evec = vec + rows, *tau = vec + 2 * rows, *fe = vec + 3 * rows;
+    double *rot = vec + 4 * rows;  // 4 * rows
+    double *bis = vec + 8 * rows;  // 4 + 200
+    PTB_TRY(ptb::gather(ctx, dH, dS, ddesc, nsm, max_m, dmap, A, B));
+    tm.mark(3, st);
+    // ---- eigendecomposition
+    PTB_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int) * nsm, st));
+    PTB_TRY(ptb::potrf_batch(ctx, ddesc, nsm, B, flags));
+    PTB_TRY(ptb::trsm_batch(ctx, ddesc, nsm, max_m, B, A, false));         // A = L^-1 H
+    PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, A, Z, false));  // Z = (L^-1 H)^T
+    PTB_TRY(ptb::trsm_batch(ctx, ddesc, nsm, max_m, B, Z, false));         // Z = K
+    PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, Z, A, true));   // A = sym(K)
+    PTB_TRY(ptb::sytrd_batch(ctx, ddesc, nsm, A, W, dvec, evec, tau, max_m));
+    PTB_TRY(ptb::zero_upper_batch(ctx, ddesc, nsm, max_m, A));              // A = V
+    // divide and conquer
+    std::vector<DcNode> leaves;
+    std::vector<std::vector<DcNode>> levels;
+    for (int s = 0; s < nsm; ++s) {
+        int h;
+        build_tree(s, 0, desc[s].m, leaves, levels, h);
+    }
+    PTB_HIP(ctx, hipMemsetAsync(Z, 0, abytes, st));
+    DcNode *dleaves;
+    PTB_TRY(upload(ctx, "sm_leaves", leaves, &dleaves));
+    PTB_TRY(ptb::dc_leaves(ctx, ddesc, dleaves, (int)leaves.size(), dvec, evec, Z));
+    for (size_t lv = 0; lv < levels.size(); ++lv) {
+        const auto &nodes = levels[lv];
+        int max_n = 0;
+        for (const auto &nd : nodes) max_n = std::max(max_n, nd.hi - nd.lo);
+        std::string nm = "sm_level" + std::to_string(lv);
+        DcNode *dnodes;
+        PTB_TRY(upload(ctx, nm.c_str(), nodes, &dnodes));
+        std::string gm = "sm_lgemm" + std::to_string(lv);
+        auto *dg = static_cast<GemmDesc *>(ptb::ws(ctx, gm.c_str(), sizeof(GemmDesc) * nodes.size()));
+        if (!dg) return PTBGPU_E_NOMEM;
+        PTB_TRY(ptb::dc_merge_level(ctx, ddesc, dnodes, (int)nodes.size(), max_n, evec, dvec, Z,
+                                    W, X, rot, dg));
+    }
+    // back-transformation Y = Q Z, reflector blocks from the last to the first
+    int max_blocks = 0;
+    for (int s = 0; s < nsm; ++s) {
+        const int nr = desc[s].m - 2;
+        max_blocks = std::max(max_blocks, nr > 0 ? (nr + kNB - 1) / kNB : 0);
+    }
+    if (max_blocks > 0) {
+        std::vector<GemmDesc> g1, g2, g3;
+        std::vector<int> cnt(max_blocks, 0), start(max_blocks + 1, 0);
+        g1.reserve((size_t)nsm * max_blocks);
+        for (int q = 0; q < max_blocks; ++q) {
+            start[q] = (int)g1.size();
+            for (int s = 0; s < nsm; ++s) {
+                const SubDesc &d = desc[s];
+                const int nr = d.m - 2;
+                const int nblk = nr > 0 ? (nr + kNB - 1) / kNB : 0;
+                const int b = nblk - 1 - q;
+                if (b < 0) continue;
+                const int b0 = b * kNB, kb = std::min(kNB, nr - b0);
+                const int rowsv = d.m - b0 - 1;
+                double *V = A + d.off + (int64_t)b0 * d.ld + b0 + 1;
+                double *Y = Z + d.off + b0 + 1;
+                // W1 = V_b^T Y and W2 = T W1 are kb x m panels inside this
+                // submatrix's W and X blocks (leading dimension <= ld)
+                const int lw = std::min(kNB, d.ld);
+                double *W1 = W + d.off, *W2 = X + d.off;
+                const double *T = Tb + (int64_t)s * kNB * kNB;
+                g1.push_back({V, Y, W1, kb, d.m, rowsv, d.ld, d.ld, lw, 1.0, 0.0});
+                g2.push_back({T, W1, W2, kb, d.m, kb, kNB, lw, lw, 1.0, 0.0});
+                g3.push_back({V, W2, Y, rowsv, d.m, kb, d.ld, lw, d.ld, -1.0, 1.0});
+            }
+        }
+        start[max_blocks] = (int)g1.size();
+        GemmDesc *dg1, *dg2, *dg3;
+        PTB_TRY(upload(ctx, "sm_bt1", g1, &dg1));
+        PTB_TRY(upload(ctx, "sm_bt2", g2, &dg2));
+        PTB_TRY(upload(ctx, "sm_bt3", g3, &dg3));
+        for (int q = 0; q < max_blocks; ++q) {
+            const int c = start[q + 1] - start[q];
+            if (c == 0) continue;
+            PTB_TRY(ptb::larft_batch(ctx, ddesc, nsm, A, tau, q, kNB, Tb));
+            PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dg1 + start[q], c, kNB, max_m));
+            PTB_TRY(ptb::gemm_vbatch(ctx, false, false, dg2 + start[q], c, kNB, max_m));
+            PTB_TRY(ptb::gemm_vbatch(ctx, false, false, dg3 + start[q], c, max_m, max_m));
+        }
+    }
+    tm.mark(4, st);
+    // ---- fe and U = L^-T Y
+    std::vector<int> hflags(nsm);
+    PTB_HIP(ctx, hipMemcpyAsync(hflags.data(), flags, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
+    PTB_TRY(ptb::fe_owned_rows(ctx, ddesc, nsm, max_m, max_own, doloc, B, Z, Lown, SY));
+    PTB_TRY(ptb::trsm_batch(ctx, ddesc, nsm, max_m, B, Z, true));
+    PTB_HIP(ctx, hipStreamSynchronize(st));
+    int64_t not_spd = 0;
+    for (int s = 0; s < nsm; ++s)
+        if (hflags[s] & 1) {
+            ++not_spd;
+            desc[s].flags |= 1;
+        }
+    if (not_spd) {
+        if (opts.strict)
+            return ptb::fail(ctx, PTBGPU_E_NOT_SPD, "%lld overlap submatrices are not positive definite",
+                             (long long)not_spd);
+        PTB_HIP(ctx, hipMemcpyAsync(ddesc, desc.data(), sizeof(SubDesc) * nsm, hipMemcpyHostToDevice, st));
+    }
+    PTB_TRY(ptb::fe_finish(ctx, ddesc, nsm, max_m, doloc, SY, Z, fe));
+    tm.mark(5, st);
+    // ---- mu bisection
+    PTB_TRY(ptb::bisect_mu(ctx, dvec, fe, rows, nelref, bis));
+    tm.mark(6, st);
+    // ---- final density
+    PTB_HIP(ctx, hipMemsetAsync(dP, 0, sizeof(double) * (size_t)(ndim * (ndim + 1) / 2), st));
+    PTB_TRY(ptb::dmat_scatter(ctx, ddesc, nsm, max_m, max_own, dgcols, doloc, dmap, dvec, Z, bis, dP));
+    tm.mark(7, st);
+    std::vector<double> hb(4 + 200);
+    PTB_HIP(ctx, hipMemcpyAsync(hb.data(), bis, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, st));
+    PTB_HIP(ctx, hipStreamSynchronize(st));
+    tm.mark(8, st);
+    PTB_HIP(ctx, hipStreamSynchronize(st));
+    rep.mu = hb[0];
+    rep.nel = hb[1];
+    rep.muiter = (int64_t)hb[2];
+    rep.nsm = nsm;
+    rep.sum_m = rows;
+    rep.max_m = max_m;
+    rep.sum_m3 = sum_m3;
+    rep.not_spd = not_spd;
+    const int slots[] = {PTBGPU_T_COMBINATION, PTBGPU_T_STATS, PTBGPU_T_CONSTRUCTION, PTBGPU_T_EIGEN,
+                         PTBGPU_T_FE, PTBGPU_T_MUITER, PTBGPU_T_FINAL};
+    for (int i = 0; i < 7; ++i) rep.t_phase[slots[i]] += 1e-3 * tm.between(i, i + 1);
+    if (opts.verbose) {
+        printf(" Stupid simple implementation of the submatrix method: %lld %lld\n", (long long)cols,
+               (long long)mode);
+        printf(" number of submatrices= %d\n average sm dim= %lld\n", nsm, (long long)(rows / nsm));
+        for (int i = 0; i < rep.muiter; ++i)
+            printf(" muiter %d mu %.17g %.17g of %lld\n", i + 1, hb[4 + 2 * i], hb[5 + 2 * i],
+                   (long long)nelref);
+        printf(" iteration of chemical potential converged\n");
+        fflush(stdout);
+    }
+    return PTBGPU_OK;
+}
+
+int check_sm_args(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const void *H, const void *S,
+                  const void *P, int64_t n, const int64_t *aoat) {
+    if (!ctx) return PTBGPU_E_ARG;
+    if (ndim < 1 || ndim > (int64_t)1 << 30 || !H || !S || !P || !aoat || n < 1 || nelref < 0)
+        return ptb::fail(ctx, PTBGPU_E_ARG, "bad arguments to sm_density");
+    for (int64_t i = 0; i < ndim; ++i)
+        if (aoat[i] < 1 || aoat[i] > n)
+            return ptb::fail(ctx, PTBGPU_E_ARG, "aoat(%lld) = %lld outside 1..n", (long long)(i + 1),
+                             (long long)aoat[i]);
+    return PTBGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ptbgpu_sm_density_dev(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const double *dH,
+                          const double *dS, double *dP, int64_t n, const double *xyz,
+                          const int64_t *aoat, int64_t submatrix_columns,
+                          int64_t submatrix_mode, const ptbgpu_opts *opts, ptbgpu_report *rep) {
+    PTB_TRY(check_sm_args(ctx, ndim, nelref, dH, dS, dP, n, aoat));
+    (void)xyz;
+    PTB_HIP(ctx, hipSetDevice(ctx->device));
+    ptbgpu_opts o = opts ? *opts : ptbgpu_opts{PTBGPU_FP64, 0, 0, 0};
+    ptbgpu_report r;
+    memset(&r, 0, sizeof(r));
+    auto t0 = std::chrono::steady_clock::now();
+    int st = sm_density_device(ctx, ndim, nelref, dH, dS, dP, aoat, submatrix_columns,
+                               submatrix_mode, o, r);
+    r.t_phase[PTBGPU_T_TOTAL] =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rep) *rep = r;
+    return st;
+}
+
+int ptbgpu_sm_spectra(ptbgpu_ctx *ctx, int64_t nsm, int64_t sum_m, int64_t *m, double *eig,
+                      double *fe) {
+    if (!ctx || nsm < 1 || sum_m < 1) return PTBGPU_E_ARG;
+    auto it = ctx->bufs.find("sm_desc");
+    auto iv = ctx->bufs.find("sm_vec");
+    if (it == ctx->bufs.end() || iv == ctx->bufs.end())
+        return ptb::fail(ctx, PTBGPU_E_ARG, "no density build on this context yet");
+    PTB_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<SubDesc> desc(nsm);
+    PTB_HIP(ctx, hipMemcpyAsync(desc.data(), it->second.ptr, sizeof(SubDesc) * nsm,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int64_t rows = 0;
+    for (int64_t s = 0; s < nsm; ++s) rows += desc[s].m;
+    if (rows != sum_m) return ptb::fail(ctx, PTBGPU_E_ARG, "sum_m does not match the last build");
+    const double *vec = static_cast<const double *>(iv->second.ptr);
+    if (m)
+        for (int64_t s = 0; s < nsm; ++s) m[s] = desc[s].m;
+    if (eig)
+        PTB_HIP(ctx, hipMemcpyAsync(eig, vec, sizeof(double) * rows, hipMemcpyDeviceToHost, ctx->stream));
+    if (fe)
+        PTB_HIP(ctx, hipMemcpyAsync(fe, vec + 3 * rows, sizeof(double) * rows, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PTBGPU_OK;
+}
+
+int ptbgpu_sm_density(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const double *H,
+                      const double *S, double *P, int64_t n, const double *xyz,
+                      const int64_t *aoat, int64_t submatrix_columns, int64_t submatrix_mode,
+                      const ptbgpu_opts *opts, ptbgpu_report *rep) {
+    PTB_TRY(check_sm_args(ctx, ndim, nelref, H, S, P, n, aoat));
+    PTB_HIP(ctx, hipSetDevice(ctx->device));
+    auto t0 = std::chrono::steady_clock::now();
+    const size_t bytes = sizeof(double) * (size_t)(ndim * (ndim + 1) / 2);
+    auto *dH = static_cast<double *>(ptb::ws(ctx, "io_h", bytes));
+    auto *dS = static_cast<double *>(ptb::ws(ctx, "io_s", bytes));
+    auto *dP = static_cast<double *>(ptb::ws(ctx, "io_p", bytes));
+    if (!dH || !dS || !dP) return PTBGPU_E_NOMEM;
+    PTB_HIP(ctx, hipMemcpyAsync(dH, H, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PTB_HIP(ctx, hipMemcpyAsync(dS, S, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double t_up = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    ptbgpu_report r;
+    int st = ptbgpu_sm_density_dev(ctx, ndim, nelref, dH, dS, dP, n, xyz, aoat, submatrix_columns,
+                                   submatrix_mode, opts, &r);
+    if (st == PTBGPU_OK) {
+        auto t1 = std::chrono::steady_clock::now();
+        PTB_HIP(ctx, hipMemcpyAsync(P, dP, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        r.t_phase[PTBGPU_T_DEALLOC] =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+    }
+    r.t_phase[PTBGPU_T_ALLOC] = t_up;
+    r.t_phase[PTBGPU_T_TOTAL] =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rep) *rep = r;
+    return st;
+}
+
+}  // extern "C"

@@ -76,6 +76,8 @@ def _lib():
     for name in ("ptbgpu_sm_density", "ptbgpu_sm_density_dev"):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = sm_args
+    if hasattr(lib, "ptbgpu_sm_spectra"):
+        lib.ptbgpu_sm_spectra.argtypes = [VP, I64, I64, VP, VP, VP]
     for name in ("ptbgpu_get_nel", "ptbgpu_band_energy", "ptbgpu_idempotency"):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [VP, I64, VP, VP, ctypes.POINTER(F64)]
@@ -202,6 +204,15 @@ class Engine:
             self.ctx, ndim, nelref, _ptr(dH), _ptr(dS), _ptr(dP), n, _ptr(x), _ptr(a),
             submatrix_columns, submatrix_mode, ctypes.byref(opts), ctypes.byref(rep)))
         return rep.as_dict()
+
+    def sm_spectra(self, nsm: int, sum_m: int):
+        """(m per submatrix, eigenvalues, fe) of the last build, split per submatrix."""
+        m = np.zeros(nsm, dtype=np.int64)
+        e = np.zeros(sum_m)
+        fe = np.zeros(sum_m)
+        self._check(self.lib.ptbgpu_sm_spectra(self.ctx, nsm, sum_m, _ptr(m), _ptr(e), _ptr(fe)))
+        cuts = np.cumsum(m)[:-1]
+        return m, np.split(e, cuts), np.split(fe, cuts)
 
     # ---- metrics -----------------------------------------------------------
     def _scalar(self, fn, ndim, A, B):
