@@ -98,6 +98,16 @@ int ptbgpu_sync(ptbgpu_ctx *ctx);
 /* The context's HIP stream (hipStream_t), for callers that time or order work. */
 int ptbgpu_stream(ptbgpu_ctx *ctx, void **stream);
 
+/* Kernel timing (no reference counterpart; used by the benchmark).  With
+ * profiling on, the engine brackets each launch of its hot kernels with HIP
+ * events on the context's stream; the query syncs and returns the summed
+ * milliseconds and the number of bracketed launch groups of one kernel name
+ * (filter, pattern, gather, potrf, trsm, sytrd, dc_leaf, dc_merge, gemm,
+ * larft, fe_rows, fe_finish, bisect, dmat_scatter, ...).  Enabling or
+ * disabling clears the records. */
+int ptbgpu_profile(ptbgpu_ctx *ctx, int64_t enable);
+int ptbgpu_profile_query(ptbgpu_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
+
 /* ---- filter / analyse ----------------------------------------------------
  * Replaces filter_matrix_sym (source/pgtb.f90:1532-1548): A(k) = 0 where
  * abs(A(k)) < filter (strict), in place over the packed matrix.  counts, if

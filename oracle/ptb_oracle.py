@@ -656,22 +656,14 @@ def sign_iterative(pur: PurSettings, guess: np.ndarray, X: np.ndarray, thrs: Thr
     return 2.0 * sign_to_density_matrix(pur, Xs, X), it
 
 
-@dataclass
-class SignDiagState:
-    """The SAVEd eigenpairs of sign_diagonalization_ (source/purification.f90:671-696):
-    computed on the first call of a process and never recomputed."""
-    vectors: np.ndarray = None
-    values: np.ndarray = None
-
-
 def sign_diagonalization(pur: PurSettings, chempot: float, guess: np.ndarray, X: np.ndarray,
-                         thrs: Thresholds, state: SignDiagState):
+                         thrs: Thresholds, state=None):
     """Internal chemical-potential bisection on the guess's eigenvalues, then
-    sign = Q sgn Q^T (source/purification.f90:650-782)."""
+    sign = Q sgn Q^T (source/purification.f90:650-782).  The SAVEd flag
+    first_time is initialised .true. and never cleared (:671,692), so the guess
+    is diagonalised on every call."""
     n = guess.shape[0]
-    if state.values is None:
-        state.values, state.vectors = sla.eigh(guess, lower=False, driver="evd")
-    Q, lam = state.vectors, state.values
+    lam, Q = sla.eigh(guess, lower=False, driver="evd")
     nel = float(pur.nel)
     incr = pur.chempot_increment
     lower = upper = 0.0
@@ -702,7 +694,6 @@ def sign_diagonalization(pur: PurSettings, chempot: float, guess: np.ndarray, X:
 def purification_search(pur: PurSettings, H, S, X, thrs: Thresholds, state=None):
     """Outer chemical-potential search (source/purification.f90:232-353).
     Returns (P dense, chempot, nel_calc, cycles)."""
-    state = state or SignDiagState()
     chempot = pur.chempot_guess
     incr = pur.chempot_increment
     nel = pur.nel

@@ -59,6 +59,8 @@ void *hws(ptbgpu_ctx *ctx, const char *name, size_t bytes) {
 
 }  // namespace ptb
 
+static void prof_clear(ptbgpu_ctx *ctx);
+
 extern "C" {
 
 const char *ptbgpu_strerror(int64_t status) {
@@ -102,6 +104,7 @@ int ptbgpu_finalize(ptbgpu_ctx *ctx) {
     if (!ctx) return PTBGPU_E_ARG;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    prof_clear(ctx);
     for (auto &kv : ctx->bufs)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     for (auto &kv : ctx->host_bufs)
@@ -120,6 +123,43 @@ int ptbgpu_sync(ptbgpu_ctx *ctx) {
 int ptbgpu_stream(ptbgpu_ctx *ctx, void **stream) {
     if (!ctx || !stream) return PTBGPU_E_ARG;
     *stream = (void *)ctx->stream;
+    return PTBGPU_OK;
+}
+
+static void prof_clear(ptbgpu_ctx *ctx) {
+    for (auto &kv : ctx->prof)
+        for (auto &pe : kv.second.ev) {
+            (void)hipEventDestroy(pe.first);
+            (void)hipEventDestroy(pe.second);
+        }
+    ctx->prof.clear();
+}
+
+int ptbgpu_profile(ptbgpu_ctx *ctx, int64_t enable) {
+    if (!ctx) return PTBGPU_E_ARG;
+    PTB_HIP(ctx, hipSetDevice(ctx->device));
+    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    prof_clear(ctx);
+    ctx->profiling = enable != 0;
+    return PTBGPU_OK;
+}
+
+int ptbgpu_profile_query(ptbgpu_ctx *ctx, const char *name, double *total_ms, int64_t *launches) {
+    if (!ctx || !name) return PTBGPU_E_ARG;
+    PTB_HIP(ctx, hipSetDevice(ctx->device));
+    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double t = 0.0;
+    int64_t c = 0;
+    auto it = ctx->prof.find(name);
+    if (it != ctx->prof.end())
+        for (auto &pe : it->second.ev) {
+            float ms = 0.f;
+            PTB_HIP(ctx, hipEventElapsedTime(&ms, pe.first, pe.second));
+            t += ms;
+            ++c;
+        }
+    if (total_ms) *total_ms = t;
+    if (launches) *launches = c;
     return PTBGPU_OK;
 }
 

@@ -19,10 +19,16 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+struct ProfRecord {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+
 struct ptbgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string last_error;
+    bool profiling = false;
+    std::map<std::string, ProfRecord> prof;  // kernel name -> bracketing events
     std::map<std::string, DevBuf> bufs;      // named device workspace, grows on demand
     std::map<std::string, DevBuf> host_bufs; // pinned host staging
     int num_cus = 0;
@@ -37,6 +43,25 @@ int fail(ptbgpu_ctx *ctx, int status, const char *fmt, ...);
 void *ws(ptbgpu_ctx *ctx, const char *name, size_t bytes);
 // Named pinned host buffer of at least `bytes`.
 void *hws(ptbgpu_ctx *ctx, const char *name, size_t bytes);
+
+// Brackets the launches issued during its lifetime with events on the
+// context's stream when profiling is enabled (ptbgpu_profile).
+struct ProfScope {
+    ptbgpu_ctx *ctx;
+    hipEvent_t b = nullptr, e = nullptr;
+    ProfRecord *rec = nullptr;
+    ProfScope(ptbgpu_ctx *c, const char *name) : ctx(c) {
+        if (!ctx->profiling) return;
+        if (hipEventCreate(&b) != hipSuccess || hipEventCreate(&e) != hipSuccess) return;
+        rec = &ctx->prof[name];
+        (void)hipEventRecord(b, ctx->stream);
+    }
+    ~ProfScope() {
+        if (!rec) return;
+        (void)hipEventRecord(e, ctx->stream);
+        rec->ev.emplace_back(b, e);
+    }
+};
 
 }  // namespace ptb
 

@@ -310,17 +310,33 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
             lo = 0.0;
             hi = rho * s2;
         }
+        // safeguarded Newton on f(t) = 1 + rho sum w_i^2 / (delta_i - t), f increasing:
+        // a Newton point inside the bracket is taken, otherwise the midpoint;
+        // every evaluation also shrinks the bracket, so the iteration ends with
+        // the bracket at adjacent doubles or f == 0
         const double dorg = dl[o];
+        double t = 0.5 * (lo + hi);
         for (int it = 0; it < 400; ++it) {
-            const double tm = 0.5 * (lo + hi);
-            if (tm == lo || tm == hi) break;
-            double f = 1.0;
-            for (int i = 0; i < K; ++i) f += rho * w[i] * w[i] / ((dl[i] - dorg) - tm);
-            if (f < 0.0) lo = tm;
-            else hi = tm;
+            double f = 1.0, fp = 0.0;
+            for (int i = 0; i < K; ++i) {
+                const double r = 1.0 / ((dl[i] - dorg) - t);
+                const double q = rho * w[i] * w[i] * r;
+                f += q;
+                fp += q * r;
+            }
+            if (f == 0.0) {
+                lo = hi = t;
+                break;
+            }
+            if (f < 0.0) lo = t;
+            else hi = t;
+            double tn = t - f / fp;
+            if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+            if (tn == t || tn == lo || tn == hi) break;
+            t = tn;
         }
         org[j] = o;
-        tau[j] = 0.5 * (lo + hi);
+        tau[j] = (lo == hi) ? lo : t;
     }
     __syncthreads();
     // Loewner vector: rho zhat_i^2 = (lam_i - d_i) prod_{j != i} (lam_j - d_i)/(d_j - d_i)
@@ -370,6 +386,7 @@ namespace ptb {
 
 int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int nleaves,
               double *d, const double *e, double *Z) {
+    ProfScope prof_(ctx, "dc_leaf");
     if (nleaves <= 0) return PTBGPU_OK;
     hipLaunchKernelGGL(k_dc_leaf, dim3(nleaves), dim3(64), 0, ctx->stream, ddesc, dleaves, d, e, Z);
     PTB_CHECK_LAUNCH(ctx);
@@ -385,9 +402,12 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
     const size_t shmem = (size_t)npow * (6 * sizeof(double) + 3 * sizeof(int));
     if (shmem > 150 * 1024)
         return fail(ctx, PTBGPU_E_ARG, "merge of %d rows exceeds the LDS budget", max_n);
-    hipLaunchKernelGGL(k_dc_merge, dim3(count), dim3(kMT), shmem, ctx->stream, ddesc, dnodes, npow,
-                       e, D, Z, W, X, rot, dgemm);
-    PTB_CHECK_LAUNCH(ctx);
+    {
+        ProfScope prof_(ctx, "dc_merge");
+        hipLaunchKernelGGL(k_dc_merge, dim3(count), dim3(kMT), shmem, ctx->stream, ddesc, dnodes,
+                           npow, e, D, Z, W, X, rot, dgemm);
+        PTB_CHECK_LAUNCH(ctx);
+    }
     return gemm_vbatch(ctx, false, false, dgemm, count, max_n, max_n);
 }
 

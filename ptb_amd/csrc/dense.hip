@@ -19,20 +19,28 @@ namespace {
 
 constexpr int kT = 256;
 
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+// 64x64 output tile per 256-thread workgroup; each wavefront owns a 32x32
+// quarter as 2x2 blocks of v_mfma_f64_16x16x4.  MFMA operand layout: lane l
+// supplies A(l%16, l/16) of a 16x4 slice and B(l/16, l%16) of a 4x16 slice;
+// result value r of lane l is D(4r + l/16, l%16).
 template <bool TA, bool TB>
 __global__ __launch_bounds__(kT) void k_gemm(const GemmDesc *__restrict__ descs) {
     const GemmDesc g = descs[blockIdx.y];
     const int tilesM = (g.M + 63) / 64, tilesN = (g.N + 63) / 64;
     if (g.M <= 0 || g.N <= 0 || (int)blockIdx.x >= tilesM * tilesN) return;
     const int m0 = (blockIdx.x % tilesM) * 64, n0 = (blockIdx.x / tilesM) * 64;
-    __shared__ double As[16][65];
-    __shared__ double Bs[16][65];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc[4][4];
+    __shared__ double As[16][68];
+    __shared__ double Bs[16][68];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    d4_t acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+        for (int j = 0; j < 2; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < g.K; k0 += 16) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -54,124 +62,103 @@ __global__ __launch_bounds__(kT) void k_gemm(const GemmDesc *__restrict__ descs)
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            double a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][ty + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        for (int ks = 0; ks < 4; ++ks) {
+            const int kk = ks * 4 + lk;
+            const double a0 = As[kk][wm + lr], a1 = As[kk][wm + 16 + lr];
+            const double b0 = Bs[kk][wn + lr], b1 = Bs[kk][wn + 16 + lr];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int gn = n0 + ty + 16 * j;
-        if (gn >= g.N) continue;
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int gm = m0 + tx + 16 * i;
-            if (gm >= g.M) continue;
-            double *c = g.C + (int64_t)gn * g.ldc + gm;
-            *c = g.beta == 0.0 ? g.alpha * acc[i][j] : g.alpha * acc[i][j] + g.beta * *c;
+        for (int j = 0; j < 2; ++j) {
+            const int gn = n0 + wn + 16 * j + lr;
+            if (gn >= g.N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm + 16 * i + 4 * r + lk;
+                if (gm >= g.M) continue;
+                double *c = g.C + (int64_t)gn * g.ldc + gm;
+                const double v = acc[i][j][r];
+                *c = g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * *c;
+            }
         }
-    }
 }
 
-__global__ __launch_bounds__(kT) void k_potrf(const SubDesc *__restrict__ desc,
-                                              double *__restrict__ Aa, int *__restrict__ flags) {
+// One 32-column panel of the lower Cholesky factor of every submatrix: factor
+// the diagonal block in LDS, invert it, and form the panel below as
+// X = P L_kk^-T (rows independent).  The trailing update that follows is a
+// batched GEMM.  A non-positive pivot sets flags bit 0 (dsygvd: INFO > 0).
+__global__ __launch_bounds__(kT) void k_potrf_panel(const SubDesc *__restrict__ desc,
+                                                    double *__restrict__ Aa,
+                                                    int *__restrict__ flags, int panel) {
     const SubDesc d = desc[blockIdx.x];
-    double *a = Aa + d.off;
     const int m = d.m, ld = d.ld;
-    __shared__ double Lk[32][33];
-    __shared__ double Pi[32][33], Pj[32][33];
+    const int k0 = panel * 32;
+    if (k0 >= m) return;
+    double *a = Aa + d.off;
+    const int kb = min(32, m - k0);
+    __shared__ double Lk[32][33], Li[32][33];
     __shared__ int bad;
     if (threadIdx.x == 0) bad = 0;
-    for (int k0 = 0; k0 < m; k0 += 32) {
-        const int kb = min(32, m - k0);
+    for (int l = threadIdx.x; l < 1024; l += kT) {
+        const int r = l & 31, c = l >> 5;
+        double v;
+        if (r < kb && c < kb) v = r >= c ? a[(int64_t)(k0 + c) * ld + k0 + r] : 0.0;
+        else v = (r == c) ? 1.0 : 0.0;  // identity padding
+        Lk[r][c] = v;
+        Li[r][c] = 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+        if (threadIdx.x == 0) {
+            double p = Lk[j][j];
+            if (!(p > 0.0)) {
+                bad = 1;
+                p = 1.0;
+            }
+            Lk[j][j] = sqrt(p);
+        }
+        __syncthreads();
+        if ((int)threadIdx.x > j && (int)threadIdx.x < kb) Lk[threadIdx.x][j] /= Lk[j][j];
         __syncthreads();
         for (int l = threadIdx.x; l < 1024; l += kT) {
             const int r = l & 31, c = l >> 5;
-            double v;
-            if (r < kb && c < kb) v = a[(int64_t)(k0 + c) * ld + k0 + r];
-            else v = (r == c) ? 1.0 : 0.0;  // pad with the identity
-            Lk[r][c] = v;
+            if (c > j && r >= c && r < kb) Lk[r][c] -= Lk[r][j] * Lk[c][j];
         }
         __syncthreads();
-        for (int j = 0; j < kb; ++j) {
-            if (threadIdx.x == 0) {
-                double p = Lk[j][j];
-                if (!(p > 0.0)) {
-                    bad = 1;
-                    p = 1.0;
-                }
-                Lk[j][j] = sqrt(p);
-            }
-            __syncthreads();
-            if (threadIdx.x > j && threadIdx.x < kb) Lk[threadIdx.x][j] /= Lk[j][j];
-            __syncthreads();
-            for (int l = threadIdx.x; l < 1024; l += kT) {
-                const int r = l & 31, c = l >> 5;
-                if (c > j && r >= c && r < kb) Lk[r][c] -= Lk[r][j] * Lk[c][j];
-            }
-            __syncthreads();
-        }
-        for (int l = threadIdx.x; l < 1024; l += kT) {
-            const int r = l & 31, c = l >> 5;
-            if (r < kb && c < kb && r >= c) a[(int64_t)(k0 + c) * ld + k0 + r] = Lk[r][c];
-        }
-        const int k1 = k0 + kb;
-        // panel rows below: x <- x L_kk^-T  (forward substitution, padded to 32)
-        for (int i = k1 + threadIdx.x; i < m; i += kT) {
-            double x[32];
-#pragma unroll
-            for (int c = 0; c < 32; ++c) x[c] = c < kb ? a[(int64_t)(k0 + c) * ld + i] : 0.0;
-#pragma unroll
-            for (int c = 0; c < 32; ++c) {
-                double s = x[c];
-#pragma unroll
-                for (int t = 0; t < c; ++t) s -= x[t] * Lk[c][t];
-                x[c] = s / Lk[c][c];
-            }
-#pragma unroll
-            for (int c = 0; c < 32; ++c)
-                if (c < kb) a[(int64_t)(k0 + c) * ld + i] = x[c];
-        }
-        __syncthreads();
-        // trailing lower triangle: A_ij -= sum_t P_it P_jt over 32x32 tiles
-        const int rem = m - k1;
-        if (rem <= 0) continue;
-        const int nt = (rem + 31) / 32;
-        const int ntile = nt * (nt + 1) / 2;
-        for (int t = 0; t < ntile; ++t) {
-            int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-            while (ti * (ti + 1) / 2 > t) --ti;
-            while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-            const int tj = t - ti * (ti + 1) / 2;  // tj <= ti
-            const int i0 = k1 + ti * 32, j0 = k1 + tj * 32;
-            for (int l = threadIdx.x; l < 1024; l += kT) {
-                const int r = l & 31, c = l >> 5;
-                Pi[r][c] = (i0 + r < m && c < kb) ? a[(int64_t)(k0 + c) * ld + i0 + r] : 0.0;
-                Pj[r][c] = (j0 + r < m && c < kb) ? a[(int64_t)(k0 + c) * ld + j0 + r] : 0.0;
-            }
-            __syncthreads();
-            for (int l = threadIdx.x; l < 1024; l += kT) {
-                const int r = l & 31, c = l >> 5;
-                const int gi = i0 + r, gj = j0 + c;
-                if (gi < m && gj < m && gi >= gj) {
-                    double s = 0.0;
-#pragma unroll 8
-                    for (int q = 0; q < 32; ++q) s += Pi[r][q] * Pj[c][q];
-                    a[(int64_t)gj * ld + gi] -= s;
-                }
-            }
-            __syncthreads();
+    }
+    for (int l = threadIdx.x; l < 1024; l += kT) {
+        const int r = l & 31, c = l >> 5;
+        if (r < kb && c < kb && r >= c) a[(int64_t)(k0 + c) * ld + k0 + r] = Lk[r][c];
+    }
+    // Li = Lk^-1, column c by thread c (forward substitution)
+    if (threadIdx.x < 32) {
+        const int c = threadIdx.x;
+        for (int r = c; r < 32; ++r) {
+            double sacc = (r == c) ? 1.0 : 0.0;
+            for (int t = c; t < r; ++t) sacc -= Lk[r][t] * Li[t][c];
+            Li[r][c] = sacc / Lk[r][r];
         }
     }
     __syncthreads();
+    // X(i, c) = sum_{t<=c} P(i, t) Li(c, t)
+    for (int i = k0 + kb + threadIdx.x; i < m; i += kT) {
+        double p[32];
+#pragma unroll
+        for (int t = 0; t < 32; ++t) p[t] = t < kb ? a[(int64_t)(k0 + t) * ld + i] : 0.0;
+        for (int c = 0; c < kb; ++c) {
+            double sacc = 0.0;
+#pragma unroll
+            for (int t = 0; t < 32; ++t) sacc += p[t] * Li[c][t];
+            a[(int64_t)(k0 + c) * ld + i] = sacc;
+        }
+    }
     if (threadIdx.x == 0 && bad) flags[blockIdx.x] |= 1;
 }
 
@@ -353,12 +340,51 @@ __global__ __launch_bounds__(kT) void k_larft(const SubDesc *__restrict__ desc,
     }
 }
 
+// Inverse of every 64x64 diagonal block of each lower-triangular L (identity
+// padding past m).  grid: (block, submatrix); Dinv block b of submatrix s at
+// dinv + s_doff + b*4096, column-major, leading dimension 64.
+__global__ __launch_bounds__(64) void k_diag_inv(const SubDesc *__restrict__ desc,
+                                                 const int64_t *__restrict__ doff,
+                                                 const double *__restrict__ La,
+                                                 double *__restrict__ dinv) {
+    const SubDesc d = desc[blockIdx.y];
+    const int b = blockIdx.x;
+    const int i0 = b * 64;
+    if (i0 >= d.m) return;
+    __shared__ double T[64][65];
+    const double *L = La + d.off;
+    const int c = threadIdx.x;
+    for (int r = 0; r < 64; ++r) {
+        double v;
+        if (i0 + r < d.m && i0 + c < d.m) v = r >= c ? L[(int64_t)(i0 + c) * d.ld + i0 + r] : 0.0;
+        else v = (r == c) ? 1.0 : 0.0;
+        T[r][c] = v;
+    }
+    __syncthreads();
+    // column c of T^-1 by forward substitution
+    double x[64];
+#pragma unroll
+    for (int r = 0; r < 64; ++r) x[r] = 0.0;
+#pragma unroll
+    for (int r = 0; r < 64; ++r) {
+        if (r < c) continue;
+        double sacc = (r == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int t = 0; t < r; ++t) sacc -= T[r][t] * x[t];
+        x[r] = sacc / T[r][r];
+    }
+    double *out = dinv + doff[blockIdx.y] + (int64_t)b * 4096 + (int64_t)c * 64;
+#pragma unroll
+    for (int r = 0; r < 64; ++r) out[r] = x[r];
+}
+
 }  // namespace
 
 namespace ptb {
 
 int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int count, int max_m,
                 int max_n) {
+    ProfScope prof_(ctx, "gemm");
     if (count <= 0 || max_m <= 0 || max_n <= 0) return PTBGPU_OK;
     const int tiles = ((max_m + 63) / 64) * ((max_n + 63) / 64);
     for (int c0 = 0; c0 < count; c0 += 65535) {
@@ -374,14 +400,115 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
     return PTBGPU_OK;
 }
 
-int potrf_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, double *A, int *flags) {
-    hipLaunchKernelGGL(k_potrf, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, A, flags);
+// X = L^-1 B (trans=false) or L^-T B (trans=true), all m columns, blocked by
+// 64 rows: per block row one update GEMM (in place on B) and one product with
+// the inverted diagonal block (into X).  B is overwritten.
+int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+              const double *L, double *B, double *X, bool trans, const char *tag) {
+    ProfScope prof_(ctx, "trsm");
+    const int nsm = (int)desc.size();
+    std::vector<int64_t> doff(nsm);
+    int64_t tot = 0;
+    int max_nb = 0, max_m = 0;
+    for (int s = 0; s < nsm; ++s) {
+        doff[s] = tot;
+        const int nb = (desc[s].m + 63) / 64;
+        tot += (int64_t)nb * 4096;
+        max_nb = std::max(max_nb, nb);
+        max_m = std::max(max_m, desc[s].m);
+    }
+    auto *dinv = static_cast<double *>(ws(ctx, "trsm_dinv", sizeof(double) * tot));
+    auto *ddoff = static_cast<int64_t *>(ws(ctx, "trsm_doff", sizeof(int64_t) * nsm));
+    if (!dinv || !ddoff) return PTBGPU_E_NOMEM;
+    PTB_HIP(ctx, hipMemcpyAsync(ddoff, doff.data(), sizeof(int64_t) * nsm, hipMemcpyHostToDevice,
+                                ctx->stream));
+    hipLaunchKernelGGL(k_diag_inv, dim3(max_nb, nsm), dim3(64), 0, ctx->stream, ddesc, ddoff, L,
+                       dinv);
     PTB_CHECK_LAUNCH(ctx);
+    std::vector<GemmDesc> g1, g2;
+    std::vector<int> s1(max_nb + 1), s2(max_nb + 1);
+    for (int q = 0; q < max_nb; ++q) {
+        s1[q] = (int)g1.size();
+        s2[q] = (int)g2.size();
+        for (int s = 0; s < nsm; ++s) {
+            const SubDesc &d = desc[s];
+            const int nb = (d.m + 63) / 64;
+            if (q >= nb) continue;
+            const int ib = trans ? nb - 1 - q : q;
+            const int i0 = ib * 64, ibs = std::min(64, d.m - i0);
+            const double *Lb = L + d.off;
+            double *Bb = B + d.off, *Xb = X + d.off;
+            const double *Di = dinv + doff[s] + (int64_t)ib * 4096;
+            if (!trans) {
+                if (i0 > 0)
+                    g1.push_back({Lb + i0, Xb, Bb + i0, ibs, d.m, i0, d.ld, d.ld, d.ld, -1.0, 1.0});
+                g2.push_back({Di, Bb + i0, Xb + i0, ibs, d.m, ibs, 64, d.ld, d.ld, 1.0, 0.0});
+            } else {
+                const int rest = d.m - i0 - ibs;
+                if (rest > 0)
+                    g1.push_back({Lb + (int64_t)i0 * d.ld + i0 + ibs, Xb + i0 + ibs, Bb + i0, ibs, d.m,
+                                  rest, d.ld, d.ld, d.ld, -1.0, 1.0});
+                g2.push_back({Di, Bb + i0, Xb + i0, ibs, d.m, ibs, 64, d.ld, d.ld, 1.0, 0.0});
+            }
+        }
+    }
+    s1[max_nb] = (int)g1.size();
+    s2[max_nb] = (int)g2.size();
+    std::string n1 = std::string("trsm_g1_") + tag, n2 = std::string("trsm_g2_") + tag;
+    auto *dg1 = static_cast<GemmDesc *>(ws(ctx, n1.c_str(), sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1)));
+    auto *dg2 = static_cast<GemmDesc *>(ws(ctx, n2.c_str(), sizeof(GemmDesc) * std::max<size_t>(g2.size(), 1)));
+    if (!dg1 || !dg2) return PTBGPU_E_NOMEM;
+    if (!g1.empty())
+        PTB_HIP(ctx, hipMemcpyAsync(dg1, g1.data(), sizeof(GemmDesc) * g1.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    PTB_HIP(ctx, hipMemcpyAsync(dg2, g2.data(), sizeof(GemmDesc) * g2.size(), hipMemcpyHostToDevice,
+                                ctx->stream));
+    for (int q = 0; q < max_nb; ++q) {
+        const int c1 = s1[q + 1] - s1[q], c2 = s2[q + 1] - s2[q];
+        if (c1 > 0) PTB_TRY(gemm_vbatch(ctx, trans, false, dg1 + s1[q], c1, 64, max_m));
+        if (c2 > 0) PTB_TRY(gemm_vbatch(ctx, trans, false, dg2 + s2[q], c2, 64, max_m));
+    }
+    return PTBGPU_OK;
+}
+
+int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                double *A, int *flags) {
+    ProfScope prof_(ctx, "potrf");
+    const int nsm = (int)desc.size();
+    int max_m = 0;
+    for (const auto &x : desc) max_m = std::max(max_m, x.m);
+    const int npanel = (max_m + 31) / 32;
+    std::vector<GemmDesc> g;
+    std::vector<int> start(npanel + 1, 0);
+    for (int p = 0; p < npanel; ++p) {
+        start[p] = (int)g.size();
+        for (int si = 0; si < nsm; ++si) {
+            const SubDesc &x = desc[si];
+            const int k0 = p * 32, s0 = k0 + 32, n = x.m - s0;
+            if (n <= 0) continue;
+            double *a = A + x.off;
+            double *P = a + (int64_t)k0 * x.ld + s0;
+            g.push_back({P, P, a + (int64_t)s0 * x.ld + s0, n, n, 32, x.ld, x.ld, x.ld, -1.0, 1.0});
+        }
+    }
+    start[npanel] = (int)g.size();
+    auto *dg = static_cast<GemmDesc *>(ws(ctx, "potrf_g", sizeof(GemmDesc) * std::max<size_t>(g.size(), 1)));
+    if (!dg) return PTBGPU_E_NOMEM;
+    if (!g.empty())
+        PTB_HIP(ctx, hipMemcpyAsync(dg, g.data(), sizeof(GemmDesc) * g.size(), hipMemcpyHostToDevice,
+                                    ctx->stream));
+    for (int p = 0; p < npanel; ++p) {
+        hipLaunchKernelGGL(k_potrf_panel, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, A, flags, p);
+        PTB_CHECK_LAUNCH(ctx);
+        const int c = start[p + 1] - start[p];
+        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dg + start[p], c, max_m, max_m));
+    }
     return PTBGPU_OK;
 }
 
 int trsm_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *L,
                double *B, bool trans) {
+    ProfScope prof_(ctx, "trsm");
     dim3 grid((max_m + 31) / 32, nsm);
     if (trans) hipLaunchKernelGGL(k_trsm<true>, grid, dim3(kT), 0, ctx->stream, ddesc, L, B);
     else hipLaunchKernelGGL(k_trsm<false>, grid, dim3(kT), 0, ctx->stream, ddesc, L, B);
@@ -391,6 +518,7 @@ int trsm_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const 
 
 int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m,
                         const double *in, double *out, bool symmetrize) {
+    ProfScope prof_(ctx, "transpose");
     const int nt = (max_m + 31) / 32;
     hipLaunchKernelGGL(k_transpose, dim3(nt * nt, nsm), dim3(kT), 0, ctx->stream, ddesc, in, out,
                        symmetrize ? 1 : 0);
@@ -399,6 +527,7 @@ int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_
 }
 
 int zero_upper_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, double *A) {
+    ProfScope prof_(ctx, "zero_upper");
     hipLaunchKernelGGL(k_zero_upper, dim3(max_m, nsm), dim3(kT), 0, ctx->stream, ddesc, A);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
@@ -406,6 +535,7 @@ int zero_upper_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, 
 
 int larft_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, const double *V,
                 const double *tau, int q, int nb, double *T) {
+    ProfScope prof_(ctx, "larft");
     hipLaunchKernelGGL(k_larft, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, V, tau, q, nb, T);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;

@@ -193,6 +193,7 @@ namespace ptb {
 
 int fe_owned_rows(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, int max_own,
                   const int *doloc, const double *L, const double *Y, double *Lown, double *SY) {
+    ProfScope prof_(ctx, "fe_rows");
     hipLaunchKernelGGL(k_own_rows, dim3(max_own, nsm), dim3(kT), 0, ctx->stream, ddesc, doloc, L,
                        max_m, Lown);
     PTB_CHECK_LAUNCH(ctx);
@@ -205,6 +206,7 @@ int fe_owned_rows(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, int
 
 int fe_finish(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const int *doloc,
               const double *SY, const double *U, double *fe) {
+    ProfScope prof_(ctx, "fe_finish");
     hipLaunchKernelGGL(k_fe_finish, dim3((max_m + kT - 1) / kT, nsm), dim3(kT), 0, ctx->stream,
                        ddesc, doloc, SY, max_m, U, fe);
     PTB_CHECK_LAUNCH(ctx);
@@ -213,6 +215,7 @@ int fe_finish(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const i
 
 int bisect_mu(ptbgpu_ctx *ctx, const double *e, const double *fe, int64_t total, int64_t nelref,
               double *dout) {
+    ProfScope prof_(ctx, "bisect");
     hipLaunchKernelGGL(k_bisect, dim3(1), dim3(kBT), 0, ctx->stream, e, fe, total, (double)nelref,
                        dout);
     PTB_CHECK_LAUNCH(ctx);
@@ -222,6 +225,7 @@ int bisect_mu(ptbgpu_ctx *ctx, const double *e, const double *fe, int64_t total,
 int dmat_scatter(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, int max_own,
                  const int *dgcols, const int *doloc, const int *dmap, const double *eig,
                  const double *U, const double *dmu, double *P) {
+    ProfScope prof_(ctx, "dmat_scatter");
     const size_t shmem = sizeof(double) * (size_t)kG * max_m + sizeof(int) * (size_t)max_m;
     if (shmem > 150 * 1024) return fail(ctx, PTBGPU_E_ARG, "submatrix too large for dmat (%d)", max_m);
     hipLaunchKernelGGL(k_dmat_scatter, dim3(nsm, (max_own + kG - 1) / kG), dim3(kT), shmem,

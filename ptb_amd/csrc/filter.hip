@@ -115,6 +115,7 @@ namespace ptb {
 
 int filter_count(ptbgpu_ctx *ctx, double *dA, int64_t ndim, double filter, bool do_filter,
                  int64_t counts[2]) {
+    ProfScope prof_(ctx, "filter");
     const int64_t len = ndim * (ndim + 1) / 2;
     auto *cnt = static_cast<unsigned long long *>(ws(ctx, "filter_cnt", 4 * sizeof(unsigned long long)));
     if (!cnt) return PTBGPU_E_NOMEM;

@@ -177,6 +177,7 @@ namespace ptb {
 
 int pattern_bits(ptbgpu_ctx *ctx, const double *dH, const double *dS, int64_t ndim,
                  int64_t words, uint32_t *bits) {
+    ProfScope prof_(ctx, "pattern");
     PTB_HIP(ctx, hipMemsetAsync(bits, 0, sizeof(uint32_t) * (size_t)(ndim * words), ctx->stream));
     int blocks = (int)std::min<int64_t>(ndim, (int64_t)ctx->num_cus * 16);
     hipLaunchKernelGGL(k_pattern_bits, dim3(std::max(blocks, 1)), dim3(kThreads), 0, ctx->stream,
@@ -187,6 +188,7 @@ int pattern_bits(ptbgpu_ctx *ctx, const double *dH, const double *dS, int64_t nd
 
 int group_union(ptbgpu_ctx *ctx, const uint32_t *bits, int64_t words, const int *dgptr,
                 const int *dgcols, int nsm, uint32_t *uni, int *dmcount) {
+    ProfScope prof_(ctx, "group_union");
     PTB_HIP(ctx, hipMemsetAsync(dmcount, 0, sizeof(int) * nsm, ctx->stream));
     int bx = (int)std::min<int64_t>((words + kThreads - 1) / kThreads, 64);
     hipLaunchKernelGGL(k_group_union, dim3(std::max(bx, 1), nsm), dim3(kThreads), 0, ctx->stream,
@@ -197,6 +199,7 @@ int group_union(ptbgpu_ctx *ctx, const uint32_t *bits, int64_t words, const int 
 
 int group_map(ptbgpu_ctx *ctx, const uint32_t *uni, int64_t words, const SubDesc *ddesc,
               int nsm, int *dmap, const int *dgptr, const int *dgcols, int *doloc) {
+    ProfScope prof_(ctx, "group_map");
     hipLaunchKernelGGL(k_group_map, dim3(nsm), dim3(kThreads), 0, ctx->stream, uni, words, ddesc,
                        dmap);
     PTB_CHECK_LAUNCH(ctx);
@@ -208,6 +211,7 @@ int group_map(ptbgpu_ctx *ctx, const uint32_t *uni, int64_t words, const SubDesc
 
 int gather(ptbgpu_ctx *ctx, const double *dH, const double *dS, const SubDesc *ddesc, int nsm,
            int max_m, const int *dmap, double *Hd, double *Sd) {
+    ProfScope prof_(ctx, "gather");
     const int nt = (max_m + 31) / 32;
     const int ntri = nt * (nt + 1) / 2;
     hipLaunchKernelGGL(k_gather, dim3(std::max(ntri, 1), nsm), dim3(kThreads), 0, ctx->stream, dH,

@@ -174,12 +174,12 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     tm.mark(3, st);
     // ---- eigendecomposition
     PTB_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int) * nsm, st));
-    PTB_TRY(ptb::potrf_batch(ctx, ddesc, nsm, B, flags));
-    PTB_TRY(ptb::trsm_batch(ctx, ddesc, nsm, max_m, B, A, false));         // A = L^-1 H
-    PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, A, Z, false));  // Z = (L^-1 H)^T
-    PTB_TRY(ptb::trsm_batch(ctx, ddesc, nsm, max_m, B, Z, false));         // Z = K
+    PTB_TRY(ptb::potrf_batch(ctx, desc, ddesc, B, flags));
+    PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, A, Z, false, "h"));        // Z = L^-1 H
+    PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, Z, A, false));  // A = (L^-1 H)^T
+    PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, A, Z, false, "k"));        // Z = K
     PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, Z, A, true));   // A = sym(K)
-    PTB_TRY(ptb::sytrd_batch(ctx, ddesc, nsm, A, W, dvec, evec, tau, max_m));
+    PTB_TRY(ptb::sytrd_batch(ctx, desc, ddesc, A, W, dvec, evec, tau));
     PTB_TRY(ptb::zero_upper_batch(ctx, ddesc, nsm, max_m, A));              // A = V
     // divide and conquer
     std::vector<DcNode> leaves;
@@ -256,7 +256,7 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     std::vector<int> hflags(nsm);
     PTB_HIP(ctx, hipMemcpyAsync(hflags.data(), flags, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
     PTB_TRY(ptb::fe_owned_rows(ctx, ddesc, nsm, max_m, max_own, doloc, B, Z, Lown, SY));
-    PTB_TRY(ptb::trsm_batch(ctx, ddesc, nsm, max_m, B, Z, true));
+    PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, Z, A, true, "u"));         // A = U = L^-T Y
     PTB_HIP(ctx, hipStreamSynchronize(st));
     int64_t not_spd = 0;
     for (int s = 0; s < nsm; ++s)
@@ -270,14 +270,14 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
                              (long long)not_spd);
         PTB_HIP(ctx, hipMemcpyAsync(ddesc, desc.data(), sizeof(SubDesc) * nsm, hipMemcpyHostToDevice, st));
     }
-    PTB_TRY(ptb::fe_finish(ctx, ddesc, nsm, max_m, doloc, SY, Z, fe));
+    PTB_TRY(ptb::fe_finish(ctx, ddesc, nsm, max_m, doloc, SY, A, fe));
     tm.mark(5, st);
     // ---- mu bisection
     PTB_TRY(ptb::bisect_mu(ctx, dvec, fe, rows, nelref, bis));
     tm.mark(6, st);
     // ---- final density
     PTB_HIP(ctx, hipMemsetAsync(dP, 0, sizeof(double) * (size_t)(ndim * (ndim + 1) / 2), st));
-    PTB_TRY(ptb::dmat_scatter(ctx, ddesc, nsm, max_m, max_own, dgcols, doloc, dmap, dvec, Z, bis, dP));
+    PTB_TRY(ptb::dmat_scatter(ctx, ddesc, nsm, max_m, max_own, dgcols, doloc, dmap, dvec, A, bis, dP));
     tm.mark(7, st);
     std::vector<double> hb(4 + 200);
     PTB_HIP(ctx, hipMemcpyAsync(hb.data(), bis, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, st));

@@ -47,18 +47,21 @@ int gather(ptbgpu_ctx *ctx, const double *dH, const double *dS, const SubDesc *d
 // dense.hip
 int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int count, int max_m,
                 int max_n);
-int potrf_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, double *A, int *flags);
+int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                double *A, int *flags);
 int trsm_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *L,
                double *B, bool trans);
 int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m,
                         const double *in, double *out, bool symmetrize);
+int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+              const double *L, double *B, double *X, bool trans, const char *tag);
 int zero_upper_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, double *A);
 int larft_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, const double *V,
                 const double *tau, int q, int nb, double *T);
 
 // tridiag.hip
-int sytrd_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, double *A, double *W,
-                double *d, double *e, double *tau, int max_m);
+int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                double *A, double *W, double *d, double *e, double *tau);
 
 // dc.hip
 int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int nleaves,

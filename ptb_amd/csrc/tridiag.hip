@@ -33,56 +33,59 @@ __device__ __forceinline__ double block_reduce(double v, double *red) {
     return t;
 }
 
-__global__ __launch_bounds__(kT) void k_sytrd(const SubDesc *__restrict__ desc,
-                                              double *__restrict__ Aa, double *__restrict__ Wa,
-                                              double *__restrict__ d_all,
-                                              double *__restrict__ e_all,
-                                              double *__restrict__ tau_all) {
+__global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict__ desc,
+                                                    double *__restrict__ Aa,
+                                                    double *__restrict__ Wa,
+                                                    double *__restrict__ d_all,
+                                                    double *__restrict__ e_all,
+                                                    double *__restrict__ tau_all, int panel) {
     extern __shared__ double sm[];
     const SubDesc ds = desc[blockIdx.x];
     const int m = ds.m, ld = ds.ld;
     double *a = Aa + ds.off;
     double *W = Wa + ds.off;  // columns 0..31 used
     double *dd = d_all + ds.mapoff, *ee = e_all + ds.mapoff, *tt = tau_all + ds.mapoff;
-    double *vs = sm;           // m
-    double *red = vs + m;      // kWaves
-    double *c1 = red + kWaves; // kNB  W[k][t]
-    double *c2 = c1 + kNB;     // kNB  V[k][k0+t]
-    double *q1 = c2 + kNB;     // kNB  W[:,t]^T v
-    double *q2 = q1 + kNB;     // kNB  V[:,k0+t]^T v
-    __shared__ double sc[4];
+    double *vs = sm;              // m   current Householder vector (rows > k)
+    double *praw = vs + m;        // m   A22 v before the pending corrections
+    double *part = praw + m;      // kT  column-group partial sums
+    double *red = part + kT;      // kWaves
+    double *c1 = red + kWaves;    // kNB  W[k][t]
+    double *c2 = c1 + kNB;        // kNB  V[k][k0+t]
+    double *q1 = c2 + kNB;        // kNB  W[:,t]^T v
+    double *q2 = q1 + kNB;        // kNB  V[:,k0+t]^T v
+    double *qred = q2 + kNB;      // kWaves x 16
     const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
 
-    if (m == 1) {
-        if (threadIdx.x == 0) dd[0] = a[0];
-        return;
-    }
-    for (int k0 = 0; k0 < m - 2; k0 += kNB) {
+    const int k0 = panel * kNB;
+    if (k0 >= m - 2) return;
+    {
         const int kb = min(kNB, m - 2 - k0);
         for (int jj = 0; jj < kb; ++jj) {
             const int k = k0 + jj;
-            // (1) pending-update coefficients of row k
+            const int n = m - k - 1;  // trailing size
+            // (a) pending-update coefficients of row k
             if ((int)threadIdx.x < jj) {
                 c1[threadIdx.x] = W[(int64_t)threadIdx.x * ld + k];
                 c2[threadIdx.x] = a[(int64_t)(k0 + threadIdx.x) * ld + k];
             }
             __syncthreads();
+            // (b) column k with the pending terms applied; sigma = |x(k+2:)|^2
+            double part_s = 0.0;
             for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
                 double x = a[(int64_t)k * ld + i];
+#pragma unroll 8
                 for (int t = 0; t < jj; ++t)
                     x -= a[(int64_t)(k0 + t) * ld + i] * c1[t] + W[(int64_t)t * ld + i] * c2[t];
                 vs[i] = x;
+                if (i > k + 1) part_s += x * x;
             }
             if (threadIdx.x == 0) {
                 double dkk = a[(int64_t)k * ld + k];
                 for (int t = 0; t < jj; ++t) dkk -= 2.0 * c2[t] * c1[t];
                 dd[k] = dkk;
             }
-            __syncthreads();
-            // (2) reflector
-            double part = 0.0;
-            for (int i = k + 2 + threadIdx.x; i < m; i += kT) part += vs[i] * vs[i];
-            const double sigma = block_reduce(part, red);
+            const double sigma = block_reduce(part_s, red);  // includes a barrier
+            // (c) reflector
             const double alpha = vs[k + 1];
             double beta, tk, scale;
             if (sigma == 0.0) {
@@ -105,39 +108,103 @@ __global__ __launch_bounds__(kT) void k_sytrd(const SubDesc *__restrict__ desc,
                 tt[k] = tk;
             }
             __syncthreads();
-            // pending dots q1[t] = W[:,t]^T v, q2[t] = V[:,k0+t]^T v over rows k+1..
-            for (int t = wid; t < jj; t += kWaves) {
-                double s1 = 0.0, s2 = 0.0;
-                for (int i = k + 1 + lane; i < m; i += PTB_WAVE) {
+            // (d) pending dots q1[t] = W[:,t]^T v, q2[t] = V[:,k0+t]^T v, eight t at a time
+            for (int t0 = 0; t0 < jj; t0 += 8) {
+                double acc[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+                for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
                     const double v = vs[i];
-                    s1 += W[(int64_t)t * ld + i] * v;
-                    s2 += a[(int64_t)(k0 + t) * ld + i] * v;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int t = min(t0 + q, jj - 1);
+                        acc[q] += W[(int64_t)t * ld + i] * v;
+                        acc[8 + q] += a[(int64_t)(k0 + t) * ld + i] * v;
+                    }
                 }
-                s1 = wave_sum(s1);
-                s2 = wave_sum(s2);
-                if (lane == 0) {
-                    q1[t] = s1;
-                    q2[t] = s2;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const double sq = wave_sum(acc[q]);
+                    if (lane == 0) qred[wid * 16 + q] = sq;
+                }
+                __syncthreads();
+                if ((int)threadIdx.x < 16) {
+                    const int q = threadIdx.x & 7;
+                    double sq = 0.0;
+                    for (int w = 0; w < kWaves; ++w) sq += qred[w * 16 + threadIdx.x];
+                    if (t0 + q < jj) {
+                        if (threadIdx.x < 8) q1[t0 + q] = sq;
+                        else q2[t0 + q] = sq;
+                    }
+                }
+                __syncthreads();
+            }
+            // (e) praw = A22 v over the full trailing matrix.  R threads per
+            // column group cover the rows, G = kT/R column groups split the
+            // columns, sixteen columns per iteration (32 loads in flight per
+            // thread with the row pairs).
+            int R = ((n + PTB_WAVE - 1) / PTB_WAVE) * PTB_WAVE;
+            if (R > kT) R = kT;
+            const int G = kT / R;
+            {
+                const int g = threadIdx.x / R, rt = threadIdx.x % R;
+                if (g < G) {
+                    const int CH = (n + G - 1) / G;
+                    const int cb = k + 1 + g * CH, ce = min(m, cb + CH);
+                    for (int i0 = k + 1 + rt; i0 < m; i0 += 2 * R) {
+                        const int i1 = i0 + R;
+                        const bool h1 = i1 < m;
+                        const int i1c = h1 ? i1 : i0;
+                        double s0[4] = {0.0, 0.0, 0.0, 0.0}, s1[4] = {0.0, 0.0, 0.0, 0.0};
+                        int c = cb;
+                        for (; c + 7 < ce; c += 8) {
+                            double x0[8], x1[8];
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) {
+                                x0[q] = a[(int64_t)(c + q) * ld + i0];
+                                x1[q] = a[(int64_t)(c + q) * ld + i1c];
+                            }
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) {
+                                const double v = vs[c + q];
+                                s0[q & 3] += x0[q] * v;
+                                s1[q & 3] += x1[q] * v;
+                            }
+                        }
+                        for (; c < ce; ++c) {
+                            const double v = vs[c];
+                            s0[0] += a[(int64_t)c * ld + i0] * v;
+                            s1[0] += a[(int64_t)c * ld + i1c] * v;
+                        }
+                        const double p0 = (s0[0] + s0[1]) + (s0[2] + s0[3]);
+                        const double p1 = (s1[0] + s1[1]) + (s1[2] + s1[3]);
+                        if (G == 1) {
+                            praw[i0] = p0;
+                            if (h1) praw[i1] = p1;
+                        } else {
+                            part[g * R + rt] = p0;  // R >= n here: one row per thread
+                        }
+                    }
+                }
+                if (G > 1) {
+                    __syncthreads();
+                    for (int r = threadIdx.x; r < n; r += kT) {
+                        double sacc = 0.0;
+                        for (int gg = 0; gg < G; ++gg) sacc += part[gg * R + r];
+                        praw[k + 1 + r] = sacc;
+                    }
                 }
             }
             __syncthreads();
-            // (3) p = A22 v - V q1 - W q2 ; (4) w = tau p - 0.5 tau^2 (p^T v) v
+            // (f) p = tau (praw - V q1 - W q2);  w = p - 0.5 tau (p^T v) v
             double pv = 0.0;
             for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
-                double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
-                int c = k + 1;
-                for (; c + 3 < m; c += 4) {
-                    acc0 += a[(int64_t)c * ld + i] * vs[c];
-                    acc1 += a[(int64_t)(c + 1) * ld + i] * vs[c + 1];
-                    acc2 += a[(int64_t)(c + 2) * ld + i] * vs[c + 2];
-                    acc3 += a[(int64_t)(c + 3) * ld + i] * vs[c + 3];
-                }
-                for (; c < m; ++c) acc0 += a[(int64_t)c * ld + i] * vs[c];
-                double p = (acc0 + acc1) + (acc2 + acc3);
+                double p = praw[i];
+#pragma unroll 8
                 for (int t = 0; t < jj; ++t)
                     p -= a[(int64_t)(k0 + t) * ld + i] * q1[t] + W[(int64_t)t * ld + i] * q2[t];
                 p *= tk;
-                W[(int64_t)jj * ld + i] = p;  // provisional, finished below
+                W[(int64_t)jj * ld + i] = p;
                 pv += p * vs[i];
             }
             const double ptv = block_reduce(pv, red);
@@ -147,63 +214,89 @@ __global__ __launch_bounds__(kT) void k_sytrd(const SubDesc *__restrict__ desc,
             for (int i = threadIdx.x; i <= k; i += kT) W[(int64_t)jj * ld + i] = 0.0;
             __syncthreads();
         }
-        // trailing update A[s:, s:] -= Vb Wb^T + Wb Vb^T over both triangles
-        const int s = k0 + kb;
-        const int rem = m - s;
-        if (rem > 0) {
-            double *Vi = sm;                 // 32 x kNB each, reusing the vector space
-            double *Wi = Vi + 32 * kNB;
-            double *Vj = Wi + 32 * kNB;
-            double *Wj = Vj + 32 * kNB;
-            const int nt = (rem + 31) / 32;
-            for (int tile = 0; tile < nt * nt; ++tile) {
-                const int i0 = s + (tile % nt) * 32, j0 = s + (tile / nt) * 32;
-                __syncthreads();
-                for (int l = threadIdx.x; l < 32 * kNB; l += kT) {
-                    const int r = l & 31, t = l >> 5;
-                    const bool okt = t < kb;
-                    Vi[r * kNB + t] = (okt && i0 + r < m) ? a[(int64_t)(k0 + t) * ld + i0 + r] : 0.0;
-                    Wi[r * kNB + t] = (okt && i0 + r < m) ? W[(int64_t)t * ld + i0 + r] : 0.0;
-                    Vj[r * kNB + t] = (okt && j0 + r < m) ? a[(int64_t)(k0 + t) * ld + j0 + r] : 0.0;
-                    Wj[r * kNB + t] = (okt && j0 + r < m) ? W[(int64_t)t * ld + j0 + r] : 0.0;
-                }
-                __syncthreads();
-                for (int l = threadIdx.x; l < 1024; l += kT) {
-                    const int r = l & 31, c = l >> 5;
-                    if (i0 + r < m && j0 + c < m) {
-                        double sacc = 0.0;
-#pragma unroll 8
-                        for (int t = 0; t < kNB; ++t)
-                            sacc += Vi[r * kNB + t] * Wj[c * kNB + t] + Wi[r * kNB + t] * Vj[c * kNB + t];
-                        a[(int64_t)(j0 + c) * ld + i0 + r] -= sacc;
-                    }
-                }
-            }
-            __syncthreads();
-        }
     }
-    if (threadIdx.x == 0) {
-        dd[m - 2] = a[(int64_t)(m - 2) * ld + m - 2];
-        dd[m - 1] = a[(int64_t)(m - 1) * ld + m - 1];
-        ee[m - 2] = a[(int64_t)(m - 2) * ld + m - 1];
-        tt[m - 2] = 0.0;
-        tt[m - 1] = 0.0;
-        ee[m - 1] = 0.0;
+}
+
+// d, e of the last two rows once every panel and trailing update is done
+__global__ void k_sytrd_finish(const SubDesc *__restrict__ desc, const double *__restrict__ Aa,
+                               double *__restrict__ d_all, double *__restrict__ e_all,
+                               double *__restrict__ tau_all, int nsm) {
+    const int si = blockIdx.x * blockDim.x + threadIdx.x;
+    if (si >= nsm) return;
+    const SubDesc ds = desc[si];
+    const int m = ds.m, ld = ds.ld;
+    const double *a = Aa + ds.off;
+    double *dd = d_all + ds.mapoff, *ee = e_all + ds.mapoff, *tt = tau_all + ds.mapoff;
+    if (m == 1) {
+        dd[0] = a[0];
+        ee[0] = 0.0;
+        tt[0] = 0.0;
+        return;
     }
-    (void)sc;
+    dd[m - 2] = a[(int64_t)(m - 2) * ld + m - 2];
+    dd[m - 1] = a[(int64_t)(m - 1) * ld + m - 1];
+    ee[m - 2] = a[(int64_t)(m - 2) * ld + m - 1];
+    tt[m - 2] = 0.0;
+    tt[m - 1] = 0.0;
+    ee[m - 1] = 0.0;
 }
 
 }  // namespace
 
 namespace ptb {
 
-int sytrd_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, double *A, double *W, double *d,
-                double *e, double *tau, int max_m) {
-    const size_t vec = (size_t)max_m + kWaves + 4 * kNB;
-    const size_t tiles = 4 * 32 * kNB;
-    const size_t shmem = sizeof(double) * std::max(vec, tiles);
+// Panel by panel: the panel kernel reduces 32 columns of every submatrix, then
+// two batched GEMMs apply the rank-64 update A(s:,s:) -= V W^T + W V^T.
+int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                double *A, double *W, double *d, double *e, double *tau) {
+    ProfScope prof_(ctx, "sytrd");
+    const int nsm = (int)desc.size();
+    int max_m = 0;
+    for (const auto &x : desc) max_m = std::max(max_m, x.m);
+    const size_t shmem = sizeof(double) * (2 * (size_t)max_m + kT + kWaves + 4 * kNB + kWaves * 16);
     if (shmem > 160 * 1024) return fail(ctx, PTBGPU_E_ARG, "submatrix too large for sytrd (%d)", max_m);
-    hipLaunchKernelGGL(k_sytrd, dim3(nsm), dim3(kT), shmem, ctx->stream, ddesc, A, W, d, e, tau);
+    const int npanel = max_m > 2 ? (max_m - 2 + kNB - 1) / kNB : 0;
+    std::vector<GemmDesc> g1, g2;
+    std::vector<int> start(npanel + 1, 0);
+    for (int p = 0; p < npanel; ++p) {
+        start[p] = (int)g1.size();
+        for (int si = 0; si < nsm; ++si) {
+            const SubDesc &x = desc[si];
+            const int k0 = p * kNB;
+            if (k0 >= x.m - 2) continue;
+            const int kb = std::min(kNB, x.m - 2 - k0);
+            const int s0 = k0 + kb, n = x.m - s0;
+            if (n <= 0) continue;
+            double *a = A + x.off;
+            double *V = a + (int64_t)k0 * x.ld + s0, *Wp = W + x.off + s0;
+            double *C = a + (int64_t)s0 * x.ld + s0;
+            g1.push_back({V, Wp, C, n, n, kb, x.ld, x.ld, x.ld, -1.0, 1.0});
+            g2.push_back({Wp, V, C, n, n, kb, x.ld, x.ld, x.ld, -1.0, 1.0});
+        }
+    }
+    start[npanel] = (int)g1.size();
+    const size_t gb = sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1);
+    auto *dg1 = static_cast<GemmDesc *>(ws(ctx, "sytrd_g1", gb));
+    auto *dg2 = static_cast<GemmDesc *>(ws(ctx, "sytrd_g2", gb));
+    if (!dg1 || !dg2) return PTBGPU_E_NOMEM;
+    if (!g1.empty()) {
+        PTB_HIP(ctx, hipMemcpyAsync(dg1, g1.data(), sizeof(GemmDesc) * g1.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        PTB_HIP(ctx, hipMemcpyAsync(dg2, g2.data(), sizeof(GemmDesc) * g2.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    }
+    for (int p = 0; p < npanel; ++p) {
+        hipLaunchKernelGGL(k_sytrd_panel, dim3(nsm), dim3(kT), shmem, ctx->stream, ddesc, A, W, d,
+                           e, tau, p);
+        PTB_CHECK_LAUNCH(ctx);
+        const int c = start[p + 1] - start[p];
+        if (c > 0) {  // two launches: both products update the same trailing block
+            PTB_TRY(gemm_vbatch(ctx, false, true, dg1 + start[p], c, max_m, max_m));
+            PTB_TRY(gemm_vbatch(ctx, false, true, dg2 + start[p], c, max_m, max_m));
+        }
+    }
+    hipLaunchKernelGGL(k_sytrd_finish, dim3((nsm + 255) / 256), dim3(256), 0, ctx->stream, ddesc, A,
+                       d, e, tau, nsm);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }

@@ -67,6 +67,9 @@ def _lib():
     for name in ("ptbgpu_finalize", "ptbgpu_sync"):
         getattr(lib, name).argtypes = [VP]
     lib.ptbgpu_stream.argtypes = [VP, ctypes.POINTER(VP)]
+    lib.ptbgpu_profile.argtypes = [VP, I64]
+    lib.ptbgpu_profile_query.argtypes = [VP, ctypes.c_char_p, ctypes.POINTER(F64),
+                                         ctypes.POINTER(I64)]
     lib.ptbgpu_filter_sym.argtypes = [VP, I64, VP, F64, VP]
     lib.ptbgpu_filter_sym_dev.argtypes = [VP, I64, VP, F64, VP]
     lib.ptbgpu_analyse_sym.argtypes = [VP, I64, VP, VP]
@@ -142,6 +145,18 @@ class Engine:
 
     def sync(self):
         self._check(self.lib.ptbgpu_sync(self.ctx))
+
+    def profile(self, enable: bool):
+        """Start (clearing old records) or stop per-kernel event timing."""
+        self._check(self.lib.ptbgpu_profile(self.ctx, 1 if enable else 0))
+
+    def profile_query(self, name: str):
+        """(total milliseconds, launch groups) recorded for one kernel name."""
+        t = F64(0.0)
+        c = I64(0)
+        self._check(self.lib.ptbgpu_profile_query(self.ctx, name.encode(), ctypes.byref(t),
+                                                  ctypes.byref(c)))
+        return t.value, c.value
 
     # ---- filter / analyse -------------------------------------------------
     def filter_matrix_sym(self, A: np.ndarray, n: int, filter_: float):

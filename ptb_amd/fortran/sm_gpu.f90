@@ -1,0 +1,212 @@
+! Drop-in Fortran modules over the C ABI of libptbgpu.so (include/ptbgpu.h).
+!
+! `module sm` keeps the name and the call signature of the reference module
+! (source/sm.f90:1,34-44): a host built with -i8 (source/Makefile:37) that does
+! `use sm` and calls sm_stupid_simple(ndim,nelref,Hvec,Svec,Pvec,n,xyz,aoat,
+! submatrix_columns,submatrix_mode) relinks against this module instead and
+! gets the density from the GPU.  Every default integer of the caller is
+! 8 bytes, which is what the C ABI takes (int64_t).
+!
+! `module ptbgpu_fortran` holds the context (one per process, created on first
+! use on device 0 or $PTBGPU_DEVICE) and the other entry points: the in-place
+! filter that replaces filter_matrix_sym (source/pgtb.f90:1532-1548) and the
+! density checks of source/metrics.f90.  The precision of the per-submatrix
+! solve comes from $PTBGPU_PRECISION (fp64, the default, or mixed), so the
+! reference's .PUR grammar and call signatures stay unchanged.
+!
+! A non-zero status from the engine becomes `error stop` with the engine's
+! message, where the reference itself would `stop` (source/sm.f90:160-161,250-251).
+module ptbgpu_fortran
+   use iso_c_binding
+   implicit none
+   private
+   public :: ptbgpu_handle, ptbgpu_check, gpu_filter_matrix_sym, gpu_get_nel, &
+      gpu_band_structure, gpu_precision
+
+   type(c_ptr), save :: ctx = c_null_ptr
+
+   interface
+      integer(c_int) function ptbgpu_init(pctx, device, flags) bind(C, name="ptbgpu_init")
+         import :: c_ptr, c_int, c_int64_t
+         type(c_ptr) :: pctx
+         integer(c_int64_t), value :: device, flags
+      end function
+      type(c_ptr) function ptbgpu_last_error(pctx) bind(C, name="ptbgpu_last_error")
+         import :: c_ptr
+         type(c_ptr), value :: pctx
+      end function
+      integer(c_int) function ptbgpu_filter_sym(pctx, ndim, a, filter, counts) &
+            bind(C, name="ptbgpu_filter_sym")
+         import :: c_ptr, c_int, c_int64_t, c_double
+         type(c_ptr), value :: pctx
+         integer(c_int64_t), value :: ndim
+         real(c_double) :: a(*)
+         real(c_double), value :: filter
+         integer(c_int64_t) :: counts(2)
+      end function
+      integer(c_int) function ptbgpu_get_nel(pctx, ndim, p, s, nel) bind(C, name="ptbgpu_get_nel")
+         import :: c_ptr, c_int, c_int64_t, c_double
+         type(c_ptr), value :: pctx
+         integer(c_int64_t), value :: ndim
+         real(c_double) :: p(*), s(*), nel
+      end function
+      integer(c_int) function ptbgpu_band_energy(pctx, ndim, p, h, e) &
+            bind(C, name="ptbgpu_band_energy")
+         import :: c_ptr, c_int, c_int64_t, c_double
+         type(c_ptr), value :: pctx
+         integer(c_int64_t), value :: ndim
+         real(c_double) :: p(*), h(*), e
+      end function
+   end interface
+
+contains
+
+   function ptbgpu_handle() result(h)
+      type(c_ptr) :: h
+      integer(c_int) :: st
+      integer(c_int64_t) :: dev
+      character(len=16) :: buf
+      integer :: ln, ios
+      if (.not. c_associated(ctx)) then
+         dev = 0
+         call get_environment_variable("PTBGPU_DEVICE", buf, ln)
+         if (ln > 0) then
+            read (buf(1:ln), *, iostat=ios) dev
+            if (ios /= 0) dev = 0
+         end if
+         st = ptbgpu_init(ctx, dev, 0_c_int64_t)
+         if (st /= 0) error stop "ptbgpu: no HIP device for the density engine"
+      end if
+      h = ctx
+   end function
+
+   ! mixed precision if $PTBGPU_PRECISION is "mixed", else plain FP64
+   integer(c_int64_t) function gpu_precision() result(p)
+      character(len=16) :: buf
+      integer :: ln
+      p = 0
+      call get_environment_variable("PTBGPU_PRECISION", buf, ln)
+      if (ln > 0) then
+         if (buf(1:ln) == "mixed") p = 1
+      end if
+   end function
+
+   subroutine ptbgpu_check(st)
+      integer(c_int), intent(in) :: st
+      character(kind=c_char), pointer :: msg(:)
+      character(len=512) :: text
+      integer :: i
+      if (st == 0) return
+      call c_f_pointer(ptbgpu_last_error(ctx), msg, [512])
+      text = ""
+      do i = 1, 512
+         if (msg(i) == c_null_char) exit
+         text(i:i) = msg(i)
+      end do
+      print *, "ptbgpu error ", st, ": ", trim(text)
+      error stop "ptbgpu: density engine failed"
+   end subroutine
+
+   ! filter_matrix_sym (source/pgtb.f90:1532-1548): A(i)=0 where abs(A(i))<filter
+   subroutine gpu_filter_matrix_sym(A, n, filter)
+      real(c_double), intent(inout) :: A(:)
+      integer(c_int64_t), intent(in) :: n
+      real(c_double), intent(in) :: filter
+      integer(c_int64_t) :: counts(2)
+      call ptbgpu_check(ptbgpu_filter_sym(ptbgpu_handle(), n, A, filter, counts))
+   end subroutine
+
+   ! get_nel_packed (source/metrics.f90:202-230)
+   real(c_double) function gpu_get_nel(ndim, P, S) result(nel)
+      integer(c_int64_t), intent(in) :: ndim
+      real(c_double), intent(in) :: P(*), S(*)
+      real(c_double), allocatable :: pc(:), sc(:)
+      allocate (pc(ndim*(ndim+1)/2), sc(ndim*(ndim+1)/2))
+      pc = P(1:ndim*(ndim+1)/2)
+      sc = S(1:ndim*(ndim+1)/2)
+      call ptbgpu_check(ptbgpu_get_nel(ptbgpu_handle(), ndim, pc, sc, nel))
+   end function
+
+   ! get_band_structure_E_packed (source/metrics.f90:253-277)
+   real(c_double) function gpu_band_structure(ndim, P, H) result(e)
+      integer(c_int64_t), intent(in) :: ndim
+      real(c_double), intent(in) :: P(*), H(*)
+      real(c_double), allocatable :: pc(:), hc(:)
+      allocate (pc(ndim*(ndim+1)/2), hc(ndim*(ndim+1)/2))
+      pc = P(1:ndim*(ndim+1)/2)
+      hc = H(1:ndim*(ndim+1)/2)
+      call ptbgpu_check(ptbgpu_band_energy(ptbgpu_handle(), ndim, pc, hc, e))
+   end function
+
+end module ptbgpu_fortran
+
+module sm
+   use iso_c_binding
+   use ptbgpu_fortran
+   implicit none
+   private
+   public :: sm_stupid_simple
+
+   type, bind(C) :: ptbgpu_opts
+      integer(c_int64_t) :: precision, strict, verbose, refine_steps
+   end type
+
+   type, bind(C) :: ptbgpu_report
+      real(c_double) :: mu, nel
+      integer(c_int64_t) :: muiter, nsm, sum_m, max_m
+      real(c_double) :: sum_m3
+      integer(c_int64_t) :: not_spd
+      real(c_double) :: t_phase(12)
+   end type
+
+   interface
+      integer(c_int) function ptbgpu_sm_density(pctx, ndim, nelref, h, s, p, n, xyz, aoat, &
+            cols, mode, opts, rep) bind(C, name="ptbgpu_sm_density")
+         import :: c_ptr, c_int, c_int64_t, c_double, ptbgpu_opts, ptbgpu_report
+         type(c_ptr), value :: pctx
+         integer(c_int64_t), value :: ndim, nelref, n, cols, mode
+         real(c_double) :: h(*), s(*), p(*), xyz(*)
+         integer(c_int64_t) :: aoat(*)
+         type(ptbgpu_opts) :: opts
+         type(ptbgpu_report) :: rep
+      end function
+   end interface
+
+contains
+
+   ! sm_stupid_simple (source/sm.f90:34-409), same arguments; Pvec overwritten
+   subroutine sm_stupid_simple(ndim, nelref, Hvec, Svec, Pvec, n, xyz, aoat, &
+         submatrix_columns, submatrix_mode)
+      integer(c_int64_t), intent(in) :: nelref
+      integer(c_int64_t), intent(in) :: ndim
+      real(c_double), intent(in) :: Hvec(:)
+      real(c_double), intent(in) :: Svec(:)
+      real(c_double), intent(inout) :: Pvec(:)
+      integer(c_int64_t), intent(in) :: n
+      real(c_double), intent(in) :: xyz(3, n)
+      integer(c_int64_t), intent(in) :: aoat(ndim)
+      integer(c_int64_t), intent(in) :: submatrix_columns
+      integer(c_int64_t), intent(in) :: submatrix_mode
+      type(ptbgpu_opts) :: opts
+      type(ptbgpu_report) :: rep
+      real(c_double), allocatable :: hc(:), sc(:), pc(:)
+      integer(c_int64_t) :: npk
+      npk = ndim*(ndim + 1)/2
+      if (size(Hvec) < npk .or. size(Svec) < npk .or. size(Pvec) < npk) &
+         error stop "sm_stupid_simple: packed arrays shorter than ndim*(ndim+1)/2"
+      ! contiguous copies (assumed-shape actuals may be strided)
+      allocate (hc(npk), sc(npk), pc(npk))
+      hc = Hvec(1:npk)
+      sc = Svec(1:npk)
+      opts%precision = gpu_precision()
+      opts%strict = 0
+      opts%verbose = 1
+      opts%refine_steps = 0
+      call ptbgpu_check(ptbgpu_sm_density(ptbgpu_handle(), ndim, nelref, hc, sc, pc, n, xyz, &
+         aoat, submatrix_columns, submatrix_mode, opts, rep))
+      Pvec(1:npk) = pc
+      print *, "number of submatrices=", rep%nsm
+      print *, "Submatrix (GPU)", rep%t_phase(10), "sec"
+   end subroutine
+
+end module sm

@@ -1,0 +1,130 @@
+"""Write the golden fixtures of this directory from the compiled reference.
+
+Runs the reference's own hot-path routines (oracle/_ref/libptbref.so, built by
+`make -C oracle` from the reference sources; only possible where those sources
+exist) on small deterministic inputs and stores inputs and outputs as .npz:
+
+  sm_<case>.npz     sm_stupid_simple (source/sm.f90:34-409): filtered packed H,
+                    S, xyz, aoat, nel, columns mode -> packed P, plus the mu
+                    trace parsed from the reference's "muiter" lines
+  metrics.npz       get_nel / band_structure (source/metrics.f90:202-277)
+  fermi.npz         FERMISMEAR and occu (source/fermi.f:5-124)
+  pur_<case>.npz    purification_wrapper (source/purification.f90:28-80) for
+                    McWeeny, Pade, Newton-Schulz and sign-diagonalisation with
+                    the inverse / inverse-square-root metric, direct and iterative
+  kmeans.npz        clustering_kmeans groups for nsm = 1..4 (flang RANDOM_NUMBER)
+
+Usage: python tests/golden/make_golden.py   (needs oracle/_ref/libptbref.so)
+"""
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from ptb_amd import geometry, surrogate  # noqa: E402
+from oracle import ptb_oracle as O  # noqa: E402
+from oracle import ref as R  # noqa: E402
+
+SM_CASES = {
+    # name: (kind, size, seed, columns, filter)
+    "h2o8_atoms": ("water", (2, 2, 2), 0, 2, 1e-4),
+    "h2o8_single": ("water", (2, 2, 2), 0, 1, 1e-4),
+    "h2o8_full": ("water", (2, 2, 2), 0, 3, 1e-4),
+    "h2o27_atoms": ("water", (3, 3, 3), 1, 2, 1e-4),
+    "c8h18_atoms": ("alkane", 8, 0, 2, 1e-5),
+    "c8h18_full": ("alkane", 8, 0, 3, 1e-5),
+}
+
+
+def system(kind, size, seed, filt):
+    mol = geometry.water_box(size, 3.1, 0.2, seed) if kind == "water" else geometry.alkane(size)
+    H, S = surrogate.build_hs(mol)
+    O.filter_matrix_sym(S, mol.ndim, filt)
+    O.filter_matrix_sym(H, mol.ndim, filt)
+    return mol, H, S
+
+
+def parse_mu_trace(log):
+    rows = []
+    for line in log.splitlines():
+        m = re.match(r"\s*muiter\s+(\d+)\s+mu\s+(\S+)\s+(\S+)\s+of\s+(\d+)", line)
+        if m:
+            rows.append((int(m.group(1)), float(m.group(2).replace("D", "E")),
+                         float(m.group(3).replace("D", "E"))))
+    return np.array(rows)
+
+
+def main():
+    if not R.available():
+        sys.exit("oracle/_ref/libptbref.so is not built; run make -C oracle first")
+    only = sys.argv[1:]  # optional subset: sm, fermi, pur, kmeans
+    for name, (kind, size, seed, cols, filt) in SM_CASES.items():
+        if only and "sm" not in only:
+            break
+        mol, H, S = system(kind, size, seed, filt)
+        log = []
+        P = R.sm_stupid_simple(mol.ndim, mol.nel, H, S, mol.n, mol.xyz, mol.aoat, cols, 2, log=log)
+        trace = parse_mu_trace(log[0])
+        np.savez_compressed(os.path.join(HERE, f"sm_{name}.npz"), H=H, S=S, xyz=mol.xyz,
+                            aoat=mol.aoat, nel=mol.nel, n=mol.n, ndim=mol.ndim, cols=cols,
+                            filter=filt, P=P, mu_trace=trace,
+                            eband=R.band_structure(mol.ndim, P, H),
+                            nelp=R.get_nel(mol.ndim, P, S))
+        print(name, mol.ndim, "muiter", len(trace), "mu", trace[-1, 1])
+    # metrics on the atoms-mode density
+    d = np.load(os.path.join(HERE, "sm_h2o8_atoms.npz"))
+    n = int(d["ndim"])
+    np.savez_compressed(os.path.join(HERE, "metrics.npz"), nel=R.get_nel(n, d["P"], d["S"]),
+                        eband=R.band_structure(n, d["P"], d["H"]),
+                        check=R.check_density(n, d["P"], d["S"], int(d["nel"])))
+    # Fermi smearing on a dense spectrum of the 8-water pencil
+    import scipy.linalg as sla
+    e = sla.eigh(O.blowsym(d["H"], n), O.blowsym(d["S"], n), eigvals_only=True)
+    nel = int(d["nel"])
+    occ, ef, fod, s = R.fermismear(n, nel // 2, 300.0, e)
+    occ_hot, ef_hot, fod_hot, s_hot = R.fermismear(n, nel // 2, 20000.0, e)
+    fa, fb, ia, ib = R.occu(n, nel, 0)
+    fa3, fb3, ia3, ib3 = R.occu(n, 7, 3)
+    np.savez_compressed(os.path.join(HERE, "fermi.npz"), eig=e, nel=nel, occ=occ, ef=ef, fod=fod,
+                        s=s, occ_hot=occ_hot, ef_hot=ef_hot, fod_hot=fod_hot, s_hot=s_hot,
+                        focca=fa, foccb=fb, ihomo=np.array([ia, ib]), focca3=fa3, foccb3=fb3,
+                        ihomo3=np.array([ia3, ib3]))
+    # purification on two waters (ndim 46), every type and metric
+    mol, H, S = system("water", (1, 1, 2), 0, 1e-4)
+    for ptype, tname in ((1, "mcweeny"), (2, "pade"), (3, "newton"), (4, "diag")):
+        for metric, mname in ((1, "inv"), (2, "invsqrt")):
+            for it in (0, 1):
+                if ptype == 1 and metric == 2:
+                    continue  # McWeeny guess is built for the S^-1 metric
+                if metric == 2 and it == 1:
+                    # Newton-Schulz for S^-1/2 from S/|S|_F^2 diverges on this
+                    # pencil; the reference ends in ERROR STOP 'Error, NaN
+                    # encountered' (source/purification.f90:179-181), which would
+                    # end this process too.  The tests check the oracle raises.
+                    continue
+                P = R.purification(mol.ndim, mol.nel, ptype, metric, it, 40, 50, 60, -0.45, 0.0,
+                                   H, S)
+                np.savez_compressed(os.path.join(HERE, f"pur_{tname}_{mname}_{it}.npz"), H=H, S=S,
+                                    nel=mol.nel, ndim=mol.ndim, ptype=ptype, metric=metric,
+                                    iterative=it, P=P)
+                print("pur", tname, mname, it, "nel", R.get_nel(mol.ndim, P, S))
+    # k-means groups of the compiled reference (compiler RNG)
+    mol, H, S = system("water", (2, 2, 2), 0, 1e-4)
+    out = {}
+    for nsm in range(1, 5):
+        groups, eff = R.clustering_kmeans(mol.n, mol.ndim, mol.xyz, H, S, mol.aoat, nsm)
+        owner = np.full(mol.ndim, -1)
+        for k, g in enumerate(groups):
+            owner[g] = k
+        out[f"owner_{nsm}"] = owner
+        out[f"effort_{nsm}"] = eff
+    np.savez_compressed(os.path.join(HERE, "kmeans.npz"), **out)
+
+
+if __name__ == "__main__":
+    main()

@@ -33,3 +33,17 @@ def test_strerror_without_device():
     lib.ptbgpu_strerror.argtypes = [ctypes.c_int64]
     assert lib.ptbgpu_strerror(0) == b"ok"
     assert b"positive definite" in lib.ptbgpu_strerror(3)
+
+
+def test_fortran_module_exports_drop_in():
+    """The Fortran shim exports the reference's module procedure name."""
+    import os
+    import subprocess
+    from ptb_amd import _native
+    so = _native.lib_path("libptb_sm_gpu.so")
+    if not os.path.exists(so):
+        import pytest
+        pytest.skip("Fortran shim not built (no flang)")
+    syms = subprocess.run(["nm", "-D", "--defined-only", so], capture_output=True,
+                          text=True).stdout
+    assert "sm_stupid_simple" in syms.lower()
