@@ -661,10 +661,12 @@ def sign_diagonalization(pur: PurSettings, chempot: float, guess: np.ndarray, X:
     """Internal chemical-potential bisection on the guess's eigenvalues, then
     sign = Q sgn Q^T (source/purification.f90:650-782).  The SAVEd flag
     first_time is initialised .true. and never cleared (:671,692), so the guess
-    is diagonalised on every call."""
+    is diagonalised on every call.  chempot is intent(inout) (:659): the
+    internally adjusted value goes back to the outer search.  Returns
+    (P, chempot)."""
     n = guess.shape[0]
     lam, Q = sla.eigh(guess, lower=False, driver="evd")
-    nel = float(pur.nel)
+    nel = float(np.float32(pur.nel))       # real(pur%nel), default REAL(4) (:683)
     incr = pur.chempot_increment
     lower = upper = 0.0
     is_lower = is_upper = False
@@ -688,7 +690,7 @@ def sign_diagonalization(pur: PurSettings, chempot: float, guess: np.ndarray, X:
             if abs(upper - lower) < thrs.high:
                 break
     sign_ = Q @ (np.diag(sgn) @ Q.T)
-    return 2.0 * sign_to_density_matrix(pur, sign_, X)
+    return 2.0 * sign_to_density_matrix(pur, sign_, X), chempot
 
 
 def purification_search(pur: PurSettings, H, S, X, thrs: Thresholds, state=None):
@@ -713,9 +715,9 @@ def purification_search(pur: PurSettings, H, S, X, thrs: Thresholds, state=None)
         elif pur.type in (SIGN_PADE, SIGN_NEWTON):
             P, _ = sign_iterative(pur, guess, X, thrs)
         else:
-            P = sign_diagonalization(pur, chempot, guess, X, thrs, state)
+            P, chempot = sign_diagonalization(pur, chempot, guess, X, thrs, state)
         nel_calc = float(np.trace(P @ S))
-        if abs(nel_calc - float(nel)) < thrs.normal:
+        if abs(nel_calc - float(np.float32(nel))) < thrs.normal:   # real(nel) (:327)
             break
         if nel_calc < nel:
             is_lower, lower = True, chempot

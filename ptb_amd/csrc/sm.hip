@@ -20,6 +20,7 @@
 
 #include "sm_internal.hpp"
 
+
 namespace {
 
 constexpr int kLeafMax = 64;
@@ -179,78 +180,7 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, Z, A, false));  // A = (L^-1 H)^T
     PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, A, Z, false, "k"));        // Z = K
     PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, Z, A, true));   // A = sym(K)
-    PTB_TRY(ptb::sytrd_batch(ctx, desc, ddesc, A, W, dvec, evec, tau));
-    PTB_TRY(ptb::zero_upper_batch(ctx, ddesc, nsm, max_m, A));              // A = V
-    // divide and conquer
-    std::vector<DcNode> leaves;
-    std::vector<std::vector<DcNode>> levels;
-    for (int s = 0; s < nsm; ++s) {
-        int h;
-        build_tree(s, 0, desc[s].m, leaves, levels, h);
-    }
-    PTB_HIP(ctx, hipMemsetAsync(Z, 0, abytes, st));
-    DcNode *dleaves;
-    PTB_TRY(upload(ctx, "sm_leaves", leaves, &dleaves));
-    PTB_TRY(ptb::dc_leaves(ctx, ddesc, dleaves, (int)leaves.size(), dvec, evec, Z));
-    for (size_t lv = 0; lv < levels.size(); ++lv) {
-        const auto &nodes = levels[lv];
-        int max_n = 0;
-        for (const auto &nd : nodes) max_n = std::max(max_n, nd.hi - nd.lo);
-        std::string nm = "sm_level" + std::to_string(lv);
-        DcNode *dnodes;
-        PTB_TRY(upload(ctx, nm.c_str(), nodes, &dnodes));
-        std::string gm = "sm_lgemm" + std::to_string(lv);
-        auto *dg = static_cast<GemmDesc *>(ptb::ws(ctx, gm.c_str(), sizeof(GemmDesc) * nodes.size()));
-        if (!dg) return PTBGPU_E_NOMEM;
-        PTB_TRY(ptb::dc_merge_level(ctx, ddesc, dnodes, (int)nodes.size(), max_n, evec, dvec, Z,
-                                    W, X, rot, dg));
-    }
-    // back-transformation Y = Q Z, reflector blocks from the last to the first
-    int max_blocks = 0;
-    for (int s = 0; s < nsm; ++s) {
-        const int nr = desc[s].m - 2;
-        max_blocks = std::max(max_blocks, nr > 0 ? (nr + kNB - 1) / kNB : 0);
-    }
-    if (max_blocks > 0) {
-        std::vector<GemmDesc> g1, g2, g3;
-        std::vector<int> cnt(max_blocks, 0), start(max_blocks + 1, 0);
-        g1.reserve((size_t)nsm * max_blocks);
-        for (int q = 0; q < max_blocks; ++q) {
-            start[q] = (int)g1.size();
-            for (int s = 0; s < nsm; ++s) {
-                const SubDesc &d = desc[s];
-                const int nr = d.m - 2;
-                const int nblk = nr > 0 ? (nr + kNB - 1) / kNB : 0;
-                const int b = nblk - 1 - q;
-                if (b < 0) continue;
-                const int b0 = b * kNB, kb = std::min(kNB, nr - b0);
-                const int rowsv = d.m - b0 - 1;
-                double *V = A + d.off + (int64_t)b0 * d.ld + b0 + 1;
-                double *Y = Z + d.off + b0 + 1;
-                // W1 = V_b^T Y and W2 = T W1 are kb x m panels inside this
-                // submatrix's W and X blocks (leading dimension <= ld)
-                const int lw = std::min(kNB, d.ld);
-                double *W1 = W + d.off, *W2 = X + d.off;
-                const double *T = Tb + (int64_t)s * kNB * kNB;
-                g1.push_back({V, Y, W1, kb, d.m, rowsv, d.ld, d.ld, lw, 1.0, 0.0});
-                g2.push_back({T, W1, W2, kb, d.m, kb, kNB, lw, lw, 1.0, 0.0});
-                g3.push_back({V, W2, Y, rowsv, d.m, kb, d.ld, lw, d.ld, -1.0, 1.0});
-            }
-        }
-        start[max_blocks] = (int)g1.size();
-        GemmDesc *dg1, *dg2, *dg3;
-        PTB_TRY(upload(ctx, "sm_bt1", g1, &dg1));
-        PTB_TRY(upload(ctx, "sm_bt2", g2, &dg2));
-        PTB_TRY(upload(ctx, "sm_bt3", g3, &dg3));
-        for (int q = 0; q < max_blocks; ++q) {
-            const int c = start[q + 1] - start[q];
-            if (c == 0) continue;
-            PTB_TRY(ptb::larft_batch(ctx, ddesc, nsm, A, tau, q, kNB, Tb));
-            PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dg1 + start[q], c, kNB, max_m));
-            PTB_TRY(ptb::gemm_vbatch(ctx, false, false, dg2 + start[q], c, kNB, max_m));
-            PTB_TRY(ptb::gemm_vbatch(ctx, false, false, dg3 + start[q], c, max_m, max_m));
-        }
-    }
+    PTB_TRY(ptb::eig_sym_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot, Tb));
     tm.mark(4, st);
     // ---- fe and U = L^-T Y
     std::vector<int> hflags(nsm);
@@ -321,6 +251,102 @@ int check_sm_args(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const void *H, 
 }
 
 }  // namespace
+
+namespace ptb {
+
+// Eigenpairs of a batch of dense symmetric matrices (full storage in A, both
+// triangles): Householder tridiagonalisation, divide and conquer, WY
+// back-transformation.  On return Z holds the eigenvectors (column order of
+// the eigenvalues in d, not sorted), A holds the Householder vectors.
+int eig_sym_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                  double *A, double *Z, double *W, double *X, double *dvec, double *evec,
+                  double *tau, double *rot, double *Tb) {
+    hipStream_t st = ctx->stream;
+    const int nsm = (int)desc.size();
+    int max_m = 0;
+    size_t arena = 0;
+    for (const auto &x : desc) {
+        max_m = std::max(max_m, x.m);
+        arena = std::max(arena, (size_t)(x.off + (int64_t)x.ld * x.m));
+    }
+    const size_t abytes = sizeof(double) * arena;
+    PTB_TRY(sytrd_batch(ctx, desc, ddesc, A, W, dvec, evec, tau));
+    PTB_TRY(zero_upper_batch(ctx, ddesc, nsm, max_m, A));              // A = V
+    // divide and conquer
+    std::vector<DcNode> leaves;
+    std::vector<std::vector<DcNode>> levels;
+    for (int s = 0; s < nsm; ++s) {
+        int h;
+        build_tree(s, 0, desc[s].m, leaves, levels, h);
+    }
+    PTB_HIP(ctx, hipMemsetAsync(Z, 0, abytes, st));
+    DcNode *dleaves;
+    PTB_TRY(upload(ctx, "sm_leaves", leaves, &dleaves));
+    PTB_TRY(dc_leaves(ctx, ddesc, dleaves, (int)leaves.size(), dvec, evec, Z));
+    for (size_t lv = 0; lv < levels.size(); ++lv) {
+        const auto &nodes = levels[lv];
+        int max_n = 0;
+        for (const auto &nd : nodes) max_n = std::max(max_n, nd.hi - nd.lo);
+        std::string nm = "sm_level" + std::to_string(lv);
+        DcNode *dnodes;
+        PTB_TRY(upload(ctx, nm.c_str(), nodes, &dnodes));
+        std::string gm = "sm_lgemm" + std::to_string(lv);
+        auto *dg = static_cast<GemmDesc *>(ws(ctx, gm.c_str(), sizeof(GemmDesc) * nodes.size()));
+        if (!dg) return PTBGPU_E_NOMEM;
+        PTB_TRY(dc_merge_level(ctx, ddesc, dnodes, (int)nodes.size(), max_n, evec, dvec, Z,
+                                    W, X, rot, dg));
+    }
+    // back-transformation Y = Q Z, reflector blocks from the last to the first
+    int max_blocks = 0;
+    for (int s = 0; s < nsm; ++s) {
+        const int nr = desc[s].m - 2;
+        max_blocks = std::max(max_blocks, nr > 0 ? (nr + kNB - 1) / kNB : 0);
+    }
+    if (max_blocks > 0) {
+        std::vector<GemmDesc> g1, g2, g3;
+        std::vector<int> cnt(max_blocks, 0), start(max_blocks + 1, 0);
+        g1.reserve((size_t)nsm * max_blocks);
+        for (int q = 0; q < max_blocks; ++q) {
+            start[q] = (int)g1.size();
+            for (int s = 0; s < nsm; ++s) {
+                const SubDesc &d = desc[s];
+                const int nr = d.m - 2;
+                const int nblk = nr > 0 ? (nr + kNB - 1) / kNB : 0;
+                const int b = nblk - 1 - q;
+                if (b < 0) continue;
+                const int b0 = b * kNB, kb = std::min(kNB, nr - b0);
+                const int rowsv = d.m - b0 - 1;
+                double *V = A + d.off + (int64_t)b0 * d.ld + b0 + 1;
+                double *Y = Z + d.off + b0 + 1;
+                // W1 = V_b^T Y and W2 = T W1 are kb x m panels inside this
+                // submatrix's W and X blocks (leading dimension <= ld)
+                const int lw = std::min(kNB, d.ld);
+                double *W1 = W + d.off, *W2 = X + d.off;
+                const double *T = Tb + (int64_t)s * kNB * kNB;
+                g1.push_back({V, Y, W1, kb, d.m, rowsv, d.ld, d.ld, lw, 1.0, 0.0});
+                g2.push_back({T, W1, W2, kb, d.m, kb, kNB, lw, lw, 1.0, 0.0});
+                g3.push_back({V, W2, Y, rowsv, d.m, kb, d.ld, lw, d.ld, -1.0, 1.0});
+            }
+        }
+        start[max_blocks] = (int)g1.size();
+        GemmDesc *dg1, *dg2, *dg3;
+        PTB_TRY(upload(ctx, "sm_bt1", g1, &dg1));
+        PTB_TRY(upload(ctx, "sm_bt2", g2, &dg2));
+        PTB_TRY(upload(ctx, "sm_bt3", g3, &dg3));
+        for (int q = 0; q < max_blocks; ++q) {
+            const int c = start[q + 1] - start[q];
+            if (c == 0) continue;
+            PTB_TRY(larft_batch(ctx, ddesc, nsm, A, tau, q, kNB, Tb));
+            PTB_TRY(gemm_vbatch(ctx, true, false, dg1 + start[q], c, kNB, max_m));
+            PTB_TRY(gemm_vbatch(ctx, false, false, dg2 + start[q], c, kNB, max_m));
+            PTB_TRY(gemm_vbatch(ctx, false, false, dg3 + start[q], c, max_m, max_m));
+        }
+    }
+    return PTBGPU_OK;
+}
+
+}  // namespace ptb
+
 
 extern "C" {
 

@@ -63,6 +63,11 @@ int larft_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, const double *V,
 int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                 double *A, double *W, double *d, double *e, double *tau);
 
+// sm.hip
+int eig_sym_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                  double *A, double *Z, double *W, double *X, double *dvec, double *evec,
+                  double *tau, double *rot, double *Tb);
+
 // dc.hip
 int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int nleaves,
               double *d, const double *e, double *Z);

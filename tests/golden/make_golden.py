@@ -12,6 +12,8 @@ exist) on small deterministic inputs and stores inputs and outputs as .npz:
   pur_<case>.npz    purification_wrapper (source/purification.f90:28-80) for
                     McWeeny, Pade, Newton-Schulz and sign-diagonalisation with
                     the inverse / inverse-square-root metric, direct and iterative
+  pursearch_<case>  the same with a moving chemical-potential search (guess and
+                    increment set, so bisection and the inner search engage)
   kmeans.npz        clustering_kmeans groups for nsm = 1..4 (flang RANDOM_NUMBER)
 
 Usage: python tests/golden/make_golden.py   (needs oracle/_ref/libptbref.so)
@@ -62,9 +64,13 @@ def parse_mu_trace(log):
 def main():
     if not R.available():
         sys.exit("oracle/_ref/libptbref.so is not built; run make -C oracle first")
-    only = sys.argv[1:]  # optional subset: sm, fermi, pur, kmeans
+    only = sys.argv[1:]  # optional subset: sm, metrics, pur, search, kmeans
+
+    def want(part):
+        return not only or part in only
+
     for name, (kind, size, seed, cols, filt) in SM_CASES.items():
-        if only and "sm" not in only:
+        if not want("sm"):
             break
         mol, H, S = system(kind, size, seed, filt)
         log = []
@@ -76,6 +82,17 @@ def main():
                             eband=R.band_structure(mol.ndim, P, H),
                             nelp=R.get_nel(mol.ndim, P, S))
         print(name, mol.ndim, "muiter", len(trace), "mu", trace[-1, 1])
+    if want("metrics"):
+        metrics_and_fermi()
+    if want("pur"):
+        purification_cases()
+    if want("search"):
+        purification_search_cases()
+    if want("kmeans"):
+        kmeans_groups()
+
+
+def metrics_and_fermi():
     # metrics on the atoms-mode density
     d = np.load(os.path.join(HERE, "sm_h2o8_atoms.npz"))
     n = int(d["ndim"])
@@ -94,6 +111,9 @@ def main():
                         s=s, occ_hot=occ_hot, ef_hot=ef_hot, fod_hot=fod_hot, s_hot=s_hot,
                         focca=fa, foccb=fb, ihomo=np.array([ia, ib]), focca3=fa3, foccb3=fb3,
                         ihomo3=np.array([ia3, ib3]))
+
+
+def purification_cases():
     # purification on two waters (ndim 46), every type and metric
     mol, H, S = system("water", (1, 1, 2), 0, 1e-4)
     for ptype, tname in ((1, "mcweeny"), (2, "pade"), (3, "newton"), (4, "diag")):
@@ -113,6 +133,34 @@ def main():
                                     nel=mol.nel, ndim=mol.ndim, ptype=ptype, metric=metric,
                                     iterative=it, P=P)
                 print("pur", tname, mname, it, "nel", R.get_nel(mol.ndim, P, S))
+
+
+# (type, metric, iterative, chempot guess, increment, chempot cycles): the outer
+# search moves the chemical potential.  For sign diagonalisation the inner
+# search's result flows back into it (chempot is intent(inout),
+# source/purification.f90:659); with 4 chempot cycles the inner search stops
+# short, so the outer loop continues from the inner search's chempot.
+SEARCH_CASES = {
+    "mcweeny_inv_0": (1, 1, 0, -1.5, 0.1, 60),
+    "pade_invsqrt_0": (2, 2, 0, -0.2, 0.05, 60),
+    "newton_inv_1": (3, 1, 1, -1.5, 0.1, 60),
+    "diag_invsqrt_0": (4, 2, 0, -1.5, 0.1, 60),
+    "diag_inv_0": (4, 1, 0, -0.2, 0.05, 60),
+    "diag_invsqrt_0_short": (4, 2, 0, -1.5, 0.1, 4),
+}
+
+
+def purification_search_cases():
+    mol, H, S = system("water", (1, 1, 2), 0, 1e-4)
+    for name, (ptype, metric, it, guess, inc, ccyc) in SEARCH_CASES.items():
+        P = R.purification(mol.ndim, mol.nel, ptype, metric, it, 40, 50, ccyc, guess, inc, H, S)
+        np.savez_compressed(os.path.join(HERE, f"pursearch_{name}.npz"), H=H, S=S, nel=mol.nel,
+                            ndim=mol.ndim, ptype=ptype, metric=metric, iterative=it,
+                            guess=guess, increment=inc, chempot_cycles=ccyc, P=P)
+        print("pursearch", name, "nel", R.get_nel(mol.ndim, P, S))
+
+
+def kmeans_groups():
     # k-means groups of the compiled reference (compiler RNG)
     mol, H, S = system("water", (2, 2, 2), 0, 1e-4)
     out = {}

@@ -87,6 +87,43 @@ def test_purification_matches_reference(fixture):
     assert np.abs(P - d["P"]).max() < 1e-8
 
 
+SEARCH_FIXTURES = sorted(os.path.basename(p)
+                         for p in glob.glob(os.path.join(GOLDEN, "pursearch_*.npz")))
+
+
+@pytest.mark.parametrize("fixture", SEARCH_FIXTURES)
+def test_purification_search_matches_reference(fixture):
+    """A moving chemical-potential search; for sign diagonalisation the inner
+    search's chempot flows back to the outer loop (intent(inout))."""
+    from oracle import ptb_oracle as O
+    d = _load(fixture)
+    pur = O.PurSettings(mode=O.FULL, type=int(d["ptype"]), cycles=40, nel=int(d["nel"]),
+                        metric_type=int(d["metric"]), metric_iterative=bool(d["iterative"]),
+                        metric_cycles=50, chempot_cycles=int(d["chempot_cycles"]),
+                        chempot_guess=float(d["guess"]), chempot_increment=float(d["increment"]))
+    P, info = O.purification_wrapper(pur, int(d["ndim"]), d["H"], d["S"])
+    assert np.abs(P - d["P"]).max() < 1e-9
+
+
+def test_sign_diagonalization_returns_adjusted_chempot():
+    """Planting the old mistake (inner chempot kept local) must change the
+    result of the short-search fixture, whose inner search stops before it
+    converges so that the outer loop goes on from the inner chempot."""
+    from oracle import ptb_oracle as O
+    d = _load("pursearch_diag_invsqrt_0_short.npz")
+    n = int(d["ndim"])
+    pur = O.PurSettings(type=O.SIGN_DIAG, nel=int(d["nel"]), metric_type=O.INV_SQRT,
+                        chempot_cycles=int(d["chempot_cycles"]),
+                        chempot_guess=float(d["guess"]), chempot_increment=float(d["increment"]))
+    real = O.sign_diagonalization
+    try:
+        O.sign_diagonalization = lambda pur_, mu, *a: (real(pur_, mu, *a)[0], mu)
+        P_local, _ = O.purification_wrapper(pur, n, d["H"], d["S"])
+    finally:
+        O.sign_diagonalization = real
+    assert np.abs(P_local - d["P"]).max() > 1e-3
+
+
 def test_iterative_inverse_sqrt_metric_stops_on_nan():
     """The reference ends in ERROR STOP on this pencil (NaN in Newton-Schulz)."""
     from oracle import ptb_oracle as O
