@@ -34,13 +34,21 @@ WORKLOADS = {
 }
 
 
+# bracketed names: whole stages (their GEMMs included) and single kernels
+PHASES = ["potrf", "trsm", "sytrd"]
+KERNELS = ["pattern", "group_union", "group_map", "gather", "potrf_panel", "diag_inv",
+           "transpose", "sytrd_panel", "zero_upper", "dc_leaf", "dc_merge", "gemm", "larft",
+           "fe_rows", "fe_finish", "bisect", "dmat_scatter"]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
 def sytrd_bytes(ms):
-    """Algorithmic HBM bytes of one Householder tridiagonalisation launch: the
-    symmetric trailing matrix (one triangle) read once per reflector."""
+    """Algorithmic HBM bytes of the Householder tridiagonalisation of one build
+    (all panel launches): the symmetric trailing matrix (one triangle) read
+    once per reflector, 8 * sum_k (m-k)(m-k+1)/2 per submatrix."""
     tot = 0.0
     for m in ms:
         n = np.arange(m - 1, 1, -1, dtype=np.float64)  # trailing sizes m-1 .. 2
@@ -49,10 +57,12 @@ def sytrd_bytes(ms):
 
 
 def roofline_model(name, ms, ndim):
-    """(bound, algorithmic units per launch, unit) for a kernel of one build."""
+    """(bound, algorithmic units of one build summed over the kernel's launches,
+    unit); divided by the kernel's bracketed time of one build this is the
+    per-launch figure averaged over its launches."""
     m = np.asarray(ms, dtype=np.float64)
     npk = ndim * (ndim + 1) / 2.0
-    if name == "sytrd":
+    if name == "sytrd_panel":
         return "hbm", sytrd_bytes(ms), "GB/s"
     if name == "gather":   # dense Hsm and Ssm written + packed reads counted once
         return "hbm", float(np.sum(16.0 * m * m)) + 16.0 * npk, "GB/s"
@@ -60,10 +70,6 @@ def roofline_model(name, ms, ndim):
         return "hbm", 16.0 * npk, "GB/s"
     if name == "filter":
         return "hbm", 8.0 * npk, "GB/s"
-    if name in ("potrf",):
-        return "mfma", float(np.sum(m ** 3 / 3.0)), "TFLOP/s"
-    if name in ("trsm",):  # three triangular solves with m right-hand sides
-        return "mfma", float(np.sum(m ** 3)), "TFLOP/s"
     return None
 
 
@@ -121,6 +127,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (calibrates HBM "
+                         "byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the density build "
@@ -131,6 +140,14 @@ def main():
     from ptb_amd import geometry, surrogate
     from ptb_amd.engine import Engine
 
+    if args.copy_mib > 0:
+        src = torch.ones(args.copy_mib * 131072, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
+        del src, dst
     wl = WORKLOADS[args.config]
     mol = geometry.config_molecule(args.config)
     H, S = surrogate.build_hs(mol)
@@ -169,9 +186,7 @@ def main():
         wall = time.perf_counter() - t0
         gpu_ms = ev0.elapsed_time(ev1)
         kern = {}
-        for name in ("pattern", "group_union", "group_map", "gather", "potrf", "trsm",
-                     "transpose", "sytrd", "zero_upper", "dc_leaf", "dc_merge", "gemm", "larft",
-                     "fe_rows", "fe_finish", "bisect", "dmat_scatter"):
+        for name in PHASES + KERNELS:
             ms, cnt = eng.profile_query(name)
             if cnt:
                 kern[name] = (ms / args.steps, cnt // args.steps)
@@ -192,6 +207,8 @@ def main():
     # roofline of the dominant kernel that has a model
     roof = None
     for name, (ms, cnt) in sorted(r["kernels"].items(), key=lambda kv: -kv[1][0]):
+        if name not in KERNELS:
+            continue
         model = roofline_model(name, r["m"], n)
         if model is None:
             continue

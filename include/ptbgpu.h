@@ -162,6 +162,45 @@ int ptbgpu_band_energy(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const dou
 int ptbgpu_idempotency(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const double *S,
                        double *deviation);
 
+/* ---- purification (source/purification.f90) -------------------------------
+ * The fields of tPurificationSet (source/purification_settings.f90:29-86) that
+ * purification_wrapper reads, with the same codes and defaults:
+ *   type 1 mcweeny, 2 sign_iter_pade, 3 sign_iter_newton, 4 sign_diagonalization
+ *   (default 4); metric 1 inv, 2 inv_sqrt (default), 3 sqrt; metric_iterative
+ *   0/1 (default 0); cycles 40, metric_cycles 50, chempot_cycles 60,
+ *   chempot_guess -1.5, chempot_increment 0.1; nel = electrons. */
+typedef struct ptbgpu_pur_settings {
+    int64_t type;
+    int64_t metric;
+    int64_t metric_iterative;
+    int64_t cycles;
+    int64_t metric_cycles;
+    int64_t chempot_cycles;
+    double chempot_guess;
+    double chempot_increment;
+    int64_t nel;
+} ptbgpu_pur_settings;
+
+/* Outcome of the chemical-potential search (source/purification.f90:232-353). */
+typedef struct ptbgpu_pur_report {
+    double chempot;       /* final chemical potential */
+    double nel;           /* tr(P S) of the last purified matrix */
+    int64_t cycles;       /* outer search cycles run */
+    int64_t metric_cycles;/* Newton-Schulz cycles of the iterative metric (0: direct) */
+} ptbgpu_pur_report;
+
+/* Replaces purification_wrapper(pur,ndim,H,S,P) (source/purification.f90:28-80):
+ * packed H, S -> packed P.  Where the reference ends in ERROR STOP (a NaN in
+ * the iterative metric or in the purified matrix, :179-181,491-492,580-581,
+ * 641-643) the call returns PTBGPU_E_NAN.  The direct metric and sign
+ * diagonalisation use the engine's symmetric eigensolver (ndim <= 2048; larger
+ * returns PTBGPU_E_ARG); the iterative metric with McWeeny, Pade or
+ * Newton-Schulz purification has no size limit.  rep may be NULL. */
+int ptbgpu_purify(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *pur, int64_t ndim, const double *H,
+                  const double *S, double *P, ptbgpu_pur_report *rep);
+int ptbgpu_purify_dev(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *pur, int64_t ndim,
+                      const double *dH, const double *dS, double *dP, ptbgpu_pur_report *rep);
+
 #ifdef __cplusplus
 }
 #endif

@@ -422,9 +422,12 @@ int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
     if (!dinv || !ddoff) return PTBGPU_E_NOMEM;
     PTB_HIP(ctx, hipMemcpyAsync(ddoff, doff.data(), sizeof(int64_t) * nsm, hipMemcpyHostToDevice,
                                 ctx->stream));
-    hipLaunchKernelGGL(k_diag_inv, dim3(max_nb, nsm), dim3(64), 0, ctx->stream, ddesc, ddoff, L,
-                       dinv);
-    PTB_CHECK_LAUNCH(ctx);
+    {
+        ProfScope kprof_(ctx, "diag_inv");
+        hipLaunchKernelGGL(k_diag_inv, dim3(max_nb, nsm), dim3(64), 0, ctx->stream, ddesc, ddoff, L,
+                           dinv);
+        PTB_CHECK_LAUNCH(ctx);
+    }
     std::vector<GemmDesc> g1, g2;
     std::vector<int> s1(max_nb + 1), s2(max_nb + 1);
     for (int q = 0; q < max_nb; ++q) {
@@ -498,8 +501,11 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
         PTB_HIP(ctx, hipMemcpyAsync(dg, g.data(), sizeof(GemmDesc) * g.size(), hipMemcpyHostToDevice,
                                     ctx->stream));
     for (int p = 0; p < npanel; ++p) {
-        hipLaunchKernelGGL(k_potrf_panel, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, A, flags, p);
-        PTB_CHECK_LAUNCH(ctx);
+        {
+            ProfScope kprof_(ctx, "potrf_panel");
+            hipLaunchKernelGGL(k_potrf_panel, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, A, flags, p);
+            PTB_CHECK_LAUNCH(ctx);
+        }
         const int c = start[p + 1] - start[p];
         if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dg + start[p], c, max_m, max_m));
     }

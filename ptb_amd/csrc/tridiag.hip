@@ -286,9 +286,12 @@ int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
                                     hipMemcpyHostToDevice, ctx->stream));
     }
     for (int p = 0; p < npanel; ++p) {
-        hipLaunchKernelGGL(k_sytrd_panel, dim3(nsm), dim3(kT), shmem, ctx->stream, ddesc, A, W, d,
-                           e, tau, p);
-        PTB_CHECK_LAUNCH(ctx);
+        {
+            ProfScope kprof_(ctx, "sytrd_panel");
+            hipLaunchKernelGGL(k_sytrd_panel, dim3(nsm), dim3(kT), shmem, ctx->stream, ddesc, A, W,
+                               d, e, tau, p);
+            PTB_CHECK_LAUNCH(ctx);
+        }
         const int c = start[p + 1] - start[p];
         if (c > 0) {  // two launches: both products update the same trailing block
             PTB_TRY(gemm_vbatch(ctx, false, true, dg1 + start[p], c, max_m, max_m));

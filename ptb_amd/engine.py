@@ -8,6 +8,8 @@ Names, argument meaning and error behaviour follow the reference routines:
   submatrix_columns, submatrix_mode)``     source/sm.f90:34-409 (Pvec overwritten)
 * ``get_nel``, ``band_structure``, ``idempotent``, ``check_density``
                                            source/metrics.f90
+* ``purification_wrapper(pur, ndim, H, S, P)``
+                                           source/purification.f90:28-80
 
 Where the reference stops the process (unsupported mode, k-means fault, NaN),
 the engine returns a status code and these wrappers raise ``PtbGpuError``.
@@ -55,6 +57,31 @@ class Report(ctypes.Structure):
         return d
 
 
+class PurSettings(ctypes.Structure):
+    """tPurificationSet fields read by purification_wrapper, with the reference's
+    defaults (source/purification_settings.f90:29-86)."""
+    _fields_ = [("type", I64), ("metric", I64), ("metric_iterative", I64), ("cycles", I64),
+                ("metric_cycles", I64), ("chempot_cycles", I64), ("chempot_guess", F64),
+                ("chempot_increment", F64), ("nel", I64)]
+
+    def __init__(self, type=4, metric=2, metric_iterative=False, cycles=40, metric_cycles=50,
+                 chempot_cycles=60, chempot_guess=-1.5, chempot_increment=0.1, nel=0):
+        super().__init__(int(type), int(metric), 1 if metric_iterative else 0, int(cycles),
+                         int(metric_cycles), int(chempot_cycles), float(chempot_guess),
+                         float(chempot_increment), int(nel))
+
+
+class PurReport(ctypes.Structure):
+    _fields_ = [("chempot", F64), ("nel", F64), ("cycles", I64), ("metric_cycles", I64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+PUR_TYPES = {"mcweeny": 1, "sign_iter_pade": 2, "sign_iter_newton": 3, "sign_diagonalization": 4}
+PUR_METRICS = {"inv": 1, "inv_sqrt": 2, "sqrt": 3}
+
+
 def _lib():
     lib = _native.engine_lib()
     if getattr(lib, "_ptb_typed", False):
@@ -84,6 +111,10 @@ def _lib():
     for name in ("ptbgpu_get_nel", "ptbgpu_band_energy", "ptbgpu_idempotency"):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [VP, I64, VP, VP, ctypes.POINTER(F64)]
+    for name in ("ptbgpu_purify", "ptbgpu_purify_dev"):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = [VP, ctypes.POINTER(PurSettings), I64, VP, VP, VP,
+                                           ctypes.POINTER(PurReport)]
     lib._ptb_typed = True
     return lib
 
@@ -228,6 +259,28 @@ class Engine:
         self._check(self.lib.ptbgpu_sm_spectra(self.ctx, nsm, sum_m, _ptr(m), _ptr(e), _ptr(fe)))
         cuts = np.cumsum(m)[:-1]
         return m, np.split(e, cuts), np.split(fe, cuts)
+
+    # ---- purification --------------------------------------------------------
+    def purification_wrapper(self, pur: PurSettings, ndim: int, H, S, P) -> dict:
+        """Drop-in for purification_wrapper(pur, ndim, H, S, P)
+        (source/purification.f90:28-80) on packed host arrays; P is overwritten.
+        Where the reference ends in ERROR STOP (NaN) this raises PtbGpuError
+        with status 4."""
+        npk = ndim * (ndim + 1) // 2
+        _f64(H, npk, "H")
+        _f64(S, npk, "S")
+        _f64(P, npk, "P")
+        rep = PurReport()
+        self._check(self.lib.ptbgpu_purify(self.ctx, ctypes.byref(pur), ndim, _ptr(H), _ptr(S),
+                                           _ptr(P), ctypes.byref(rep)))
+        return rep.as_dict()
+
+    def purification_wrapper_dev(self, pur: PurSettings, ndim: int, dH, dS, dP) -> dict:
+        """Same with device-resident packed H, S, P (torch CUDA tensors)."""
+        rep = PurReport()
+        self._check(self.lib.ptbgpu_purify_dev(self.ctx, ctypes.byref(pur), ndim, _ptr(dH),
+                                               _ptr(dS), _ptr(dP), ctypes.byref(rep)))
+        return rep.as_dict()
 
     # ---- metrics -----------------------------------------------------------
     def _scalar(self, fn, ndim, A, B):
