@@ -5,14 +5,15 @@
 //               earlier in the stream can size the next product).  64x64 output
 //               tile per workgroup, 16-deep k slices staged in LDS, 4x4
 //               accumulators per thread.
-// * k_potrf:    lower Cholesky S = L L^T in place, blocked by 32: the diagonal
-//               block is factored in LDS, the panel by forward substitution in
-//               registers, the trailing lower triangle by 32x32 LDS tiles.  A
+// * k_potrf_panel: one 32-column panel of the lower Cholesky S = L L^T: the
+//               diagonal block factored and inverted in LDS, the panel below as
+//               X = P L_kk^-T; the trailing update is a batched GEMM.  A
 //               non-positive pivot sets flags bit 0 (dsygvd would report INFO>0).
-// * k_trsm:     X = L^-1 B or L^-T B in place, one workgroup per 32 right-hand
-//               sides of one submatrix.
-// * k_transpose, k_zero_upper, k_larft: helpers for K = L^-1 H L^-T and for the
-//               blocked (WY) application of the Householder reflectors.
+// * k_diag_inv: inverses of the 64x64 diagonal blocks of L, for the blocked
+//               triangular solves (one GEMM update + one GEMM with the
+//               inverted block per 64 rows).
+// * k_transpose, k_zero_upper, k_larft_all: helpers for K = L^-1 H L^-T and for
+//               the blocked (WY) application of the Householder reflectors.
 #include "sm_internal.hpp"
 
 namespace {
@@ -162,99 +163,6 @@ __global__ __launch_bounds__(kT) void k_potrf_panel(const SubDesc *__restrict__ 
     if (threadIdx.x == 0 && bad) flags[blockIdx.x] |= 1;
 }
 
-// In place B <- L^-1 B (TRANS=false) or L^-T B (TRANS=true), L lower (only the
-// lower triangle is read).  grid.x: blocks of 32 right-hand sides, grid.y: submatrix.
-template <bool TRANS>
-__global__ __launch_bounds__(kT) void k_trsm(const SubDesc *__restrict__ desc,
-                                             const double *__restrict__ La,
-                                             double *__restrict__ Ba) {
-    const SubDesc d = desc[blockIdx.y];
-    const int rb = blockIdx.x * 32;
-    if (rb >= d.m) return;
-    const double *L = La + d.off;
-    double *B = Ba + d.off;
-    const int m = d.m, ld = d.ld;
-    const int nbk = (m + 31) / 32;
-    __shared__ double Lt[32][33], Xt[32][33], Bt[32][33];
-    for (int step = 0; step < nbk; ++step) {
-        const int ib = TRANS ? nbk - 1 - step : step;
-        const int i0 = ib * 32;
-        double acc[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int l = threadIdx.x + q * kT, r = l & 31, c = l >> 5;
-            acc[q] = (i0 + r < m && rb + c < m) ? B[(int64_t)(rb + c) * ld + i0 + r] : 0.0;
-        }
-        const int jlo = TRANS ? ib + 1 : 0, jhi = TRANS ? nbk : ib;
-        for (int jb = jlo; jb < jhi; ++jb) {
-            const int j0 = jb * 32;
-            __syncthreads();
-            for (int l = threadIdx.x; l < 1024; l += kT) {
-                const int r = l & 31, c = l >> 5;
-                double lv = 0.0;
-                if (!TRANS) {  // L(i0+r, j0+c)
-                    if (i0 + r < m && j0 + c < m) lv = L[(int64_t)(j0 + c) * ld + i0 + r];
-                } else {       // L^T(i0+r, j0+c) = L(j0+c, i0+r)
-                    if (i0 + r < m && j0 + c < m) lv = L[(int64_t)(i0 + r) * ld + j0 + c];
-                }
-                Lt[r][c] = lv;
-                Xt[r][c] = (j0 + r < m && rb + c < m) ? B[(int64_t)(rb + c) * ld + j0 + r] : 0.0;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int l = threadIdx.x + q * kT, r = l & 31, c = l >> 5;
-                double s = 0.0;
-#pragma unroll 8
-                for (int t = 0; t < 32; ++t) s += Lt[r][t] * Xt[t][c];
-                acc[q] -= s;
-            }
-        }
-        __syncthreads();
-        // diagonal block of L (identity padding) and the right-hand sides
-        for (int l = threadIdx.x; l < 1024; l += kT) {
-            const int r = l & 31, c = l >> 5;
-            double lv;
-            if (i0 + r < m && i0 + c < m) {
-                lv = !TRANS ? (r >= c ? L[(int64_t)(i0 + c) * ld + i0 + r] : 0.0)
-                            : (c >= r ? L[(int64_t)(i0 + r) * ld + i0 + c] : 0.0);
-            } else {
-                lv = (r == c) ? 1.0 : 0.0;
-            }
-            Lt[r][c] = lv;  // Lt = diag block of op(L)
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int l = threadIdx.x + q * kT, r = l & 31, c = l >> 5;
-            Bt[r][c] = acc[q];
-        }
-        __syncthreads();
-        if (threadIdx.x < 32) {
-            const int c = threadIdx.x;
-            if (!TRANS) {
-                for (int r = 0; r < 32; ++r) {
-                    double s = Bt[r][c];
-                    for (int t = 0; t < r; ++t) s -= Lt[r][t] * Bt[t][c];
-                    Bt[r][c] = s / Lt[r][r];
-                }
-            } else {
-                for (int r = 31; r >= 0; --r) {
-                    double s = Bt[r][c];
-                    for (int t = r + 1; t < 32; ++t) s -= Lt[r][t] * Bt[t][c];
-                    Bt[r][c] = s / Lt[r][r];
-                }
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int l = threadIdx.x + q * kT, r = l & 31, c = l >> 5;
-            if (i0 + r < m && rb + c < m) B[(int64_t)(rb + c) * ld + i0 + r] = Bt[r][c];
-        }
-        __syncthreads();
-    }
-}
-
 // out = in^T (or 0.5 (in + in^T)); grid.x: 32x32 tiles, grid.y: submatrix
 __global__ __launch_bounds__(kT) void k_transpose(const SubDesc *__restrict__ desc,
                                                   const double *__restrict__ ina,
@@ -291,36 +199,46 @@ __global__ __launch_bounds__(kT) void k_zero_upper(const SubDesc *__restrict__ d
     for (int r = threadIdx.x; r <= c; r += kT) col[r] = 0.0;
 }
 
-// T (nb x nb, upper) with H_b0 ... H_{b0+nb-1} = I - V T V^T for the reflectors
-// in columns b0.. of V (zero above row k+1, unit at row k+1).  One workgroup per
-// submatrix; submatrices without a block b0 skip.  T stored nb x nb per submatrix
-// at T + s*nb*nb (column-major).
-__global__ __launch_bounds__(kT) void k_larft(const SubDesc *__restrict__ desc,
-                                              const double *__restrict__ Va,
-                                              const double *__restrict__ tau_all, int q, int nb,
-                                              double *__restrict__ Ta) {
-    const SubDesc d = desc[blockIdx.x];
-    const int nr = d.m - 2;  // number of reflectors
-    double *T = Ta + (int64_t)blockIdx.x * nb * nb;
-    // q-th block counted from the last one
-    const int nblk = nr > 0 ? (nr + nb - 1) / nb : 0;
-    const int b = nblk - 1 - q;
-    if (b < 0) return;
-    const int b0 = b * nb;
-    const int kb = min(nb, nr - b0);
+// T factor of every reflector block of every submatrix in one launch (grid:
+// block b counted from the first, submatrix).  Q_b = H_b0 ... H_b0+kb-1 =
+// I - V_b T_b V_b^T with T_b upper triangular (LAPACK dlarft, forward,
+// columnwise).  G(i,j) = v_i^T v_j is one wavefront dot product per pair.
+// T_b is stored 32 x 32 column-major at Ta + toff[s] + b * 1024.
+__global__ __launch_bounds__(kT) void k_larft_all(const SubDesc *__restrict__ desc,
+                                                  const double *__restrict__ Va,
+                                                  const double *__restrict__ tau_all,
+                                                  const int64_t *__restrict__ toff,
+                                                  double *__restrict__ Ta) {
+    const SubDesc d = desc[blockIdx.y];
+    const int nr = d.m - 2;
+    const int nblk = nr > 0 ? (nr + 31) / 32 : 0;
+    const int b = blockIdx.x;
+    if (b >= nblk) return;
+    const int b0 = b * 32;
+    const int kb = min(32, nr - b0);
     const double *V = Va + d.off + (int64_t)b0 * d.ld;
     const double *tau = tau_all + d.mapoff + b0;
-    __shared__ double G[32][33];   // G(i,j) = v_i^T v_j, j < i
+    double *T = Ta + toff[blockIdx.y] + (int64_t)b * 1024;
+    __shared__ double G[32][33];
     __shared__ double Ts[32][33];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int l = threadIdx.x; l < 1024; l += kT) {
-        const int i = l & 31, j = l >> 5;
+        G[l & 31][l >> 5] = 0.0;
+        Ts[l & 31][l >> 5] = 0.0;
+    }
+    __syncthreads();
+    // pairs (i, j), j < i < kb, enumerated as p = i (i - 1) / 2 + j
+    const int npair = kb * (kb - 1) / 2;
+    for (int p = wid; p < npair; p += kT / 64) {
+        int i = 1;
+        while ((i + 1) * i / 2 <= p) ++i;
+        const int j = p - i * (i - 1) / 2;
+        const double *vi = V + (int64_t)i * d.ld, *vj = V + (int64_t)j * d.ld;
         double s = 0.0;
-        if (j < i && i < kb) {
-            const double *vi = V + (int64_t)i * d.ld, *vj = V + (int64_t)j * d.ld;
-            for (int r = b0 + i + 1; r < d.m; ++r) s += vj[r] * vi[r];
-        }
-        G[i][j] = s;
-        Ts[i][j] = 0.0;
+        // v_i is zero above row b0 + i + 1 (its stored unit)
+        for (int r = b0 + i + 1 + lane; r < d.m; r += 64) s += vj[r] * vi[r];
+        s = wave_sum(s);
+        if (lane == 0) G[i][j] = s;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -334,9 +252,9 @@ __global__ __launch_bounds__(kT) void k_larft(const SubDesc *__restrict__ desc,
         }
     }
     __syncthreads();
-    for (int l = threadIdx.x; l < nb * nb; l += kT) {
-        const int r = l % nb, c = l / nb;
-        T[(int64_t)c * nb + r] = (r < 32 && c < 32) ? Ts[r][c] : 0.0;
+    for (int l = threadIdx.x; l < 1024; l += kT) {
+        const int r = l & 31, c = l >> 5;
+        T[(int64_t)c * 32 + r] = Ts[r][c];
     }
 }
 
@@ -404,7 +322,8 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
 // 64 rows: per block row one update GEMM (in place on B) and one product with
 // the inverted diagonal block (into X).  B is overwritten.
 int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
-              const double *L, double *B, double *X, bool trans, const char *tag) {
+              const double *L, double *B, double *X, bool trans, const char *tag,
+              const RhsLayout *rhs) {
     ProfScope prof_(ctx, "trsm");
     const int nsm = (int)desc.size();
     std::vector<int64_t> doff(nsm);
@@ -439,19 +358,21 @@ int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
             if (q >= nb) continue;
             const int ib = trans ? nb - 1 - q : q;
             const int i0 = ib * 64, ibs = std::min(64, d.m - i0);
+            const int nc = rhs ? rhs->cols[s] : d.m;
+            if (nc <= 0) continue;
             const double *Lb = L + d.off;
-            double *Bb = B + d.off, *Xb = X + d.off;
+            double *Bb = B + (rhs ? rhs->boff[s] : d.off), *Xb = X + (rhs ? rhs->xoff[s] : d.off);
             const double *Di = dinv + doff[s] + (int64_t)ib * 4096;
             if (!trans) {
                 if (i0 > 0)
-                    g1.push_back({Lb + i0, Xb, Bb + i0, ibs, d.m, i0, d.ld, d.ld, d.ld, -1.0, 1.0});
-                g2.push_back({Di, Bb + i0, Xb + i0, ibs, d.m, ibs, 64, d.ld, d.ld, 1.0, 0.0});
+                    g1.push_back({Lb + i0, Xb, Bb + i0, ibs, nc, i0, d.ld, d.ld, d.ld, -1.0, 1.0});
+                g2.push_back({Di, Bb + i0, Xb + i0, ibs, nc, ibs, 64, d.ld, d.ld, 1.0, 0.0});
             } else {
                 const int rest = d.m - i0 - ibs;
                 if (rest > 0)
-                    g1.push_back({Lb + (int64_t)i0 * d.ld + i0 + ibs, Xb + i0 + ibs, Bb + i0, ibs, d.m,
+                    g1.push_back({Lb + (int64_t)i0 * d.ld + i0 + ibs, Xb + i0 + ibs, Bb + i0, ibs, nc,
                                   rest, d.ld, d.ld, d.ld, -1.0, 1.0});
-                g2.push_back({Di, Bb + i0, Xb + i0, ibs, d.m, ibs, 64, d.ld, d.ld, 1.0, 0.0});
+                g2.push_back({Di, Bb + i0, Xb + i0, ibs, nc, ibs, 64, d.ld, d.ld, 1.0, 0.0});
             }
         }
     }
@@ -512,16 +433,6 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
     return PTBGPU_OK;
 }
 
-int trsm_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *L,
-               double *B, bool trans) {
-    ProfScope prof_(ctx, "trsm");
-    dim3 grid((max_m + 31) / 32, nsm);
-    if (trans) hipLaunchKernelGGL(k_trsm<true>, grid, dim3(kT), 0, ctx->stream, ddesc, L, B);
-    else hipLaunchKernelGGL(k_trsm<false>, grid, dim3(kT), 0, ctx->stream, ddesc, L, B);
-    PTB_CHECK_LAUNCH(ctx);
-    return PTBGPU_OK;
-}
-
 int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m,
                         const double *in, double *out, bool symmetrize) {
     ProfScope prof_(ctx, "transpose");
@@ -539,11 +450,99 @@ int zero_upper_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, 
     return PTBGPU_OK;
 }
 
-int larft_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, const double *V,
-                const double *tau, int q, int nb, double *T) {
+int larft_all(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+              const double *V, const double *tau, QFactor &qf) {
     ProfScope prof_(ctx, "larft");
-    hipLaunchKernelGGL(k_larft, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, V, tau, q, nb, T);
-    PTB_CHECK_LAUNCH(ctx);
+    const int nsm = (int)desc.size();
+    qf.toff.assign(nsm, 0);
+    int64_t tot = 0;
+    int maxblk = 0;
+    for (int s = 0; s < nsm; ++s) {
+        const int nr = desc[s].m - 2;
+        const int nblk = nr > 0 ? (nr + 31) / 32 : 0;
+        qf.toff[s] = tot;
+        tot += (int64_t)nblk * 1024;
+        maxblk = std::max(maxblk, nblk);
+    }
+    qf.V = V;
+    qf.maxblk = maxblk;
+    qf.T = static_cast<double *>(ws(ctx, "q_T", sizeof(double) * std::max<int64_t>(tot, 1)));
+    auto *dtoff = static_cast<int64_t *>(ws(ctx, "q_toff", sizeof(int64_t) * nsm));
+    if (!qf.T || !dtoff) return PTBGPU_E_NOMEM;
+    PTB_HIP(ctx, hipMemcpyAsync(dtoff, qf.toff.data(), sizeof(int64_t) * nsm, hipMemcpyHostToDevice,
+                                ctx->stream));
+    if (maxblk > 0) {
+        hipLaunchKernelGGL(k_larft_all, dim3(maxblk, nsm), dim3(kT), 0, ctx->stream, ddesc, V, tau,
+                           dtoff, qf.T);
+        PTB_CHECK_LAUNCH(ctx);
+    }
+    return PTBGPU_OK;
+}
+
+// Y <- Q Y (trans = false; blocks from the last to the first) or Y <- Q^T Y
+// (trans = true; blocks from the first, Q_b^T = I - V_b T_b^T V_b^T) for
+// ncols[s] columns of submatrix s at Y + yoff[s] (leading dimension ld).  Per
+// block: W1 = V_b^T Y, W2 = T_b W1 (or T_b^T W1), Y -= V_b W2, each a batched
+// GEMM; W1 and W2 are 32 x ncols panels in engine workspace.
+int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf, double *Y,
+            const std::vector<int64_t> &yoff, const std::vector<int> &ncols, bool trans,
+            const char *tag) {
+    ProfScope prof_(ctx, "apply_q");
+    const int nsm = (int)desc.size();
+    if (qf.maxblk == 0) return PTBGPU_OK;
+    std::vector<int64_t> woff(nsm);
+    int64_t wtot = 0;
+    int max_m = 0, max_c = 0;
+    for (int s = 0; s < nsm; ++s) {
+        woff[s] = wtot;
+        wtot += 32 * (int64_t)std::max(ncols[s], 0);
+        max_m = std::max(max_m, desc[s].m);
+        max_c = std::max(max_c, ncols[s]);
+    }
+    if (max_c == 0) return PTBGPU_OK;
+    auto *W1 = static_cast<double *>(ws(ctx, "q_w1", sizeof(double) * std::max<int64_t>(wtot, 1)));
+    auto *W2 = static_cast<double *>(ws(ctx, "q_w2", sizeof(double) * std::max<int64_t>(wtot, 1)));
+    if (!W1 || !W2) return PTBGPU_E_NOMEM;
+    std::vector<GemmDesc> g1, g2, g3;
+    std::vector<int> start(qf.maxblk + 1, 0);
+    for (int q = 0; q < qf.maxblk; ++q) {
+        start[q] = (int)g1.size();
+        for (int s = 0; s < nsm; ++s) {
+            const SubDesc &d = desc[s];
+            const int nc = ncols[s];
+            const int nr = d.m - 2;
+            const int nblk = nr > 0 ? (nr + 31) / 32 : 0;
+            if (nc <= 0 || q >= nblk) continue;
+            const int b = trans ? q : nblk - 1 - q;
+            const int b0 = b * 32, kb = std::min(32, nr - b0);
+            const int rowsv = d.m - b0 - 1;
+            const double *V = qf.V + d.off + (int64_t)b0 * d.ld + b0 + 1;
+            double *Yb = Y + yoff[s] + b0 + 1;
+            const double *T = qf.T + qf.toff[s] + (int64_t)b * 1024;
+            double *w1 = W1 + woff[s], *w2 = W2 + woff[s];
+            g1.push_back({V, Yb, w1, kb, nc, rowsv, d.ld, d.ld, 32, 1.0, 0.0});
+            g2.push_back({T, w1, w2, kb, nc, kb, 32, 32, 32, 1.0, 0.0});
+            g3.push_back({V, w2, Yb, rowsv, nc, kb, d.ld, 32, d.ld, -1.0, 1.0});
+        }
+    }
+    start[qf.maxblk] = (int)g1.size();
+    const std::string t(tag);
+    GemmDesc *dg1 = static_cast<GemmDesc *>(ws(ctx, ("q_g1_" + t).c_str(), sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1)));
+    GemmDesc *dg2 = static_cast<GemmDesc *>(ws(ctx, ("q_g2_" + t).c_str(), sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1)));
+    GemmDesc *dg3 = static_cast<GemmDesc *>(ws(ctx, ("q_g3_" + t).c_str(), sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1)));
+    if (!dg1 || !dg2 || !dg3) return PTBGPU_E_NOMEM;
+    if (g1.empty()) return PTBGPU_OK;
+    const size_t gb = sizeof(GemmDesc) * g1.size();
+    PTB_HIP(ctx, hipMemcpyAsync(dg1, g1.data(), gb, hipMemcpyHostToDevice, ctx->stream));
+    PTB_HIP(ctx, hipMemcpyAsync(dg2, g2.data(), gb, hipMemcpyHostToDevice, ctx->stream));
+    PTB_HIP(ctx, hipMemcpyAsync(dg3, g3.data(), gb, hipMemcpyHostToDevice, ctx->stream));
+    for (int q = 0; q < qf.maxblk; ++q) {
+        const int c = start[q + 1] - start[q];
+        if (c == 0) continue;
+        PTB_TRY(gemm_vbatch(ctx, true, false, dg1 + start[q], c, 32, max_c));
+        PTB_TRY(gemm_vbatch(ctx, trans, false, dg2 + start[q], c, 32, max_c));
+        PTB_TRY(gemm_vbatch(ctx, false, false, dg3 + start[q], c, max_m, max_c));
+    }
     return PTBGPU_OK;
 }
 

@@ -1,11 +1,12 @@
 // Electron-count weights, chemical-potential bisection and the owned-column
 // density scatter (source/sm.f90:304-362, :190-198, :365-383, :387).
 //
-// fe(k) = sum_{owned rows i} (Ssm U)(i,k) U(i,k).  With Ssm = L L^T and
-// U = L^-T Y this is sum_i (L Y)(i,k) U(i,k), so the overlap block is not
-// needed after the Cholesky: fe_rows computes (L Y)(owned, :) from L and Y
-// before the back-substitution, fe_finish multiplies with U(owned, :) after it,
-// accumulating owned rows in the reference's order.
+// fe(k) = sum_{owned rows i} (Ssm U)(i,k) U(i,k).  With Ssm = L L^T,
+// U = L^-T Q Z (Q the Householder product, Z the tridiagonal eigenvectors) the
+// owned rows are (L Q Z)(O,:) and (L^-T Q Z)(O,:): 2c row vectors
+// [L(O,:); (L^-1 e_O)^T] are pushed through Q and Z (c = owned columns), so no
+// full back-transformation is needed for fe; owned rows accumulate in the
+// reference's order.
 //
 // The bisection runs in one 1024-thread workgroup over every (eigenvalue, fe)
 // pair of the batch with fixed-order reductions, so it is deterministic; it
@@ -15,8 +16,10 @@
 // more pass at the same mu.
 //
 // The density of owned column j is P(:, j) = U diag(occ) U(loc_j, :)^T over the
-// occupied states only; packed P (zeroed first) receives rows map(a) <= j, which
-// is the reference's packsym of P(map, comb) with no symmetrisation.
+// occupied states only, so after the bisection only the occupied eigenvector
+// columns are compacted, back-transformed and solved with L^-T; packed P
+// (zeroed first) receives rows map(a) <= j, which is the reference's packsym of
+// P(map, comb) with no symmetrisation.
 #include "sm_internal.hpp"
 
 namespace {
@@ -24,63 +27,47 @@ namespace {
 constexpr int kT = 256;
 constexpr int kG = 8;  // owned columns per pass
 
-// Lown[g][t] = L(oloc_g, t) for t <= oloc_g (0 above): contiguous owned rows of L
-__global__ __launch_bounds__(kT) void k_own_rows(const SubDesc *__restrict__ desc,
-                                                 const int *__restrict__ oloc,
-                                                 const double *__restrict__ La, int rowld,
-                                                 double *__restrict__ Lown) {
+// Right-hand sides of the owned-row projections, per submatrix with c owned
+// columns (local rows o_g = oloc[g]):
+//   XA(:, g)     = L(o_g, :)^T            (row o_g of L, zero past o_g)
+//   XB(:, g)     = e_{o_g}                (solved to L^-1 e_{o_g} into XA(:, c + g))
+// grid: (owned column, submatrix)
+__global__ __launch_bounds__(kT) void k_xall_build(const SubDesc *__restrict__ desc,
+                                                   const int *__restrict__ oloc,
+                                                   const double *__restrict__ La,
+                                                   const int64_t *__restrict__ xoff,
+                                                   const int64_t *__restrict__ xboff,
+                                                   double *__restrict__ XA,
+                                                   double *__restrict__ XB) {
     const SubDesc d = desc[blockIdx.y];
     const int g = d.gbeg + blockIdx.x;
     if (g >= d.gend) return;
     const int row = oloc[g];
     const double *L = La + d.off;
-    double *out = Lown + (int64_t)g * rowld;
-    for (int t = threadIdx.x; t < d.m; t += kT) out[t] = t <= row ? L[(int64_t)t * d.ld + row] : 0.0;
-}
-
-// SY[g][k] = sum_t Lown[g][t] Y(t, k): one wavefront per column k, kG owned rows
-// at a time
-__global__ __launch_bounds__(kT) void k_fe_rows(const SubDesc *__restrict__ desc,
-                                                const double *__restrict__ Lown, int rowld,
-                                                const double *__restrict__ Ya,
-                                                double *__restrict__ SY) {
-    const SubDesc d = desc[blockIdx.y];
-    const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
-    const int k = blockIdx.x * (kT / PTB_WAVE) + wid;
-    if (k >= d.m) return;
-    const double *Y = Ya + d.off + (int64_t)k * d.ld;
-    for (int g0 = d.gbeg; g0 < d.gend; g0 += kG) {
-        double acc[kG];
-#pragma unroll
-        for (int q = 0; q < kG; ++q) acc[q] = 0.0;
-        for (int t = lane; t < d.m; t += PTB_WAVE) {
-            const double y = Y[t];
-#pragma unroll
-            for (int q = 0; q < kG; ++q)
-                if (g0 + q < d.gend) acc[q] += Lown[(int64_t)(g0 + q) * rowld + t] * y;
-        }
-#pragma unroll
-        for (int q = 0; q < kG; ++q) {
-            const double s = wave_sum(acc[q]);
-            if (lane == 0 && g0 + q < d.gend) SY[(int64_t)(g0 + q) * rowld + k] = s;
-        }
+    double *xa = XA + xoff[blockIdx.y] + (int64_t)blockIdx.x * d.ld;
+    double *xb = XB + xboff[blockIdx.y] + (int64_t)blockIdx.x * d.ld;
+    for (int t = threadIdx.x; t < d.m; t += kT) {
+        xa[t] = t <= row ? L[(int64_t)t * d.ld + row] : 0.0;
+        xb[t] = t == row ? 1.0 : 0.0;
     }
 }
 
-// fe(k) = sum_g SY[g][k] U(oloc_g, k) in owned-column order; zero for a
-// submatrix whose overlap block failed the Cholesky
-__global__ __launch_bounds__(kT) void k_fe_finish(const SubDesc *__restrict__ desc,
-                                                  const int *__restrict__ oloc,
-                                                  const double *__restrict__ SY, int rowld,
-                                                  const double *__restrict__ Ua,
-                                                  double *__restrict__ fe_all) {
+// fe(k) = sum_g F(k, g) F(k, c + g) over owned columns in order, with
+// F = Z^T Q^T [L(O,:)^T, L^-1 e_O]: F(k, g) = (L Y)(o_g, k) = (Ssm U)(o_g, k) and
+// F(k, c + g) = (L^-T Y)(o_g, k) = U(o_g, k).  Zero for a submatrix whose
+// overlap block failed the Cholesky.
+__global__ __launch_bounds__(kT) void k_fe_proj(const SubDesc *__restrict__ desc,
+                                                const int64_t *__restrict__ xoff,
+                                                const double *__restrict__ F,
+                                                double *__restrict__ fe_all) {
     const SubDesc d = desc[blockIdx.y];
     const int k = blockIdx.x * kT + threadIdx.x;
     if (k >= d.m) return;
-    const double *U = Ua + d.off + (int64_t)k * d.ld;
+    const int c = d.gend - d.gbeg;
+    const double *f = F + xoff[blockIdx.y] + k;
     double s = 0.0;
     if (!(d.flags & 1))
-        for (int g = d.gbeg; g < d.gend; ++g) s += SY[(int64_t)g * rowld + k] * U[oloc[g]];
+        for (int g = 0; g < c; ++g) s += f[(int64_t)g * d.ld] * f[(int64_t)(c + g) * d.ld];
     fe_all[d.mapoff + k] = s;
 }
 
@@ -131,50 +118,94 @@ __global__ __launch_bounds__(kBT) void k_bisect(const double *__restrict__ e,
     }
 }
 
-// grid: (submatrix, chunk of kG owned columns)
-__global__ __launch_bounds__(kT) void k_dmat_scatter(const SubDesc *__restrict__ desc,
+// Occupied states of each submatrix at the final mu, in ascending state
+// order: weights occ(e_k) (2, or 1 for e_k == mu) into wocc at the submatrix's
+// row offset, their count into nocc, and the eigenvector columns Z(:, k) copied
+// to the first nocc columns of Zo.  One workgroup per submatrix; a submatrix
+// whose overlap block failed the Cholesky contributes no states.
+__global__ __launch_bounds__(kT) void k_occ_compact(const SubDesc *__restrict__ desc,
+                                                    const double *__restrict__ eig_all,
+                                                    const double *__restrict__ mu_p,
+                                                    const double *__restrict__ Za,
+                                                    double *__restrict__ Zo,
+                                                    double *__restrict__ wocc,
+                                                    int *__restrict__ nocc) {
+    extern __shared__ int occk[];  // m
+    __shared__ int wtot[kT / PTB_WAVE];
+    __shared__ int base;
+    const SubDesc d = desc[blockIdx.x];
+    const double mu = mu_p[0];
+    const double *eig = eig_all + d.mapoff;
+    const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    for (int k0 = 0; k0 < d.m; k0 += kT) {
+        const int k = k0 + threadIdx.x;
+        const double w = (k < d.m && !(d.flags & 1)) ? occ_of(eig[k], mu) : 0.0;
+        const unsigned long long mask = __ballot(w != 0.0);
+        const int before = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) wtot[wid] = __popcll(mask);
+        __syncthreads();
+        int off = base;
+        for (int q = 0; q < wid; ++q) off += wtot[q];
+        if (w != 0.0) {
+            occk[off + before] = k;
+            wocc[d.mapoff + off + before] = w;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int q = 0; q < kT / PTB_WAVE; ++q) t += wtot[q];
+            base += t;
+        }
+        __syncthreads();
+    }
+    const int n = base;
+    if (threadIdx.x == 0) nocc[blockIdx.x] = n;
+    const double *Z = Za + d.off;
+    double *Y = Zo + d.off;
+    for (int t = 0; t < n; ++t) {
+        const double *src = Z + (int64_t)occk[t] * d.ld;
+        double *dst = Y + (int64_t)t * d.ld;
+        for (int r = threadIdx.x; r < d.m; r += kT) dst[r] = src[r];
+    }
+}
+
+// P(:, j) = U_o diag(w) U_o(loc_j, :)^T for the owned columns j, with U_o the
+// nocc occupied generalized eigenvectors (first columns of the arena block);
+// packed P (zeroed first) receives rows map(a) <= j, the reference's packsym of
+// P(map, comb) with no symmetrisation.  grid: (submatrix, chunk of kG owned
+// columns).
+__global__ __launch_bounds__(kT) void k_dmat_compact(const SubDesc *__restrict__ desc,
                                                      const int *__restrict__ gcols,
                                                      const int *__restrict__ oloc,
                                                      const int *__restrict__ map_all,
-                                                     const double *__restrict__ eig_all,
+                                                     const double *__restrict__ wocc,
+                                                     const int *__restrict__ nocc,
                                                      const double *__restrict__ Ua,
-                                                     const double *__restrict__ mu_p,
                                                      double *__restrict__ P) {
-    extern __shared__ double wsh[];  // [kG][m] weights then int list of occupied k
+    extern __shared__ double wsh[];  // [kG][n]
     const SubDesc d = desc[blockIdx.x];
     const int g0 = d.gbeg + blockIdx.y * kG;
-    if (g0 >= d.gend || (d.flags & 1)) return;
+    const int n = nocc[blockIdx.x];
+    if (g0 >= d.gend || n == 0) return;
     const int ng = min(kG, d.gend - g0);
-    const int m = d.m;
-    const double mu = mu_p[0];
-    const double *eig = eig_all + d.mapoff;
     const double *U = Ua + d.off;
+    const double *w = wocc + d.mapoff;
     const int *map = map_all + d.mapoff;
-    int *occk = reinterpret_cast<int *>(wsh + (size_t)kG * m);
-    __shared__ int nocc;
-    if (threadIdx.x == 0) {
-        int c = 0;
-        for (int k = 0; k < m; ++k)
-            if (occ_of(eig[k], mu) != 0.0) occk[c++] = k;
-        nocc = c;
+    for (int l = threadIdx.x; l < kG * n; l += kT) {
+        const int q = l / n, t = l % n;
+        wsh[(size_t)q * n + t] = q < ng ? w[t] * U[(int64_t)t * d.ld + oloc[g0 + q]] : 0.0;
     }
     __syncthreads();
-    const int no = nocc;
-    for (int l = threadIdx.x; l < kG * no; l += kT) {
-        const int q = l / no, kk = l % no;
-        const int k = occk[kk];
-        wsh[(size_t)q * m + kk] =
-            q < ng ? occ_of(eig[k], mu) * U[(int64_t)k * d.ld + oloc[g0 + q]] : 0.0;
-    }
-    __syncthreads();
-    for (int a = threadIdx.x; a < m; a += kT) {
+    for (int a = threadIdx.x; a < d.m; a += kT) {
         double acc[kG];
 #pragma unroll
         for (int q = 0; q < kG; ++q) acc[q] = 0.0;
-        for (int kk = 0; kk < no; ++kk) {
-            const double u = U[(int64_t)occk[kk] * d.ld + a];
+        for (int t = 0; t < n; ++t) {
+            const double u = U[(int64_t)t * d.ld + a];
 #pragma unroll
-            for (int q = 0; q < kG; ++q) acc[q] += u * wsh[(size_t)q * m + kk];
+            for (int q = 0; q < kG; ++q) acc[q] += u * wsh[(size_t)q * n + t];
         }
         const int64_t row = map[a];
 #pragma unroll
@@ -191,24 +222,22 @@ __global__ __launch_bounds__(kT) void k_dmat_scatter(const SubDesc *__restrict__
 
 namespace ptb {
 
-int fe_owned_rows(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, int max_own,
-                  const int *doloc, const double *L, const double *Y, double *Lown, double *SY) {
+int xall_build(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_own, const int *doloc,
+               const double *L, const int64_t *dxoff, const int64_t *dxboff, double *XA,
+               double *XB) {
     ProfScope prof_(ctx, "fe_rows");
-    hipLaunchKernelGGL(k_own_rows, dim3(max_own, nsm), dim3(kT), 0, ctx->stream, ddesc, doloc, L,
-                       max_m, Lown);
-    PTB_CHECK_LAUNCH(ctx);
-    const int wpb = kT / PTB_WAVE;
-    hipLaunchKernelGGL(k_fe_rows, dim3((max_m + wpb - 1) / wpb, nsm), dim3(kT), 0, ctx->stream,
-                       ddesc, Lown, max_m, Y, SY);
+    if (max_own <= 0) return PTBGPU_OK;
+    hipLaunchKernelGGL(k_xall_build, dim3(max_own, nsm), dim3(kT), 0, ctx->stream, ddesc, doloc, L,
+                       dxoff, dxboff, XA, XB);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
 
-int fe_finish(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const int *doloc,
-              const double *SY, const double *U, double *fe) {
+int fe_proj(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const int64_t *dxoff,
+            const double *F, double *fe) {
     ProfScope prof_(ctx, "fe_finish");
-    hipLaunchKernelGGL(k_fe_finish, dim3((max_m + kT - 1) / kT, nsm), dim3(kT), 0, ctx->stream,
-                       ddesc, doloc, SY, max_m, U, fe);
+    hipLaunchKernelGGL(k_fe_proj, dim3((max_m + kT - 1) / kT, nsm), dim3(kT), 0, ctx->stream,
+                       ddesc, dxoff, F, fe);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
@@ -222,14 +251,25 @@ int bisect_mu(ptbgpu_ctx *ctx, const double *e, const double *fe, int64_t total,
     return PTBGPU_OK;
 }
 
-int dmat_scatter(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, int max_own,
-                 const int *dgcols, const int *doloc, const int *dmap, const double *eig,
-                 const double *U, const double *dmu, double *P) {
+int occ_compact(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *eig,
+                const double *dmu, const double *Z, double *Zo, double *wocc, int *nocc) {
+    ProfScope prof_(ctx, "occ_compact");
+    hipLaunchKernelGGL(k_occ_compact, dim3(nsm), dim3(kT), sizeof(int) * (size_t)max_m,
+                       ctx->stream, ddesc, eig, dmu, Z, Zo, wocc, nocc);
+    PTB_CHECK_LAUNCH(ctx);
+    return PTBGPU_OK;
+}
+
+int dmat_compact(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, int max_own,
+                 const int *dgcols, const int *doloc, const int *dmap, const double *wocc,
+                 const int *nocc, const double *U, double *P) {
     ProfScope prof_(ctx, "dmat_scatter");
-    const size_t shmem = sizeof(double) * (size_t)kG * max_m + sizeof(int) * (size_t)max_m;
-    if (shmem > 150 * 1024) return fail(ctx, PTBGPU_E_ARG, "submatrix too large for dmat (%d)", max_m);
-    hipLaunchKernelGGL(k_dmat_scatter, dim3(nsm, (max_own + kG - 1) / kG), dim3(kT), shmem,
-                       ctx->stream, ddesc, dgcols, doloc, dmap, eig, U, dmu, P);
+    if (max_nocc <= 0 || max_own <= 0) return PTBGPU_OK;
+    const size_t shmem = sizeof(double) * (size_t)kG * max_nocc;
+    if (shmem > 150 * 1024)
+        return fail(ctx, PTBGPU_E_ARG, "too many occupied states for dmat (%d)", max_nocc);
+    hipLaunchKernelGGL(k_dmat_compact, dim3(nsm, (max_own + kG - 1) / kG), dim3(kT), shmem,
+                       ctx->stream, ddesc, dgcols, doloc, dmap, wocc, nocc, U, P);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }

@@ -230,7 +230,7 @@ struct Eig {
     std::vector<SubDesc> desc;
     SubDesc *ddesc = nullptr;
     int ld = 0;
-    double *A = nullptr, *Z = nullptr, *W = nullptr, *X = nullptr, *vec = nullptr, *Tb = nullptr;
+    double *A = nullptr, *Z = nullptr, *W = nullptr, *X = nullptr, *vec = nullptr;
 
     int setup(ptbgpu_ctx *ctx, int64_t n) {
         ld = (int)((n + 3) & ~3);
@@ -241,9 +241,8 @@ struct Eig {
         W = static_cast<double *>(ptb::ws(ctx, "pur_eW", ab));
         X = static_cast<double *>(ptb::ws(ctx, "pur_eX", ab));
         vec = static_cast<double *>(ptb::ws(ctx, "pur_evec", sizeof(double) * 8 * n + 4096));
-        Tb = static_cast<double *>(ptb::ws(ctx, "pur_eT", sizeof(double) * 32 * 32));
         ddesc = static_cast<SubDesc *>(ptb::ws(ctx, "pur_edesc", sizeof(SubDesc)));
-        if (!A || !Z || !W || !X || !vec || !Tb || !ddesc) return PTBGPU_E_NOMEM;
+        if (!A || !Z || !W || !X || !vec || !ddesc) return PTBGPU_E_NOMEM;
         PTB_HIP(ctx, hipMemcpy(ddesc, desc.data(), sizeof(SubDesc), hipMemcpyHostToDevice));
         return PTBGPU_OK;
     }
@@ -254,7 +253,7 @@ struct Eig {
         hipLaunchKernelGGL(k_load_upper, dim3(gb), dim3(kT), 0, ctx->stream, M, n, A, (int64_t)ld);
         PTB_CHECK_LAUNCH(ctx);
         double *dvec = vec, *evec = vec + n, *tau = vec + 2 * n, *rot = vec + 3 * n;
-        PTB_TRY(ptb::eig_sym_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot, Tb));
+        PTB_TRY(ptb::eig_sym_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot));
         PTB_HIP(ctx, hipMemcpy2DAsync(V, sizeof(double) * n, Z, sizeof(double) * ld,
                                       sizeof(double) * n, n, hipMemcpyDeviceToDevice, ctx->stream));
         w.resize(n);

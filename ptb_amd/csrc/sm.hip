@@ -6,11 +6,13 @@
 //   [submatrix stats]        pattern bitmap, per-group row union, map (device)
 //   [submatrix construction] gather of dense Hsm, Ssm
 //   [submatrix eigendecomp]  Cholesky Ssm = L L^T, K = L^-1 Hsm L^-T,
-//                            Householder tridiagonalisation, divide & conquer,
-//                            back-transformation Y = Q Z
-//   [submatrix fe]           fe weights, U = L^-T Y
+//                            Householder tridiagonalisation K = Q T Q^T,
+//                            divide & conquer T = Z diag(e) Z^T
+//   [submatrix fe]           fe weights from the owned rows of L Q Z and
+//                            L^-T Q Z (2c row vectors through Q and Z)
 //   [mu iter]                bisection of the chemical potential
-//   [submatrix final iter]   owned-column density, scatter into packed P
+//   [submatrix final iter]   occupied columns only: U_o = L^-T Q Z_o, owned-
+//                            column density, scatter into packed P
 // Every phase is batched over all submatrices: one launch (or one per tree level
 // / reflector block) covers the whole batch.
 #include <algorithm>
@@ -24,7 +26,6 @@
 namespace {
 
 constexpr int kLeafMax = 64;
-constexpr int kNB = 32;  // reflector block of the back-transformation
 
 struct Timer {
     hipEvent_t ev[PTBGPU_T_NSLOT + 1];
@@ -162,15 +163,42 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     auto *Z = static_cast<double *>(ptb::ws(ctx, "sm_Z", abytes));
     auto *W = static_cast<double *>(ptb::ws(ctx, "sm_W", abytes));
     auto *X = static_cast<double *>(ptb::ws(ctx, "sm_X", abytes));
-    auto *vec = static_cast<double *>(ptb::ws(ctx, "sm_vec", sizeof(double) * rows * 9 + 4096));
+    auto *vec = static_cast<double *>(ptb::ws(ctx, "sm_vec", sizeof(double) * rows * 10 + 4096));
     auto *flags = static_cast<int *>(ptb::ws(ctx, "sm_flags", sizeof(int) * nsm));
-    auto *Lown = static_cast<double *>(ptb::ws(ctx, "sm_lown", sizeof(double) * ndim * max_m));
-    auto *SY = static_cast<double *>(ptb::ws(ctx, "sm_sy", sizeof(double) * ndim * max_m));
-    auto *Tb = static_cast<double *>(ptb::ws(ctx, "sm_T", sizeof(double) * nsm * kNB * kNB));
-    if (!A || !B || !Z || !W || !X || !vec || !flags || !Lown || !SY || !Tb) return PTBGPU_E_NOMEM;
+    auto *nocc = static_cast<int *>(ptb::ws(ctx, "sm_nocc", sizeof(int) * nsm));
+    if (!A || !B || !Z || !W || !X || !vec || !flags || !nocc) return PTBGPU_E_NOMEM;
     double *dvec = vec, *evec = vec + rows, *tau = vec + 2 * rows, *fe = vec + 3 * rows;
-    double *rot = vec + 4 * rows;  // 4 * rows
-    double *bis = vec + 8 * rows;  // 4 + 200
+    double *rot = vec + 4 * rows;   // 4 * rows
+    double *wocc = vec + 8 * rows;  // rows
+    double *bis = vec + 9 * rows;   // 4 + 200
+    // owned-row projection buffers: XA = [L(O,:)^T, L^-1 e_O] (m x 2c), XB the
+    // unit right-hand sides (m x c), F = Z^T Q^T XA (m x 2c)
+    std::vector<int64_t> xoff(nsm), xboff(nsm);
+    std::vector<int> c2(nsm), c1(nsm);
+    RhsLayout lrhs;
+    lrhs.boff.resize(nsm);
+    lrhs.xoff.resize(nsm);
+    lrhs.cols.resize(nsm);
+    int64_t xtot = 0, xbtot = 0;
+    for (int s = 0; s < nsm; ++s) {
+        const int c = desc[s].gend - desc[s].gbeg;
+        xoff[s] = xtot;
+        xboff[s] = xbtot;
+        c1[s] = c;
+        c2[s] = 2 * c;
+        lrhs.boff[s] = xbtot;
+        lrhs.xoff[s] = xtot + (int64_t)c * desc[s].ld;
+        lrhs.cols[s] = c;
+        xtot += (int64_t)desc[s].ld * 2 * c;
+        xbtot += (int64_t)desc[s].ld * c;
+    }
+    auto *XA = static_cast<double *>(ptb::ws(ctx, "sm_xa", sizeof(double) * std::max<int64_t>(xtot, 1)));
+    auto *XB = static_cast<double *>(ptb::ws(ctx, "sm_xb", sizeof(double) * std::max<int64_t>(xbtot, 1)));
+    auto *F = static_cast<double *>(ptb::ws(ctx, "sm_f", sizeof(double) * std::max<int64_t>(xtot, 1)));
+    int64_t *dxoff, *dxboff;
+    PTB_TRY(upload(ctx, "sm_xoff", xoff, &dxoff));
+    PTB_TRY(upload(ctx, "sm_xboff", xboff, &dxboff));
+    if (!XA || !XB || !F) return PTBGPU_E_NOMEM;
     PTB_TRY(ptb::gather(ctx, dH, dS, ddesc, nsm, max_m, dmap, A, B));
     tm.mark(3, st);
     // ---- eigendecomposition
@@ -180,13 +208,25 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, Z, A, false));  // A = (L^-1 H)^T
     PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, A, Z, false, "k"));        // Z = K
     PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, Z, A, true));   // A = sym(K)
-    PTB_TRY(ptb::eig_sym_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot, Tb));
+    QFactor qf;
+    PTB_TRY(ptb::eig_tri_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot, qf));
     tm.mark(4, st);
-    // ---- fe and U = L^-T Y
+    // ---- fe from the owned rows: F = Z^T Q^T [L(O,:)^T, L^-1 e_O]
     std::vector<int> hflags(nsm);
     PTB_HIP(ctx, hipMemcpyAsync(hflags.data(), flags, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
-    PTB_TRY(ptb::fe_owned_rows(ctx, ddesc, nsm, max_m, max_own, doloc, B, Z, Lown, SY));
-    PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, Z, A, true, "u"));         // A = U = L^-T Y
+    PTB_TRY(ptb::xall_build(ctx, ddesc, nsm, max_own, doloc, B, dxoff, dxboff, XA, XB));
+    PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, XB, XA, false, "e", &lrhs));  // L^-1 e_O
+    PTB_TRY(ptb::apply_q(ctx, desc, qf, XA, xoff, c2, true, "x"));           // Q^T XA
+    {
+        std::vector<GemmDesc> gf(nsm);
+        for (int s = 0; s < nsm; ++s) {
+            const SubDesc &d = desc[s];
+            gf[s] = {Z + d.off, XA + xoff[s], F + xoff[s], d.m, c2[s], d.m, d.ld, d.ld, d.ld, 1.0, 0.0};
+        }
+        GemmDesc *dgf;
+        PTB_TRY(upload(ctx, "sm_gf", gf, &dgf));
+        PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgf, nsm, max_m, 2 * max_own));
+    }
     PTB_HIP(ctx, hipStreamSynchronize(st));
     int64_t not_spd = 0;
     for (int s = 0; s < nsm; ++s)
@@ -200,14 +240,33 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
                              (long long)not_spd);
         PTB_HIP(ctx, hipMemcpyAsync(ddesc, desc.data(), sizeof(SubDesc) * nsm, hipMemcpyHostToDevice, st));
     }
-    PTB_TRY(ptb::fe_finish(ctx, ddesc, nsm, max_m, doloc, SY, A, fe));
+    PTB_TRY(ptb::fe_proj(ctx, ddesc, nsm, max_m, dxoff, F, fe));
     tm.mark(5, st);
     // ---- mu bisection
     PTB_TRY(ptb::bisect_mu(ctx, dvec, fe, rows, nelref, bis));
     tm.mark(6, st);
-    // ---- final density
+    // ---- final density: occupied columns only, Y_o = Q Z_o, U_o = L^-T Y_o
+    PTB_TRY(ptb::occ_compact(ctx, ddesc, nsm, max_m, dvec, bis, Z, W, wocc, nocc));
+    std::vector<int> hnocc(nsm);
+    PTB_HIP(ctx, hipMemcpyAsync(hnocc.data(), nocc, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
+    PTB_HIP(ctx, hipStreamSynchronize(st));
+    int max_nocc = 0;
+    std::vector<int64_t> aoff(nsm);
+    RhsLayout urhs;
+    urhs.boff.resize(nsm);
+    urhs.xoff.resize(nsm);
+    urhs.cols = hnocc;
+    for (int s = 0; s < nsm; ++s) {
+        max_nocc = std::max(max_nocc, hnocc[s]);
+        aoff[s] = desc[s].off;
+        urhs.boff[s] = desc[s].off;
+        urhs.xoff[s] = desc[s].off;
+    }
+    PTB_TRY(ptb::apply_q(ctx, desc, qf, W, aoff, hnocc, false, "o"));        // W = Q Z_o
+    PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, W, X, true, "u", &urhs));    // X = L^-T Q Z_o
     PTB_HIP(ctx, hipMemsetAsync(dP, 0, sizeof(double) * (size_t)(ndim * (ndim + 1) / 2), st));
-    PTB_TRY(ptb::dmat_scatter(ctx, ddesc, nsm, max_m, max_own, dgcols, doloc, dmap, dvec, A, bis, dP));
+    PTB_TRY(ptb::dmat_compact(ctx, ddesc, nsm, max_nocc, max_own, dgcols, doloc, dmap, wocc, nocc,
+                              X, dP));
     tm.mark(7, st);
     std::vector<double> hb(4 + 200);
     PTB_HIP(ctx, hipMemcpyAsync(hb.data(), bis, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, st));
@@ -254,13 +313,14 @@ int check_sm_args(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const void *H, 
 
 namespace ptb {
 
-// Eigenpairs of a batch of dense symmetric matrices (full storage in A, both
-// triangles): Householder tridiagonalisation, divide and conquer, WY
-// back-transformation.  On return Z holds the eigenvectors (column order of
-// the eigenvalues in d, not sorted), A holds the Householder vectors.
-int eig_sym_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+// Tridiagonal eigendecomposition of a batch of dense symmetric matrices (full
+// storage in A, both triangles): Householder tridiagonalisation A = Q T Q^T and
+// divide and conquer T = Z diag(d) Z^T.  On return Z holds the eigenvectors of
+// T (column order of the eigenvalues in d, not sorted), A the Householder
+// vectors and qf the blocked form of Q (apply_q multiplies by Q or Q^T).
+int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
-                  double *tau, double *rot, double *Tb) {
+                  double *tau, double *rot, QFactor &qf) {
     hipStream_t st = ctx->stream;
     const int nsm = (int)desc.size();
     int max_m = 0;
@@ -272,6 +332,7 @@ int eig_sym_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
     const size_t abytes = sizeof(double) * arena;
     PTB_TRY(sytrd_batch(ctx, desc, ddesc, A, W, dvec, evec, tau));
     PTB_TRY(zero_upper_batch(ctx, ddesc, nsm, max_m, A));              // A = V
+    PTB_TRY(larft_all(ctx, desc, ddesc, A, tau, qf));
     // divide and conquer
     std::vector<DcNode> leaves;
     std::vector<std::vector<DcNode>> levels;
@@ -296,53 +357,23 @@ int eig_sym_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
         PTB_TRY(dc_merge_level(ctx, ddesc, dnodes, (int)nodes.size(), max_n, evec, dvec, Z,
                                     W, X, rot, dg));
     }
-    // back-transformation Y = Q Z, reflector blocks from the last to the first
-    int max_blocks = 0;
-    for (int s = 0; s < nsm; ++s) {
-        const int nr = desc[s].m - 2;
-        max_blocks = std::max(max_blocks, nr > 0 ? (nr + kNB - 1) / kNB : 0);
-    }
-    if (max_blocks > 0) {
-        std::vector<GemmDesc> g1, g2, g3;
-        std::vector<int> cnt(max_blocks, 0), start(max_blocks + 1, 0);
-        g1.reserve((size_t)nsm * max_blocks);
-        for (int q = 0; q < max_blocks; ++q) {
-            start[q] = (int)g1.size();
-            for (int s = 0; s < nsm; ++s) {
-                const SubDesc &d = desc[s];
-                const int nr = d.m - 2;
-                const int nblk = nr > 0 ? (nr + kNB - 1) / kNB : 0;
-                const int b = nblk - 1 - q;
-                if (b < 0) continue;
-                const int b0 = b * kNB, kb = std::min(kNB, nr - b0);
-                const int rowsv = d.m - b0 - 1;
-                double *V = A + d.off + (int64_t)b0 * d.ld + b0 + 1;
-                double *Y = Z + d.off + b0 + 1;
-                // W1 = V_b^T Y and W2 = T W1 are kb x m panels inside this
-                // submatrix's W and X blocks (leading dimension <= ld)
-                const int lw = std::min(kNB, d.ld);
-                double *W1 = W + d.off, *W2 = X + d.off;
-                const double *T = Tb + (int64_t)s * kNB * kNB;
-                g1.push_back({V, Y, W1, kb, d.m, rowsv, d.ld, d.ld, lw, 1.0, 0.0});
-                g2.push_back({T, W1, W2, kb, d.m, kb, kNB, lw, lw, 1.0, 0.0});
-                g3.push_back({V, W2, Y, rowsv, d.m, kb, d.ld, lw, d.ld, -1.0, 1.0});
-            }
-        }
-        start[max_blocks] = (int)g1.size();
-        GemmDesc *dg1, *dg2, *dg3;
-        PTB_TRY(upload(ctx, "sm_bt1", g1, &dg1));
-        PTB_TRY(upload(ctx, "sm_bt2", g2, &dg2));
-        PTB_TRY(upload(ctx, "sm_bt3", g3, &dg3));
-        for (int q = 0; q < max_blocks; ++q) {
-            const int c = start[q + 1] - start[q];
-            if (c == 0) continue;
-            PTB_TRY(larft_batch(ctx, ddesc, nsm, A, tau, q, kNB, Tb));
-            PTB_TRY(gemm_vbatch(ctx, true, false, dg1 + start[q], c, kNB, max_m));
-            PTB_TRY(gemm_vbatch(ctx, false, false, dg2 + start[q], c, kNB, max_m));
-            PTB_TRY(gemm_vbatch(ctx, false, false, dg3 + start[q], c, max_m, max_m));
-        }
-    }
     return PTBGPU_OK;
+}
+
+// Full eigenvectors: eig_tri_batch, then Z <- Q Z for all columns.
+int eig_sym_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                  double *A, double *Z, double *W, double *X, double *dvec, double *evec,
+                  double *tau, double *rot) {
+    QFactor qf;
+    PTB_TRY(eig_tri_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot, qf));
+    const int nsm = (int)desc.size();
+    std::vector<int64_t> zoff(nsm);
+    std::vector<int> cols(nsm);
+    for (int s = 0; s < nsm; ++s) {
+        zoff[s] = desc[s].off;
+        cols[s] = desc[s].m;
+    }
+    return apply_q(ctx, desc, qf, Z, zoff, cols, false, "z");
 }
 
 }  // namespace ptb

@@ -32,6 +32,23 @@ struct DcNode {
     int lo, sp, hi;
 };
 
+// Blocked Householder factor of a batch: reflectors in the columns of V (the
+// arena of the reduced matrices), T factors of all 32-reflector blocks.
+struct QFactor {
+    const double *V = nullptr;
+    double *T = nullptr;
+    std::vector<int64_t> toff;  // per submatrix, element offset of its first T block
+    int maxblk = 0;
+};
+
+// Right-hand sides of a batched triangular solve that are not the submatrices'
+// own arena blocks: cols[s] columns at B + boff[s] (input) and X + xoff[s]
+// (output), leading dimension of the submatrix.
+struct RhsLayout {
+    std::vector<int64_t> boff, xoff;
+    std::vector<int> cols;
+};
+
 namespace ptb {
 
 // pattern.hip
@@ -49,24 +66,29 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
                 int max_n);
 int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                 double *A, int *flags);
-int trsm_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *L,
-               double *B, bool trans);
 int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m,
                         const double *in, double *out, bool symmetrize);
 int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
-              const double *L, double *B, double *X, bool trans, const char *tag);
+              const double *L, double *B, double *X, bool trans, const char *tag,
+              const RhsLayout *rhs = nullptr);
 int zero_upper_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, double *A);
-int larft_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, const double *V,
-                const double *tau, int q, int nb, double *T);
+int larft_all(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+              const double *V, const double *tau, QFactor &qf);
+int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf, double *Y,
+            const std::vector<int64_t> &yoff, const std::vector<int> &ncols, bool trans,
+            const char *tag);
 
 // tridiag.hip
 int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                 double *A, double *W, double *d, double *e, double *tau);
 
 // sm.hip
+int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                  double *A, double *Z, double *W, double *X, double *dvec, double *evec,
+                  double *tau, double *rot, QFactor &qf);
 int eig_sym_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
-                  double *tau, double *rot, double *Tb);
+                  double *tau, double *rot);
 
 // dc.hip
 int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int nleaves,
@@ -76,14 +98,17 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
                    double *rot, GemmDesc *dgemm);
 
 // density.hip
-int fe_owned_rows(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, int max_own,
-                  const int *doloc, const double *L, const double *Y, double *Lown, double *SY);
-int fe_finish(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const int *doloc,
-              const double *SY, const double *U, double *fe);
+int xall_build(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_own, const int *doloc,
+               const double *L, const int64_t *dxoff, const int64_t *dxboff, double *XA,
+               double *XB);
+int fe_proj(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const int64_t *dxoff,
+            const double *F, double *fe);
 int bisect_mu(ptbgpu_ctx *ctx, const double *e, const double *fe, int64_t total, int64_t nelref,
               double *dout);
-int dmat_scatter(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, int max_own,
-                 const int *dgcols, const int *doloc, const int *dmap, const double *eig,
-                 const double *U, const double *dmu, double *P);
+int occ_compact(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *eig,
+                const double *dmu, const double *Z, double *Zo, double *wocc, int *nocc);
+int dmat_compact(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, int max_own,
+                 const int *dgcols, const int *doloc, const int *dmap, const double *wocc,
+                 const int *nocc, const double *U, double *P);
 
 }  // namespace ptb
