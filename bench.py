@@ -35,10 +35,12 @@ WORKLOADS = {
 
 
 # bracketed names: whole stages (their GEMMs included) and single kernels
-PHASES = ["potrf", "trsm", "sytrd"]
+PHASES = ["potrf", "trsm", "trsm_h", "trsm_k", "trsm_e", "trsm_u", "sytrd", "apply_q",
+          "apply_q_x", "apply_q_o", "gemm_sytrd", "gemm_potrf", "gemm_trsm", "gemm_dc", "gemm_q",
+          "gemm_fe", "gemm_other"]
 KERNELS = ["pattern", "group_union", "group_map", "gather", "potrf_panel", "diag_inv",
            "transpose", "sytrd_panel", "zero_upper", "dc_leaf", "dc_merge", "gemm", "larft",
-           "fe_rows", "fe_finish", "bisect", "dmat_scatter"]
+           "fe_rows", "fe_finish", "bisect", "occ_compact", "dmat_scatter"]
 
 
 def log(*a):

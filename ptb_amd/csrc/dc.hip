@@ -376,6 +376,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         g->ldc = ld;
         g->alpha = 1.0;
         g->beta = 0.0;
+        g->uplo = 0;
     }
     (void)srho;
 }
@@ -408,7 +409,7 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
                            npow, e, D, Z, W, X, rot, dgemm);
         PTB_CHECK_LAUNCH(ctx);
     }
-    return gemm_vbatch(ctx, false, false, dgemm, count, max_n, max_n);
+    return gemm_vbatch(ctx, false, false, dgemm, count, max_n, max_n, "dc");
 }
 
 }  // namespace ptb

@@ -32,6 +32,7 @@ __global__ __launch_bounds__(kT) void k_gemm(const GemmDesc *__restrict__ descs)
     const int tilesM = (g.M + 63) / 64, tilesN = (g.N + 63) / 64;
     if (g.M <= 0 || g.N <= 0 || (int)blockIdx.x >= tilesM * tilesN) return;
     const int m0 = (blockIdx.x % tilesM) * 64, n0 = (blockIdx.x / tilesM) * 64;
+    if (g.uplo && n0 > m0 + 63) return;  // tile strictly above the diagonal
     __shared__ double As[16][68];
     __shared__ double Bs[16][68];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(kT) void k_gemm(const GemmDesc *__restrict__ descs)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gm = m0 + wm + 16 * i + 4 * r + lk;
-                if (gm >= g.M) continue;
+                if (gm >= g.M || (g.uplo && gm < gn)) continue;
                 double *c = g.C + (int64_t)gn * g.ldc + gm;
                 const double v = acc[i][j][r];
                 *c = g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * *c;
@@ -202,8 +203,11 @@ __global__ __launch_bounds__(kT) void k_zero_upper(const SubDesc *__restrict__ d
 // T factor of every reflector block of every submatrix in one launch (grid:
 // block b counted from the first, submatrix).  Q_b = H_b0 ... H_b0+kb-1 =
 // I - V_b T_b V_b^T with T_b upper triangular (LAPACK dlarft, forward,
-// columnwise).  G(i,j) = v_i^T v_j is one wavefront dot product per pair.
-// T_b is stored 32 x 32 column-major at Ta + toff[s] + b * 1024.
+// columnwise).  G = V_b^T V_b is accumulated over 64-row chunks of the block
+// staged in LDS (each column read once); the recurrence
+// T(r, i) = -tau_i sum_{c=r}^{i-1} T(r, c) G(i, c) runs one column i at a time
+// with the rows r < i in parallel.  T_b is stored 32 x 32 column-major at
+// Ta + toff[s] + b * 1024.
 __global__ __launch_bounds__(kT) void k_larft_all(const SubDesc *__restrict__ desc,
                                                   const double *__restrict__ Va,
                                                   const double *__restrict__ tau_all,
@@ -219,39 +223,59 @@ __global__ __launch_bounds__(kT) void k_larft_all(const SubDesc *__restrict__ de
     const double *V = Va + d.off + (int64_t)b0 * d.ld;
     const double *tau = tau_all + d.mapoff + b0;
     double *T = Ta + toff[blockIdx.y] + (int64_t)b * 1024;
+    __shared__ double Vs[64][33];
     __shared__ double G[32][33];
     __shared__ double Ts[32][33];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // pairs (i, j), j < i < kb, p = i (i - 1) / 2 + j; at most 496 < 2 * kT
+    const int npair = kb * (kb - 1) / 2;
+    int pi[2], pj[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int p = threadIdx.x + h * kT;
+        int i = 1;
+        while (p < npair && (i + 1) * i / 2 <= p) ++i;
+        pi[h] = i;
+        pj[h] = p < npair ? p - i * (i - 1) / 2 : 0;
+    }
+    double acc[2] = {0.0, 0.0};
+    // column c is zero above its unit at row b0 + c + 1, so every pair can run
+    // over rows b0 + 1 .. m - 1
+    for (int r0 = b0 + 1; r0 < d.m; r0 += 64) {
+        for (int l = threadIdx.x; l < 64 * 32; l += kT) {
+            const int rr = l & 63, cc = l >> 6, r = r0 + rr;
+            Vs[rr][cc] = (r < d.m && cc < kb) ? V[(int64_t)cc * d.ld + r] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (threadIdx.x + h * kT < npair) {
+                double sacc = 0.0;
+#pragma unroll 8
+                for (int rr = 0; rr < 64; ++rr) sacc += Vs[rr][pi[h]] * Vs[rr][pj[h]];
+                acc[h] += sacc;
+            }
+        }
+        __syncthreads();
+    }
     for (int l = threadIdx.x; l < 1024; l += kT) {
         G[l & 31][l >> 5] = 0.0;
         Ts[l & 31][l >> 5] = 0.0;
     }
     __syncthreads();
-    // pairs (i, j), j < i < kb, enumerated as p = i (i - 1) / 2 + j
-    const int npair = kb * (kb - 1) / 2;
-    for (int p = wid; p < npair; p += kT / 64) {
-        int i = 1;
-        while ((i + 1) * i / 2 <= p) ++i;
-        const int j = p - i * (i - 1) / 2;
-        const double *vi = V + (int64_t)i * d.ld, *vj = V + (int64_t)j * d.ld;
-        double s = 0.0;
-        // v_i is zero above row b0 + i + 1 (its stored unit)
-        for (int r = b0 + i + 1 + lane; r < d.m; r += 64) s += vj[r] * vi[r];
-        s = wave_sum(s);
-        if (lane == 0) G[i][j] = s;
-    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+        if (threadIdx.x + h * kT < npair) G[pi[h]][pj[h]] = acc[h];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < kb; ++i) {
-            Ts[i][i] = tau[i];
-            for (int r = 0; r < i; ++r) {
-                double s = 0.0;
-                for (int c = r; c < i; ++c) s += Ts[r][c] * G[i][c];
-                Ts[r][i] = -tau[i] * s;
-            }
+    for (int i = 0; i < kb; ++i) {
+        const int r = threadIdx.x;
+        if (r == i) Ts[i][i] = tau[i];
+        if (r < i) {
+            double sacc = 0.0;
+            for (int c = r; c < i; ++c) sacc += Ts[r][c] * G[i][c];
+            Ts[r][i] = -tau[i] * sacc;
         }
+        __syncthreads();
     }
-    __syncthreads();
     for (int l = threadIdx.x; l < 1024; l += kT) {
         const int r = l & 31, c = l >> 5;
         T[(int64_t)c * 32 + r] = Ts[r][c];
@@ -301,8 +325,9 @@ __global__ __launch_bounds__(64) void k_diag_inv(const SubDesc *__restrict__ des
 namespace ptb {
 
 int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int count, int max_m,
-                int max_n) {
+                int max_n, const char *site) {
     ProfScope prof_(ctx, "gemm");
+    ProfScope sprof_(ctx, site ? (std::string("gemm_") + site).c_str() : "gemm_other");
     if (count <= 0 || max_m <= 0 || max_n <= 0) return PTBGPU_OK;
     const int tiles = ((max_m + 63) / 64) * ((max_n + 63) / 64);
     for (int c0 = 0; c0 < count; c0 += 65535) {
@@ -325,6 +350,7 @@ int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
               const double *L, double *B, double *X, bool trans, const char *tag,
               const RhsLayout *rhs) {
     ProfScope prof_(ctx, "trsm");
+    ProfScope tprof_(ctx, (std::string("trsm_") + tag).c_str());
     const int nsm = (int)desc.size();
     std::vector<int64_t> doff(nsm);
     int64_t tot = 0;
@@ -389,8 +415,8 @@ int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
                                 ctx->stream));
     for (int q = 0; q < max_nb; ++q) {
         const int c1 = s1[q + 1] - s1[q], c2 = s2[q + 1] - s2[q];
-        if (c1 > 0) PTB_TRY(gemm_vbatch(ctx, trans, false, dg1 + s1[q], c1, 64, max_m));
-        if (c2 > 0) PTB_TRY(gemm_vbatch(ctx, trans, false, dg2 + s2[q], c2, 64, max_m));
+        if (c1 > 0) PTB_TRY(gemm_vbatch(ctx, trans, false, dg1 + s1[q], c1, 64, max_m, "trsm"));
+        if (c2 > 0) PTB_TRY(gemm_vbatch(ctx, trans, false, dg2 + s2[q], c2, 64, max_m, "trsm"));
     }
     return PTBGPU_OK;
 }
@@ -412,7 +438,7 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
             if (n <= 0) continue;
             double *a = A + x.off;
             double *P = a + (int64_t)k0 * x.ld + s0;
-            g.push_back({P, P, a + (int64_t)s0 * x.ld + s0, n, n, 32, x.ld, x.ld, x.ld, -1.0, 1.0});
+            g.push_back({P, P, a + (int64_t)s0 * x.ld + s0, n, n, 32, x.ld, x.ld, x.ld, -1.0, 1.0, 1, 0});
         }
     }
     start[npanel] = (int)g.size();
@@ -428,7 +454,7 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
             PTB_CHECK_LAUNCH(ctx);
         }
         const int c = start[p + 1] - start[p];
-        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dg + start[p], c, max_m, max_m));
+        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dg + start[p], c, max_m, max_m, "potrf"));
     }
     return PTBGPU_OK;
 }
@@ -488,6 +514,7 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
             const std::vector<int64_t> &yoff, const std::vector<int> &ncols, bool trans,
             const char *tag) {
     ProfScope prof_(ctx, "apply_q");
+    ProfScope tprof_(ctx, (std::string("apply_q_") + tag).c_str());
     const int nsm = (int)desc.size();
     if (qf.maxblk == 0) return PTBGPU_OK;
     std::vector<int64_t> woff(nsm);
@@ -539,9 +566,9 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
     for (int q = 0; q < qf.maxblk; ++q) {
         const int c = start[q + 1] - start[q];
         if (c == 0) continue;
-        PTB_TRY(gemm_vbatch(ctx, true, false, dg1 + start[q], c, 32, max_c));
-        PTB_TRY(gemm_vbatch(ctx, trans, false, dg2 + start[q], c, 32, max_c));
-        PTB_TRY(gemm_vbatch(ctx, false, false, dg3 + start[q], c, max_m, max_c));
+        PTB_TRY(gemm_vbatch(ctx, true, false, dg1 + start[q], c, 32, max_c, "q"));
+        PTB_TRY(gemm_vbatch(ctx, trans, false, dg2 + start[q], c, 32, max_c, "q"));
+        PTB_TRY(gemm_vbatch(ctx, false, false, dg3 + start[q], c, max_m, max_c, "q"));
     }
     return PTBGPU_OK;
 }

@@ -225,7 +225,7 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
         }
         GemmDesc *dgf;
         PTB_TRY(upload(ctx, "sm_gf", gf, &dgf));
-        PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgf, nsm, max_m, 2 * max_own));
+        PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgf, nsm, max_m, 2 * max_own, "fe"));
     }
     PTB_HIP(ctx, hipStreamSynchronize(st));
     int64_t not_spd = 0;

@@ -16,7 +16,9 @@ struct SubDesc {
     int flags;  // bit 0: overlap block not positive definite
 };
 
-// One product C = alpha op(A) op(B) + beta C of a variable-size batch.
+// One product C = alpha op(A) op(B) + beta C of a variable-size batch.  With
+// uplo = 1 only the lower triangle of C (row >= column) is formed: tiles above
+// the diagonal are skipped (symmetric updates such as A -= V W^T + W V^T).
 struct GemmDesc {
     const double *A;
     const double *B;
@@ -24,6 +26,8 @@ struct GemmDesc {
     int M, N, K;
     int lda, ldb, ldc;
     double alpha, beta;
+    int uplo;
+    int reserved;
 };
 
 // A divide-and-conquer merge node: rows/cols [lo, hi) of submatrix s, split at sp.
@@ -63,7 +67,7 @@ int gather(ptbgpu_ctx *ctx, const double *dH, const double *dS, const SubDesc *d
 
 // dense.hip
 int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int count, int max_m,
-                int max_n);
+                int max_n, const char *site = nullptr);
 int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                 double *A, int *flags);
 int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m,

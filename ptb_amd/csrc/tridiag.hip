@@ -4,11 +4,13 @@
 // One 512-thread workgroup per submatrix, blocked by 32 columns (dlatrd scheme):
 // inside a panel the trailing matrix stays as it was at the panel start and the
 // pending rank-2 terms are applied on the fly as V (W^T v) + W (V^T v); at the
-// end of the panel one rank-64 update refreshes the trailing matrix.  The
-// per-column product p = A22 v is a GEMV over the full (both triangles)
-// trailing matrix: threads own rows, so a wavefront reads 64 consecutive
-// elements of one column.  This GEMV is the bandwidth-bound core of the solve:
-// sum_k (m-k)^2 * 8 bytes per submatrix.
+// end of the panel one rank-64 update refreshes the lower triangle of the
+// trailing matrix (the upper triangle is never read again).  The per-column
+// product p = A22 v is a symmetric GEMV over the lower triangle: lanes own
+// rows, so a wavefront reads 64 consecutive elements of one column, and the
+// transposed half of each element's work is reduced across the wavefront.
+// This GEMV is the bandwidth-bound core of the solve:
+// sum_k 8 (m-k)(m-k+1)/2 bytes per submatrix.
 //
 // Output: d (diagonal), e (subdiagonal, e[k] couples k and k+1), tau, and v_k
 // stored in column k rows k+1.. of A (unit at row k+1).  W is a work panel
@@ -33,6 +35,38 @@ __device__ __forceinline__ double block_reduce(double v, double *red) {
     return t;
 }
 
+// Sum over the wavefront of 16 values per lane, returned in the lanes with
+// (lane >> 2) & 15 == q for value q: a reduce-scatter butterfly (partners
+// 32, 16, 8, 4 halve the values held, partners 1 and 2 finish the sums) in
+// 17 exchanges instead of 16 full reductions.
+__device__ __forceinline__ double butterfly16(const double (&p)[16], int lane) {
+    double h8[8], h4[4], h2[2];
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const double send = b5 ? p[j] : p[j + 8];
+        const double keep = b5 ? p[j + 8] : p[j];
+        h8[j] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double send = b4 ? h8[j] : h8[j + 4];
+        const double keep = b4 ? h8[j + 4] : h8[j];
+        h4[j] = keep + __shfl_xor(send, 16);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const double send = b3 ? h4[j] : h4[j + 2];
+        const double keep = b3 ? h4[j + 2] : h4[j];
+        h2[j] = keep + __shfl_xor(send, 8);
+    }
+    const double send = b2 ? h2[0] : h2[1];
+    double v = (b2 ? h2[1] : h2[0]) + __shfl_xor(send, 4);
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    return v;
+}
+
 __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict__ desc,
                                                     double *__restrict__ Aa,
                                                     double *__restrict__ Wa,
@@ -46,9 +80,9 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
     double *W = Wa + ds.off;  // columns 0..31 used
     double *dd = d_all + ds.mapoff, *ee = e_all + ds.mapoff, *tt = tau_all + ds.mapoff;
     double *vs = sm;              // m   current Householder vector (rows > k)
-    double *praw = vs + m;        // m   A22 v before the pending corrections
-    double *part = praw + m;      // kT  column-group partial sums
-    double *red = part + kT;      // kWaves
+    double *cp = vs + m;          // kWaves x m  per-wavefront partials of A22 v
+    double *praw = cp;            // m   A22 v before the pending corrections
+    double *red = cp + kWaves * m; // kWaves
     double *c1 = red + kWaves;    // kNB  W[k][t]
     double *c2 = c1 + kNB;        // kNB  V[k][k0+t]
     double *q1 = c2 + kNB;        // kNB  W[:,t]^T v
@@ -139,61 +173,70 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
                 }
                 __syncthreads();
             }
-            // (e) praw = A22 v over the full trailing matrix.  R threads per
-            // column group cover the rows, G = kT/R column groups split the
-            // columns, sixteen columns per iteration (32 loads in flight per
-            // thread with the row pairs).
-            int R = ((n + PTB_WAVE - 1) / PTB_WAVE) * PTB_WAVE;
-            if (R > kT) R = kT;
-            const int G = kT / R;
+            // (e) praw = A22 v from the lower triangle only (A22 is symmetric
+            // and only its lower triangle is kept up to date), tile by tile
+            // (64 rows, one per lane, x 16 columns).  A tile's row sums and
+            // its transposed (strictly lower) products, summed over the
+            // wavefront by a butterfly, are added to the wavefront's own
+            // partial vector cp[wid][:]; praw = sum of the partial vectors in
+            // fixed order.
+            for (int i = k + 1 + threadIdx.x; i < m; i += kT)
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w) cp[w * m + i] = 0.0;
+            __syncthreads();
             {
-                const int g = threadIdx.x / R, rt = threadIdx.x % R;
-                if (g < G) {
-                    const int CH = (n + G - 1) / G;
-                    const int cb = k + 1 + g * CH, ce = min(m, cb + CH);
-                    for (int i0 = k + 1 + rt; i0 < m; i0 += 2 * R) {
-                        const int i1 = i0 + R;
-                        const bool h1 = i1 < m;
-                        const int i1c = h1 ? i1 : i0;
-                        double s0[4] = {0.0, 0.0, 0.0, 0.0}, s1[4] = {0.0, 0.0, 0.0, 0.0};
-                        int c = cb;
-                        for (; c + 7 < ce; c += 8) {
-                            double x0[8], x1[8];
+                // lower-triangle tiles of 64 rows x 16 columns, numbered row
+                // block by row block: block bi holds 4 (bi + 1) tiles (the
+                // last one ceil(n / 16)), so tile t lies in the block bi with
+                // 2 bi (bi + 1) <= t < 2 (bi + 1) (bi + 2).  Wavefront wid
+                // takes the tile pairs wid, wid + kWaves, ...
+                const int nrb = (n + 63) / 64;
+                const int ntile = 2 * (nrb - 1) * nrb + (n + 15) / 16;
+                double *cpw = cp + wid * m;
+                for (int t0 = 2 * wid; t0 < ntile; t0 += 2 * kWaves) {
+                    double x[2][16];
+                    int irow[2], cbeg[2];
 #pragma unroll
-                            for (int q = 0; q < 8; ++q) {
-                                x0[q] = a[(int64_t)(c + q) * ld + i0];
-                                x1[q] = a[(int64_t)(c + q) * ld + i1c];
-                            }
+                    for (int h = 0; h < 2; ++h) {
+                        const int t = t0 + h;
+                        int bi = (int)((sqrtf(2.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+                        while (2 * (bi + 1) * (bi + 2) <= t) ++bi;
+                        while (bi > 0 && 2 * bi * (bi + 1) > t) --bi;
+                        const int cj = t - 2 * bi * (bi + 1);
+                        const int i = k + 1 + bi * 64 + lane;
+                        const int c0 = k + 1 + cj * 16;
+                        const bool ok = t < ntile && i < m;
+                        irow[h] = ok ? i : -1;
+                        cbeg[h] = c0;
 #pragma unroll
-                            for (int q = 0; q < 8; ++q) {
-                                const double v = vs[c + q];
-                                s0[q & 3] += x0[q] * v;
-                                s1[q & 3] += x1[q] * v;
-                            }
-                        }
-                        for (; c < ce; ++c) {
-                            const double v = vs[c];
-                            s0[0] += a[(int64_t)c * ld + i0] * v;
-                            s1[0] += a[(int64_t)c * ld + i1c] * v;
-                        }
-                        const double p0 = (s0[0] + s0[1]) + (s0[2] + s0[3]);
-                        const double p1 = (s1[0] + s1[1]) + (s1[2] + s1[3]);
-                        if (G == 1) {
-                            praw[i0] = p0;
-                            if (h1) praw[i1] = p1;
-                        } else {
-                            part[g * R + rt] = p0;  // R >= n here: one row per thread
+                        for (int q = 0; q < 16; ++q) {
+                            const int c = c0 + q;
+                            x[h][q] = (ok && c <= i) ? a[(int64_t)c * ld + i] : 0.0;
                         }
                     }
-                }
-                if (G > 1) {
-                    __syncthreads();
-                    for (int r = threadIdx.x; r < n; r += kT) {
-                        double sacc = 0.0;
-                        for (int gg = 0; gg < G; ++gg) sacc += part[gg * R + r];
-                        praw[k + 1 + r] = sacc;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        if (t0 + h >= ntile) break;
+                        const int i = irow[h], c0 = cbeg[h];
+                        const double vi = i >= 0 ? vs[i] : 0.0;
+                        double yrow = 0.0;
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) yrow += x[h][q] * vs[min(c0 + q, m - 1)];
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) x[h][q] = (c0 + q < i) ? x[h][q] * vi : 0.0;
+                        const double colsum = butterfly16(x[h], lane);
+                        const int cq = c0 + ((lane >> 2) & 15);
+                        if ((lane & 3) == 0 && cq < m) cpw[cq] += colsum;
+                        if (i >= 0) cpw[i] += yrow;
                     }
                 }
+            }
+            __syncthreads();
+            for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
+                double sacc = 0.0;
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w) sacc += cp[w * m + i];
+                praw[i] = sacc;
             }
             __syncthreads();
             // (f) p = tau (praw - V q1 - W q2);  w = p - 0.5 tau (p^T v) v
@@ -253,7 +296,7 @@ int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
     const int nsm = (int)desc.size();
     int max_m = 0;
     for (const auto &x : desc) max_m = std::max(max_m, x.m);
-    const size_t shmem = sizeof(double) * (2 * (size_t)max_m + kT + kWaves + 4 * kNB + kWaves * 16);
+    const size_t shmem = sizeof(double) * ((1 + kWaves) * (size_t)max_m + kWaves + 4 * kNB + kWaves * 16);
     if (shmem > 160 * 1024) return fail(ctx, PTBGPU_E_ARG, "submatrix too large for sytrd (%d)", max_m);
     const int npanel = max_m > 2 ? (max_m - 2 + kNB - 1) / kNB : 0;
     std::vector<GemmDesc> g1, g2;
@@ -270,8 +313,8 @@ int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
             double *a = A + x.off;
             double *V = a + (int64_t)k0 * x.ld + s0, *Wp = W + x.off + s0;
             double *C = a + (int64_t)s0 * x.ld + s0;
-            g1.push_back({V, Wp, C, n, n, kb, x.ld, x.ld, x.ld, -1.0, 1.0});
-            g2.push_back({Wp, V, C, n, n, kb, x.ld, x.ld, x.ld, -1.0, 1.0});
+            g1.push_back({V, Wp, C, n, n, kb, x.ld, x.ld, x.ld, -1.0, 1.0, 1, 0});
+            g2.push_back({Wp, V, C, n, n, kb, x.ld, x.ld, x.ld, -1.0, 1.0, 1, 0});
         }
     }
     start[npanel] = (int)g1.size();
@@ -294,8 +337,8 @@ int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
         }
         const int c = start[p + 1] - start[p];
         if (c > 0) {  // two launches: both products update the same trailing block
-            PTB_TRY(gemm_vbatch(ctx, false, true, dg1 + start[p], c, max_m, max_m));
-            PTB_TRY(gemm_vbatch(ctx, false, true, dg2 + start[p], c, max_m, max_m));
+            PTB_TRY(gemm_vbatch(ctx, false, true, dg1 + start[p], c, max_m, max_m, "sytrd"));
+            PTB_TRY(gemm_vbatch(ctx, false, true, dg2 + start[p], c, max_m, max_m, "sytrd"));
         }
     }
     hipLaunchKernelGGL(k_sytrd_finish, dim3((nsm + 255) / 256), dim3(256), 0, ctx->stream, ddesc, A,
