@@ -201,6 +201,33 @@ int ptbgpu_purify(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *pur, int64_t ndim,
 int ptbgpu_purify_dev(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *pur, int64_t ndim,
                       const double *dH, const double *dS, double *dP, ptbgpu_pur_report *rep);
 
+/* ---- dense spectral routines ----------------------------------------------
+ * matrix_sign(ndim,A,S) and matrix_root(ndim,A,sqrtAinv) (source/sm.f90:410-457):
+ * S = V sgn(e) V^T (sgn 0 for e == 0) and V e^-1/2 V^T from the eigenpairs of
+ * the symmetric A (dense ndim x ndim, column-major; upper triangle read, as
+ * dsyevd('U')).  ndim <= 2048. */
+int ptbgpu_matrix_sign(ptbgpu_ctx *ctx, int64_t ndim, const double *A, double *S);
+int ptbgpu_matrix_root(ptbgpu_ctx *ctx, int64_t ndim, const double *A, double *sqrtAinv);
+
+/* FERMISMEAR(prt,norbs,nel,t,eig,occ,fod,e_fermi,s) (source/fermi.f:5-67),
+ * host only: Fermi occupations at temperature t (K) of the ascending eig[norbs]
+ * with nel occupied (1 <= nel < norbs); fod, e_fermi, s may be NULL. */
+int ptbgpu_fermismear(int64_t norbs, int64_t nel, double t, const double *eig, double *occ,
+                      double *fod, double *e_fermi, double *s);
+
+/* solve2(mode,ndim,nel,nopen,homo,et,focc,H,S,P,e,U,fail)
+ * (source/pgtb.f90:1066-1170), the -check reference density: full dsygvd of
+ * the packed H, S; for mode >= 2 the eigenvalues are shifted to
+ * e + shift0 + shift1 e (glob_par(15), glob_par(17) of the caller,
+ * :1139-1143); occupations from occu + FERMISMEAR at et (K) for alpha and beta
+ * (source/fermi.f); P = U diag(focc) U^T packed.  e[ndim] (ascending),
+ * focc[ndim], U[ndim*ndim] (column-major, columns in e's order) and fail
+ * (1 when the overlap is not positive definite: dsygvd INFO != 0) may be NULL.
+ * ndim <= 2048. */
+int ptbgpu_solve2(ptbgpu_ctx *ctx, int64_t mode, int64_t ndim, int64_t nel, int64_t nopen,
+                  double et, double shift0, double shift1, const double *H, const double *S,
+                  double *P, double *e, double *focc, double *U, int64_t *fail);
+
 #ifdef __cplusplus
 }
 #endif

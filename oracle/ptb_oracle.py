@@ -804,12 +804,17 @@ def occu(ndim: int, nel: int, nopen: int = 0):
     return focca, foccb, ihomoa, ihomob
 
 
-def solve2(ndim: int, nel: int, Hvec, Svec, et: float = 300.0, nopen: int = 0):
-    """Dense reference P (iteration-1 form: no eigenvalue shift) with Fermi
-    smearing (source/pgtb.f90:1066-1170).  Returns (packed P, e, focc)."""
+def solve2(ndim: int, nel: int, Hvec, Svec, et: float = 300.0, nopen: int = 0, mode: int = 1,
+           shift=(0.0, 0.0)):
+    """Dense reference P with Fermi smearing (source/pgtb.f90:1066-1170); for
+    mode >= 2 the eigenvalues are shifted to e + shift[0] + shift[1] e
+    (glob_par(15), glob_par(17), :1139-1143) before the smearing.  Returns
+    (packed P, e, focc)."""
     H = blowsym(Hvec, ndim)
     S = blowsym(Svec, ndim)
     e, U = sla.eigh(H, S, lower=False, driver="gvd")
+    if mode >= 2:
+        e = e + shift[0] + shift[1] * e
     focca, foccb, ia, ib = occu(ndim, nel, nopen)
     if ia + 1 <= ndim:
         focca, _, _, _ = fermismear(ndim, ia, et, e)

@@ -42,15 +42,6 @@ __global__ void k_pack(const double *__restrict__ A, int64_t n, double *__restri
     }
 }
 
-// the symmetric matrix an upper-triangle eigensolver (dsyevd with 'U') sees:
-// B(i,j) = M(min(i,j), max(i,j)), into leading dimension ld
-__global__ void k_load_upper(const double *__restrict__ M, int64_t n, double *__restrict__ B,
-                             int64_t ld) {
-    for (int64_t j = blockIdx.x; j < n; j += gridDim.x)
-        for (int64_t i = threadIdx.x; i < n; i += kT)
-            B[j * ld + i] = i <= j ? M[j * n + i] : M[i * n + j];
-}
-
 // C = a A + b B + c I   (B may be null; C may alias A or B)
 __global__ void k_axpby(double a, const double *A, double b, const double *B, double c,
                         int64_t n, double *C) {
@@ -225,45 +216,6 @@ struct Dense {
     }
 };
 
-// one-matrix batch of the engine's symmetric eigensolver
-struct Eig {
-    std::vector<SubDesc> desc;
-    SubDesc *ddesc = nullptr;
-    int ld = 0;
-    double *A = nullptr, *Z = nullptr, *W = nullptr, *X = nullptr, *vec = nullptr;
-
-    int setup(ptbgpu_ctx *ctx, int64_t n) {
-        ld = (int)((n + 3) & ~3);
-        desc = {SubDesc{0, (int)n, ld, 0, 0, 0, 0}};
-        const size_t ab = sizeof(double) * (size_t)ld * n;
-        A = static_cast<double *>(ptb::ws(ctx, "pur_eA", ab));
-        Z = static_cast<double *>(ptb::ws(ctx, "pur_eZ", ab));
-        W = static_cast<double *>(ptb::ws(ctx, "pur_eW", ab));
-        X = static_cast<double *>(ptb::ws(ctx, "pur_eX", ab));
-        vec = static_cast<double *>(ptb::ws(ctx, "pur_evec", sizeof(double) * 8 * n + 4096));
-        ddesc = static_cast<SubDesc *>(ptb::ws(ctx, "pur_edesc", sizeof(SubDesc)));
-        if (!A || !Z || !W || !X || !vec || !ddesc) return PTBGPU_E_NOMEM;
-        PTB_HIP(ctx, hipMemcpy(ddesc, desc.data(), sizeof(SubDesc), hipMemcpyHostToDevice));
-        return PTBGPU_OK;
-    }
-    // eigenpairs of the symmetric matrix given by M's upper triangle:
-    // eigenvectors into V (leading dimension n), eigenvalues into w (host)
-    int solve(ptbgpu_ctx *ctx, const double *M, int64_t n, double *V, std::vector<double> &w) {
-        const int gb = (int)std::min<int64_t>(n, 2048);
-        hipLaunchKernelGGL(k_load_upper, dim3(gb), dim3(kT), 0, ctx->stream, M, n, A, (int64_t)ld);
-        PTB_CHECK_LAUNCH(ctx);
-        double *dvec = vec, *evec = vec + n, *tau = vec + 2 * n, *rot = vec + 3 * n;
-        PTB_TRY(ptb::eig_sym_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot));
-        PTB_HIP(ctx, hipMemcpy2DAsync(V, sizeof(double) * n, Z, sizeof(double) * ld,
-                                      sizeof(double) * n, n, hipMemcpyDeviceToDevice, ctx->stream));
-        w.resize(n);
-        PTB_HIP(ctx, hipMemcpyAsync(w.data(), dvec, sizeof(double) * n, hipMemcpyDeviceToHost,
-                                    ctx->stream));
-        PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        return PTBGPU_OK;
-    }
-};
-
 const char *kNaNPur = "Error: NaN is encountered during purification";
 
 }  // namespace
@@ -317,8 +269,6 @@ int ptbgpu_purify_dev(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *ps, int64_t n,
     hipLaunchKernelGGL(k_unpack, dim3(ub), dim3(kT), 0, ctx->stream, dHp, n, H);
     hipLaunchKernelGGL(k_unpack, dim3(ub), dim3(kT), 0, ctx->stream, dSp, n, S);
     PTB_CHECK_LAUNCH(ctx);
-    Eig E;
-    if (need_eig) PTB_TRY(E.setup(ctx, n));
     std::vector<double> w, h;
     double r[4];
     int64_t metric_cycles = 0;
@@ -351,7 +301,7 @@ int ptbgpu_purify_dev(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *ps, int64_t n,
             if (r[3] > 0.0) return ptb::fail(ctx, PTBGPU_E_NAN, "Error, NaN encountered");
         }
     } else {
-        PTB_TRY(E.solve(ctx, S, n, T1, w));                             // dsyevd, :191
+        PTB_TRY(ptb::dense_eig(ctx, S, n, T1, w, "pur_e"));               // dsyevd, :191
         std::vector<double> f(n, 0.0);   // 'sqrt' selects no branch: D stays 0
         for (int64_t i = 0; i < n; ++i) {
             if (ps->metric == INV) f[i] = 1.0 / w[i];
@@ -437,7 +387,7 @@ int ptbgpu_purify_dev(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *ps, int64_t n,
                 // sign_diagonalization_ (:650-782): the guess is diagonalised on
                 // every call (the SAVEd first_time is never cleared, :671,692);
                 // chempot is intent(inout) and returns adjusted (:659)
-                PTB_TRY(E.solve(ctx, G, n, T1, w));                      // Q, lambda
+                PTB_TRY(ptb::dense_eig(ctx, G, n, T1, w, "pur_e"));       // Q, lambda
                 PTB_TRY(D.col_sums(T1, h));                              // |Q(:,i)|^2 at 2n + i
                 std::vector<double> sgn(n, 0.0);
                 double inc = ps->chempot_increment, lo = 0.0, up = 0.0;

@@ -94,6 +94,11 @@ int eig_sym_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
                   double *tau, double *rot);
 
+// dense_solve.hip: eigenpairs of one dense symmetric matrix (upper triangle of
+// M, leading dimension n), eigenvalues ascending
+int dense_eig(ptbgpu_ctx *ctx, const double *M, int64_t n, double *V, std::vector<double> &w,
+              const char *pfx);
+
 // dc.hip
 int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int nleaves,
               double *d, const double *e, double *Z);

@@ -111,6 +111,15 @@ def _lib():
     for name in ("ptbgpu_get_nel", "ptbgpu_band_energy", "ptbgpu_idempotency"):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [VP, I64, VP, VP, ctypes.POINTER(F64)]
+    for name in ("ptbgpu_matrix_sign", "ptbgpu_matrix_root"):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = [VP, I64, VP, VP]
+    if hasattr(lib, "ptbgpu_fermismear"):
+        lib.ptbgpu_fermismear.argtypes = [I64, I64, F64, VP, VP, ctypes.POINTER(F64),
+                                          ctypes.POINTER(F64), ctypes.POINTER(F64)]
+    if hasattr(lib, "ptbgpu_solve2"):
+        lib.ptbgpu_solve2.argtypes = [VP, I64, I64, I64, I64, F64, F64, F64, VP, VP, VP, VP, VP,
+                                      VP, ctypes.POINTER(I64)]
     for name in ("ptbgpu_purify", "ptbgpu_purify_dev"):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [VP, ctypes.POINTER(PurSettings), I64, VP, VP, VP,
@@ -133,6 +142,20 @@ def _f64(a, n, name):
     if a.size < n:
         raise ValueError(f"{name} has {a.size} elements, needs {n}")
     return a
+
+
+def fermismear(norbs, nel, t, eig):
+    """FERMISMEAR (source/fermi.f:5-67) on the host through the C ABI (no
+    device context): returns (occ, e_fermi, fod, s)."""
+    lib = _lib()
+    e = np.ascontiguousarray(eig, dtype=np.float64)
+    occ = np.zeros(norbs)
+    fod, ef, s = F64(), F64(), F64()
+    st = lib.ptbgpu_fermismear(norbs, nel, t, _ptr(e), _ptr(occ), ctypes.byref(fod),
+                               ctypes.byref(ef), ctypes.byref(s))
+    if st != 0:
+        raise PtbGpuError(st, lib.ptbgpu_strerror(st).decode())
+    return occ, ef.value, fod.value, s.value
 
 
 class Engine:
@@ -281,6 +304,40 @@ class Engine:
         self._check(self.lib.ptbgpu_purify_dev(self.ctx, ctypes.byref(pur), ndim, _ptr(dH),
                                                _ptr(dS), _ptr(dP), ctypes.byref(rep)))
         return rep.as_dict()
+
+    # ---- dense spectral routines ---------------------------------------------
+    def _spectral(self, fn, A):
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        n = A.shape[0]
+        out = np.zeros((n, n))
+        # column-major in the ABI: pass A^T's buffer, read out^T back
+        a = np.ascontiguousarray(A.T)
+        self._check(fn(self.ctx, n, _ptr(a), _ptr(out)))
+        return np.ascontiguousarray(out.T)
+
+    def matrix_sign(self, A):
+        """matrix_sign (source/sm.f90:410-435) of the dense symmetric A."""
+        return self._spectral(self.lib.ptbgpu_matrix_sign, A)
+
+    def matrix_root(self, A):
+        """matrix_root (source/sm.f90:436-457): A^-1/2 of the dense SPD A."""
+        return self._spectral(self.lib.ptbgpu_matrix_root, A)
+
+    def solve2(self, mode, ndim, nel, nopen, et, H, S, shift=(0.0, 0.0)):
+        """solve2 (source/pgtb.f90:1066-1170) on packed H, S: returns
+        (P packed, e ascending, focc, U (ndim x ndim), fail)."""
+        npk = ndim * (ndim + 1) // 2
+        _f64(H, npk, "H")
+        _f64(S, npk, "S")
+        P = np.zeros(npk)
+        e = np.zeros(ndim)
+        focc = np.zeros(ndim)
+        U = np.zeros((ndim, ndim))
+        fail = I64(0)
+        self._check(self.lib.ptbgpu_solve2(self.ctx, mode, ndim, nel, nopen, et, shift[0],
+                                           shift[1], _ptr(H), _ptr(S), _ptr(P), _ptr(e),
+                                           _ptr(focc), _ptr(U), ctypes.byref(fail)))
+        return P, e, focc, np.ascontiguousarray(U.T), bool(fail.value)
 
     # ---- metrics -----------------------------------------------------------
     def _scalar(self, fn, ndim, A, B):
