@@ -57,6 +57,15 @@ typedef struct {
                              PTBGPU_E_NOT_SPD; 0: count it in the report and go on */
     int64_t verbose;      /* 1: print the reference's progress lines */
     int64_t refine_steps; /* MIXED only: FP64 refinement steps, 0 = default */
+    /* submatrix_columns 4 (k-means): the uniform random numbers k-means++ draws,
+     * restarted for every candidate as the reference's initseed does
+     * (source/sm.f90:20-32,459-470): stream[0..nstream), nstream >= 64 covers
+     * every candidate.  A Fortran host fills it from RANDOM_NUMBER after
+     * RANDOM_SEED(PUT = 42*(i-1)) and one discarded draw, which reproduces the
+     * reference's groups with the same compiler.  NULL: the engine's own
+     * generator (splitmix64 seeded with 42), restarted per candidate. */
+    const double *kmeans_stream;
+    int64_t nstream;
 } ptbgpu_opts;
 
 /* Phase slots of ptbgpu_report.t_phase (labels of source/sm.f90:76-386). */

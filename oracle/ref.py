@@ -79,6 +79,14 @@ def sm_stupid_simple(ndim, nelref, H, S, n, xyz, aoat, cols, mode, log=None):
     return P
 
 
+def random_stream(count):
+    """The compiler's RANDOM_NUMBER values after initseed, in the order KMPP
+    draws them."""
+    out = np.zeros(count)
+    lib().ref_random_stream(I64(count), _p(out))
+    return out
+
+
 def clustering_kmeans(n, ndim, xyz, H, S, aoat, nsm):
     """Column groups (0-based arrays) and effort of one k-means candidate."""
     x = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(n, 3))

@@ -23,6 +23,19 @@ contains
    end subroutine
 
    ! clustering_kmeans (source/sm.f90:459-514) for one nsm: groups and effort
+   ! The RANDOM_NUMBER stream KMPP consumes after initseed
+   ! (source/sm.f90:20-32 reseeds before every k-means candidate).
+   subroutine ref_random_stream(count, out) bind(C, name="ref_random_stream")
+      use sm, only: initseed
+      integer(c_int64_t), value :: count
+      real(c_double), intent(out) :: out(count)
+      integer(c_int64_t) :: i
+      call initseed()
+      do i = 1, count
+         call random_number(out(i))
+      end do
+   end subroutine
+
    subroutine ref_clustering_kmeans(n, ndim, xyz, Hp, Sp, aoat, nsm, ncomb, comb, effort) &
          bind(C, name="ref_clustering_kmeans")
       use sm, only: clustering_kmeans

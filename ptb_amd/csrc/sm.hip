@@ -65,9 +65,126 @@ int column_groups(int64_t mode, int64_t ndim, const int64_t *aoat, std::vector<i
     } else if (mode == 3) {
         gptr.push_back((int)ndim);
     } else {
-        return PTBGPU_E_MODE;
+        return PTBGPU_E_MODE;  // 4 (k-means) is handled by kmeans_groups
     }
     return PTBGPU_OK;
+}
+
+// The uniform numbers k-means++ draws: the caller's stream (the compiler's
+// RANDOM_NUMBER values after the reference's initseed), or splitmix64 seeded
+// with 42.  Restarted for every candidate, as initseed reseeds
+// (source/sm.f90:20-32,469).
+struct Uniform {
+    const double *stream;
+    int64_t len, pos = 0;
+    uint64_t state = 42;
+    Uniform(const double *s, int64_t n) : stream(s), len(n) {}
+    bool next(double &u) {
+        if (stream) {
+            if (pos >= len) return false;
+            u = stream[pos++];
+            return true;
+        }
+        state += 0x9E3779B97F4A7C15ull;
+        uint64_t z = state;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        u = (double)(z >> 11) * 0x1.0p-53;
+        return true;
+    }
+};
+
+// k-means++ seeding and Lloyd iterations (KMPP, source/kmpp.f:24-179) on the
+// atom positions x(3, n) (column-major): z receives 1-based cluster labels.
+// Returns the reference's IFAULT: 0 converged, 1 empty cluster, 2 no
+// convergence in 1000 sweeps, 3 bad k; -1 when the stream runs out.
+int kmpp(const double *x, int n, int k, Uniform &rng, std::vector<int> &z) {
+    const double big = (double)1e33f;  // REAL(4) literal
+    z.assign(n, 0);
+    if (k < 1 || k > n) return 3;
+    std::vector<double> c(3 * (size_t)k, 0.0), work(n, big), cnt(k);
+    double w;
+    if (!rng.next(w)) return -1;
+    int i1 = std::min((int)(w * (double)(float)n) + 1, n);
+    for (int j = 0; j < 3; ++j) c[j] = x[3 * (i1 - 1) + j];
+    for (int l = 2; l <= k; ++l) {
+        double tot = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double best = work[i], d2 = 0.0;
+            bool skip = false;
+            for (int j = 0; j < 3; ++j) {
+                const double d = x[3 * i + j] - c[3 * (l - 2) + j];
+                d2 += d * d;
+                if (d2 >= best) {
+                    skip = true;
+                    break;
+                }
+            }
+            if (!skip && d2 < best) {
+                best = d2;
+                work[i] = best;
+            }
+            tot += best;
+        }
+        if (!rng.next(w)) return -1;
+        w *= tot;
+        tot = 0.0;
+        i1 = n;
+        for (int i = 0; i < n; ++i) {
+            i1 = i + 1;
+            tot += work[i];
+            if (tot > w) break;
+        }
+        for (int j = 0; j < 3; ++j) c[3 * (l - 1) + j] = x[3 * (i1 - 1) + j];
+    }
+    for (int sweep = 0; sweep < 1000; ++sweep) {
+        bool change = false;
+        for (int i = 0; i < n; ++i) {
+            int l1 = 0;
+            double best = big;
+            for (int l = 1; l <= k; ++l) {
+                double d2 = 0.0;
+                for (int j = 0; j < 3; ++j) {
+                    const double d = x[3 * i + j] - c[3 * (l - 1) + j];
+                    d2 += d * d;
+                    if (d2 >= best) break;
+                }
+                if (d2 < best) {
+                    best = d2;
+                    l1 = l;
+                }
+            }
+            if (z[i] != l1) {
+                z[i] = l1;
+                change = true;
+            }
+        }
+        if (!change) return 0;
+        std::fill(cnt.begin(), cnt.end(), 0.0);
+        std::fill(c.begin(), c.end(), 0.0);
+        for (int i = 0; i < n; ++i) {
+            cnt[z[i] - 1] += 1.0;
+            for (int j = 0; j < 3; ++j) c[3 * (z[i] - 1) + j] += x[3 * i + j];
+        }
+        for (int l = 0; l < k; ++l)
+            if (cnt[l] < 0.5) return 1;
+        for (int l = 0; l < k; ++l)
+            for (int j = 0; j < 3; ++j) c[3 * l + j] *= 1.0 / cnt[l];
+    }
+    return 2;
+}
+
+// Columns of every cluster: atom a's columns go to group z(a) - 1, ascending
+// (source/sm.f90:476-490).  Appends nsm groups to gptr / gcols.
+void cluster_columns(const std::vector<int> &z, int nsm, int64_t ndim, const int64_t *aoat,
+                     std::vector<int> &gptr, std::vector<int> &gcols) {
+    std::vector<std::vector<int>> groups(nsm);
+    for (int64_t j = 0; j < ndim; ++j) groups[z[aoat[j] - 1] - 1].push_back((int)j);
+    for (int g = 0; g < nsm; ++g) {
+        gcols.insert(gcols.end(), groups[g].begin(), groups[g].end());
+        gptr.push_back((int)gcols.size());
+    }
 }
 
 void build_tree(int s, int lo, int hi, std::vector<DcNode> &leaves,
@@ -96,9 +213,68 @@ int upload(ptbgpu_ctx *ctx, const char *name, const std::vector<T> &v, T **out) 
     return PTBGPU_OK;
 }
 
+// k-means column groups (submatrix_columns 4, source/sm.f90:129-157): for
+// nsm = 1 .. min(64, n) cluster the atoms, take every cluster's columns and
+// cost the candidate by sum m^2.5 over its submatrix dimensions; the first
+// minimum wins.  All candidates' row unions are counted in one launch on the
+// pattern bitmap.
+int kmeans_groups(ptbgpu_ctx *ctx, int64_t ndim, int64_t n, const double *xyz,
+                  const int64_t *aoat, const uint32_t *bits, int64_t words,
+                  const ptbgpu_opts &opts, std::vector<int> &gptr, std::vector<int> &gcols) {
+    if (!xyz || n < 1) return ptb::fail(ctx, PTBGPU_E_ARG, "k-means needs the atom positions");
+    for (int64_t j = 0; j < ndim; ++j)
+        if (aoat[j] < 1 || aoat[j] > n) return ptb::fail(ctx, PTBGPU_E_ARG, "aoat out of range");
+    const int kmax = (int)std::min<int64_t>(64, n);
+    std::vector<int> cptr{0}, ccols, z;
+    std::vector<int> first(kmax + 1, 0);
+    for (int nsm = 1; nsm <= kmax; ++nsm) {
+        Uniform rng(opts.kmeans_stream, opts.nstream);
+        const int ifault = kmpp(xyz, (int)n, nsm, rng, z);
+        if (ifault == -1)
+            return ptb::fail(ctx, PTBGPU_E_ARG, "k-means stream of %lld numbers too short",
+                             (long long)opts.nstream);
+        if (ifault != 0)
+            return ptb::fail(ctx, PTBGPU_E_KMEANS, "kmeans failed %d (nsm %d)", ifault, nsm);
+        first[nsm - 1] = (int)cptr.size() - 1;
+        cluster_columns(z, nsm, ndim, aoat, cptr, ccols);
+    }
+    const int total = (int)cptr.size() - 1;
+    int *dcptr, *dccols;
+    PTB_TRY(upload(ctx, "km_gptr", cptr, &dcptr));
+    PTB_TRY(upload(ctx, "km_gcols", ccols, &dccols));
+    auto *uni = static_cast<uint32_t *>(ptb::ws(ctx, "km_union", sizeof(uint32_t) * total * words));
+    auto *dcount = static_cast<int *>(ptb::ws(ctx, "km_count", sizeof(int) * total));
+    if (!uni || !dcount) return PTBGPU_E_NOMEM;
+    PTB_TRY(ptb::group_union(ctx, bits, words, dcptr, dccols, total, uni, dcount));
+    std::vector<int> m(total);
+    PTB_HIP(ctx, hipMemcpyAsync(m.data(), dcount, sizeof(int) * total, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double best = HUGE_VAL;
+    int best_nsm = 1;
+    for (int nsm = 1; nsm <= kmax; ++nsm) {
+        double effort = 0.0;
+        for (int g = 0; g < nsm; ++g) effort += std::pow((double)m[first[nsm - 1] + g], 2.5);
+        if (opts.verbose) printf(" kmeans %d %.17g\n", nsm, effort);
+        if (effort < best) {
+            best = effort;
+            best_nsm = nsm;
+        }
+    }
+    gptr.assign(1, 0);
+    gcols.clear();
+    const int g0 = first[best_nsm - 1];
+    for (int g = 0; g < best_nsm; ++g) {
+        gcols.insert(gcols.end(), ccols.begin() + cptr[g0 + g], ccols.begin() + cptr[g0 + g + 1]);
+        gptr.push_back((int)gcols.size());
+    }
+    return PTBGPU_OK;
+}
+
 int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const double *dH,
-                      const double *dS, double *dP, const int64_t *aoat, int64_t cols,
-                      int64_t mode, const ptbgpu_opts &opts, ptbgpu_report &rep) {
+                      const double *dS, double *dP, int64_t natoms, const double *xyz,
+                      const int64_t *aoat, int64_t cols, int64_t mode, const ptbgpu_opts &opts,
+                      ptbgpu_report &rep) {
     hipStream_t st = ctx->stream;
     Timer tm;
     tm.init(st);
@@ -108,22 +284,27 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
                          "(the reference stops, source/sm.f90:249-251,312-314)", (long long)mode);
     if (opts.precision != PTBGPU_FP64 && opts.precision != PTBGPU_MIXED)
         return ptb::fail(ctx, PTBGPU_E_ARG, "unknown precision %lld", (long long)opts.precision);
+    // ---- coupling pattern (the k-means candidates are costed on it)
+    const int64_t words = (ndim + 31) / 32;
+    auto *bits = static_cast<uint32_t *>(ptb::ws(ctx, "sm_bits", sizeof(uint32_t) * ndim * words));
+    if (!bits) return PTBGPU_E_NOMEM;
+    PTB_TRY(ptb::pattern_bits(ctx, dH, dS, ndim, words, bits));
     // ---- combination
     std::vector<int> gptr, gcols;
-    if (column_groups(cols, ndim, aoat, gptr, gcols) != PTBGPU_OK)
+    if (cols == 4) {
+        PTB_TRY(kmeans_groups(ctx, ndim, natoms, xyz, aoat, bits, words, opts, gptr, gcols));
+    } else if (column_groups(cols, ndim, aoat, gptr, gcols) != PTBGPU_OK) {
         return ptb::fail(ctx, PTBGPU_E_MODE, "submatrix_columns %lld not implemented", (long long)cols);
+    }
     const int nsm = (int)gptr.size() - 1;
     int *dgptr, *dgcols;
     PTB_TRY(upload(ctx, "sm_gptr", gptr, &dgptr));
     PTB_TRY(upload(ctx, "sm_gcols", gcols, &dgcols));
     tm.mark(1, st);
-    // ---- stats: pattern, unions, dimensions
-    const int64_t words = (ndim + 31) / 32;
-    auto *bits = static_cast<uint32_t *>(ptb::ws(ctx, "sm_bits", sizeof(uint32_t) * ndim * words));
+    // ---- stats: unions, dimensions
     auto *uni = static_cast<uint32_t *>(ptb::ws(ctx, "sm_union", sizeof(uint32_t) * nsm * words));
     auto *dcount = static_cast<int *>(ptb::ws(ctx, "sm_count", sizeof(int) * nsm));
-    if (!bits || !uni || !dcount) return PTBGPU_E_NOMEM;
-    PTB_TRY(ptb::pattern_bits(ctx, dH, dS, ndim, words, bits));
+    if (!uni || !dcount) return PTBGPU_E_NOMEM;
     PTB_TRY(ptb::group_union(ctx, bits, words, dgptr, dgcols, nsm, uni, dcount));
     std::vector<int> mcount(nsm);
     PTB_HIP(ctx, hipMemcpyAsync(mcount.data(), dcount, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
@@ -386,13 +567,12 @@ int ptbgpu_sm_density_dev(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const d
                           const int64_t *aoat, int64_t submatrix_columns,
                           int64_t submatrix_mode, const ptbgpu_opts *opts, ptbgpu_report *rep) {
     PTB_TRY(check_sm_args(ctx, ndim, nelref, dH, dS, dP, n, aoat));
-    (void)xyz;
     PTB_HIP(ctx, hipSetDevice(ctx->device));
-    ptbgpu_opts o = opts ? *opts : ptbgpu_opts{PTBGPU_FP64, 0, 0, 0};
+    ptbgpu_opts o = opts ? *opts : ptbgpu_opts{PTBGPU_FP64, 0, 0, 0, nullptr, 0};
     ptbgpu_report r;
     memset(&r, 0, sizeof(r));
     auto t0 = std::chrono::steady_clock::now();
-    int st = sm_density_device(ctx, ndim, nelref, dH, dS, dP, aoat, submatrix_columns,
+    int st = sm_density_device(ctx, ndim, nelref, dH, dS, dP, n, xyz, aoat, submatrix_columns,
                                submatrix_mode, o, r);
     r.t_phase[PTBGPU_T_TOTAL] =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

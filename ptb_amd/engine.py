@@ -44,7 +44,8 @@ class PtbGpuError(RuntimeError):
 
 
 class Opts(ctypes.Structure):
-    _fields_ = [("precision", I64), ("strict", I64), ("verbose", I64), ("refine_steps", I64)]
+    _fields_ = [("precision", I64), ("strict", I64), ("verbose", I64), ("refine_steps", I64),
+                ("kmeans_stream", ctypes.c_void_p), ("nstream", I64)]
 
 
 class Report(ctypes.Structure):
@@ -238,13 +239,18 @@ class Engine:
                 "average": tot // n if n else 0}
 
     # ---- submatrix density build -----------------------------------------
-    def _opts(self, precision, strict, verbose, refine_steps):
-        return Opts(PRECISION[precision] if isinstance(precision, str) else int(precision),
-                    1 if strict else 0, 1 if verbose else 0, int(refine_steps))
+    def _opts(self, precision, strict, verbose, refine_steps, kmeans_stream=None):
+        o = Opts(PRECISION[precision] if isinstance(precision, str) else int(precision),
+                 1 if strict else 0, 1 if verbose else 0, int(refine_steps), None, 0)
+        if kmeans_stream is not None:
+            self._stream = np.ascontiguousarray(kmeans_stream, dtype=np.float64)
+            o.kmeans_stream = self._stream.ctypes.data
+            o.nstream = self._stream.size
+        return o
 
     def sm_stupid_simple(self, ndim, nelref, Hvec, Svec, Pvec, n, xyz, aoat,
                          submatrix_columns, submatrix_mode, precision="fp64", strict=False,
-                         verbose=False, refine_steps=0) -> dict:
+                         verbose=False, refine_steps=0, kmeans_stream=None) -> dict:
         """Drop-in for sm_stupid_simple on host arrays; Pvec is overwritten."""
         npk = ndim * (ndim + 1) // 2
         _f64(Hvec, npk, "Hvec")
@@ -255,7 +261,7 @@ class Engine:
         if a.size != ndim:
             raise ValueError("aoat must have ndim entries")
         rep = Report()
-        opts = self._opts(precision, strict, verbose, refine_steps)
+        opts = self._opts(precision, strict, verbose, refine_steps, kmeans_stream)
         self._check(self.lib.ptbgpu_sm_density(
             self.ctx, ndim, nelref, _ptr(Hvec), _ptr(Svec), _ptr(Pvec), n, _ptr(x), _ptr(a),
             submatrix_columns, submatrix_mode, ctypes.byref(opts), ctypes.byref(rep)))
@@ -263,12 +269,13 @@ class Engine:
 
     def sm_stupid_simple_dev(self, ndim, nelref, dH, dS, dP, n, xyz, aoat,
                              submatrix_columns, submatrix_mode, precision="fp64",
-                             strict=False, verbose=False, refine_steps=0) -> dict:
+                             strict=False, verbose=False, refine_steps=0,
+                             kmeans_stream=None) -> dict:
         """Same with device-resident packed H, S, P (torch CUDA tensors)."""
         x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(n, 3)
         a = np.ascontiguousarray(aoat, dtype=np.int64)
         rep = Report()
-        opts = self._opts(precision, strict, verbose, refine_steps)
+        opts = self._opts(precision, strict, verbose, refine_steps, kmeans_stream)
         self._check(self.lib.ptbgpu_sm_density_dev(
             self.ctx, ndim, nelref, _ptr(dH), _ptr(dS), _ptr(dP), n, _ptr(x), _ptr(a),
             submatrix_columns, submatrix_mode, ctypes.byref(opts), ctypes.byref(rep)))

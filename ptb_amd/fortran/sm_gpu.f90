@@ -149,6 +149,8 @@ module sm
 
    type, bind(C) :: ptbgpu_opts
       integer(c_int64_t) :: precision, strict, verbose, refine_steps
+      type(c_ptr) :: kmeans_stream
+      integer(c_int64_t) :: nstream
    end type
 
    type, bind(C) :: ptbgpu_report
@@ -190,6 +192,7 @@ contains
       type(ptbgpu_opts) :: opts
       type(ptbgpu_report) :: rep
       real(c_double), allocatable :: hc(:), sc(:), pc(:)
+      real(c_double), allocatable, target :: stream(:)
       integer(c_int64_t) :: npk
       npk = ndim*(ndim + 1)/2
       if (size(Hvec) < npk .or. size(Svec) < npk .or. size(Pvec) < npk) &
@@ -202,11 +205,37 @@ contains
       opts%strict = 0
       opts%verbose = 1
       opts%refine_steps = 0
+      opts%kmeans_stream = c_null_ptr
+      opts%nstream = 0
+      if (submatrix_columns == 4) then
+         call kmeans_stream(stream)
+         opts%kmeans_stream = c_loc(stream)
+         opts%nstream = size(stream, kind=c_int64_t)
+      end if
       call ptbgpu_check(ptbgpu_sm_density(ptbgpu_handle(), ndim, nelref, hc, sc, pc, n, xyz, &
          aoat, submatrix_columns, submatrix_mode, opts, rep))
       Pvec(1:npk) = pc
       print *, "number of submatrices=", rep%nsm
       print *, "Submatrix (GPU)", rep%t_phase(10), "sec"
+   end subroutine
+
+   ! The uniform numbers k-means++ consumes for one candidate: this
+   ! compiler's generator seeded with 42*(i-1) and one draw discarded, which is
+   ! how the reference reseeds before every candidate (source/sm.f90:20-32).
+   subroutine kmeans_stream(stream)
+      real(c_double), allocatable, intent(out) :: stream(:)
+      integer :: nseed, i
+      integer, allocatable :: seed(:)
+      real(c_double) :: discard
+      call random_seed(size=nseed)
+      allocate (seed(nseed))
+      seed = [(42*(i - 1), i=1, nseed)]
+      call random_seed(put=seed)
+      call random_number(discard)
+      allocate (stream(64))
+      do i = 1, 64
+         call random_number(stream(i))
+      end do
    end subroutine
 
 end module sm

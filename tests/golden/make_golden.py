@@ -64,7 +64,7 @@ def parse_mu_trace(log):
 def main():
     if not R.available():
         sys.exit("oracle/_ref/libptbref.so is not built; run make -C oracle first")
-    only = sys.argv[1:]  # optional subset: sm, metrics, pur, search, kmeans
+    only = sys.argv[1:]  # optional subset: sm, metrics, pur, search, kmeans, kmeans_sm
 
     def want(part):
         return not only or part in only
@@ -90,6 +90,8 @@ def main():
         purification_search_cases()
     if want("kmeans"):
         kmeans_groups()
+    if want("kmeans_sm"):
+        kmeans_density()
 
 
 def metrics_and_fermi():
@@ -160,17 +162,47 @@ def purification_search_cases():
         print("pursearch", name, "nel", R.get_nel(mol.ndim, P, S))
 
 
+KMEANS_SM_CASES = {
+    # name: (kind, size, seed, filter); k-means picks 2 and 3 clusters here
+    "w16_kmeans": ("water", (8, 2, 1), 0, 1e-4),
+    "c40_kmeans": ("alkane", 40, 0, 1e-5),
+}
+
+
+def kmeans_density():
+    # sm_stupid_simple with submatrix_columns = 4 (k-means).  Its candidate
+    # loop runs under OpenMP with every candidate reseeding the one generator,
+    # so it is generated with OMP_NUM_THREADS=1 (serial candidate order).
+    if os.environ.get("OMP_NUM_THREADS") != "1":
+        sys.exit("run the kmeans_sm part with OMP_NUM_THREADS=1")
+    for name, (kind, size, seed, filt) in KMEANS_SM_CASES.items():
+        mol, H, S = system(kind, size, seed, filt)
+        log = []
+        P = R.sm_stupid_simple(mol.ndim, mol.nel, H, S, mol.n, mol.xyz, mol.aoat, 4, 2, log=log)
+        trace = parse_mu_trace(log[0])
+        nsm = int(re.search(r"number of submatrices=\s*(\d+)", log[0]).group(1))
+        np.savez_compressed(os.path.join(HERE, f"sm_{name}.npz"), H=H, S=S, xyz=mol.xyz,
+                            aoat=mol.aoat, nel=mol.nel, n=mol.n, ndim=mol.ndim, cols=4,
+                            filter=filt, P=P, mu_trace=trace, nsm=nsm,
+                            stream=R.random_stream(256), eband=R.band_structure(mol.ndim, P, H),
+                            nelp=R.get_nel(mol.ndim, P, S))
+        print(name, mol.ndim, "nsm", nsm, "muiter", len(trace))
+
+
 def kmeans_groups():
-    # k-means groups of the compiled reference (compiler RNG)
-    mol, H, S = system("water", (2, 2, 2), 0, 1e-4)
-    out = {}
-    for nsm in range(1, 5):
-        groups, eff = R.clustering_kmeans(mol.n, mol.ndim, mol.xyz, H, S, mol.aoat, nsm)
-        owner = np.full(mol.ndim, -1)
-        for k, g in enumerate(groups):
-            owner[g] = k
-        out[f"owner_{nsm}"] = owner
-        out[f"effort_{nsm}"] = eff
+    # k-means groups of the compiled reference (compiler RNG) and the
+    # RANDOM_NUMBER stream KMPP draws after initseed, so that the restated
+    # KMPP can be fed the same stream
+    out = {"stream": R.random_stream(256)}
+    for tag, (size, nmax) in {"": ((2, 2, 2), 4), "_w32": ((4, 4, 2), 8)}.items():
+        mol, H, S = system("water", size, 0, 1e-4)
+        for nsm in range(1, nmax + 1):
+            groups, eff = R.clustering_kmeans(mol.n, mol.ndim, mol.xyz, H, S, mol.aoat, nsm)
+            owner = np.full(mol.ndim, -1)
+            for k, g in enumerate(groups):
+                owner[g] = k
+            out[f"owner{tag}_{nsm}"] = owner
+            out[f"effort{tag}_{nsm}"] = eff
     np.savez_compressed(os.path.join(HERE, "kmeans.npz"), **out)
 
 

@@ -42,3 +42,27 @@ def test_fortran_drop_in(tmp_path, cols):
     assert np.abs(P - res.P).max() <= 1e-8
     assert abs(eband - O.band_structure(n, res.P, Hf)) <= 1e-7
     assert abs(nelp - O.get_nel(n, res.P, Sf)) <= 1e-9
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no flang)")
+@pytest.mark.parametrize("fixture", ["sm_w16_kmeans.npz", "sm_c40_kmeans.npz"])
+def test_fortran_kmeans_matches_reference(tmp_path, fixture):
+    """The drop-in draws the k-means numbers from the compiler's generator with
+    the reference's seeding, so mode 4 reproduces the reference's density
+    (tests/golden/sm_*_kmeans.npz, already filtered H and S)."""
+    d = np.load(os.path.join(ROOT, "tests", "golden", fixture))
+    n = int(d["ndim"])
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([n, int(d["nel"]), int(d["n"]), 4, 2], dtype=np.int64).tofile(f)
+        np.array([float(d["filter"])]).tofile(f)
+        np.ascontiguousarray(d["xyz"]).tofile(f)
+        np.ascontiguousarray(d["aoat"], dtype=np.int64).tofile(f)
+        d["H"].tofile(f)
+        d["S"].tofile(f)
+    out = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True,
+                         timeout=120)
+    print(out.stdout[-2000:], out.stderr[-2000:])
+    assert out.returncode == 0
+    P = np.fromfile(fout, dtype=np.float64)[:-2]
+    assert np.abs(P - d["P"]).max() <= 1e-8

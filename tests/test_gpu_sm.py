@@ -75,3 +75,72 @@ def test_unsupported_modes_raise(engine):
     with pytest.raises(PtbGpuError) as ex:
         engine.sm_stupid_simple(mol.ndim, mol.nel, H, S, P, mol.n, mol.xyz, mol.aoat, 5, 2)
     assert ex.value.status == 2
+
+
+def splitmix_stream(count, seed=42):
+    """The engine's own k-means generator (splitmix64 seeded with 42, uniform
+    = top 53 bits / 2^53), restated to feed the oracle the same numbers."""
+    mask = (1 << 64) - 1
+    x, out = seed, []
+    for _ in range(count):
+        x = (x + 0x9E3779B97F4A7C15) & mask
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        z ^= z >> 31
+        out.append((z >> 11) * 2.0 ** -53)
+    return np.array(out)
+
+
+def _restart(stream):
+    def uniform(_nsm):
+        it = iter(stream)
+        return lambda: next(it)
+    return uniform
+
+
+KMEANS_FIXTURES = ["sm_w16_kmeans.npz", "sm_c40_kmeans.npz"]
+
+
+@pytest.mark.parametrize("fixture", KMEANS_FIXTURES)
+def test_kmeans_columns_match_reference(engine, fixture):
+    """submatrix_columns = 4 with the compiler's RANDOM_NUMBER stream: the
+    reference's clusters, nsm choice and density (2 and 3 clusters)."""
+    import os
+    from conftest import ROOT
+    d = np.load(os.path.join(ROOT, "tests", "golden", fixture))
+    n = int(d["ndim"])
+    P = np.zeros_like(d["P"])
+    rep = engine.sm_stupid_simple(n, int(d["nel"]), d["H"].copy(), d["S"].copy(), P, int(d["n"]),
+                                  d["xyz"], d["aoat"], 4, 2, kmeans_stream=d["stream"])
+    err = np.abs(P - d["P"]).max()
+    print(f"k-means nsm {rep['nsm']} mu {rep['mu']} muiter {rep['muiter']}: max|P-P_ref| {err:.3e}")
+    assert rep["nsm"] == int(d["nsm"])
+    assert rep["muiter"] == len(d["mu_trace"])
+    assert rep["mu"] == d["mu_trace"][-1][1]
+    assert err < 1e-8
+
+
+def test_kmeans_columns_own_generator(engine):
+    """Without a stream the engine draws from its own generator; the oracle fed
+    the same numbers gives the same clusters and density."""
+    from oracle import ptb_oracle as O
+    mol, H, S = small_system("water", (8, 2, 1), 1)
+    n = mol.ndim
+    P = np.zeros_like(H)
+    rep = engine.sm_stupid_simple(n, mol.nel, H, S, P, mol.n, mol.xyz, mol.aoat, 4, 2)
+    res = O.sm_stupid_simple(n, mol.nel, H, S, mol.n, mol.xyz, mol.aoat, 4, 2,
+                             uniform=_restart(splitmix_stream(64)))
+    print(f"nsm {rep['nsm']} vs {len(res.groups)}, max|dP| {np.abs(P - res.P).max():.3e}")
+    assert rep["nsm"] == len(res.groups)
+    assert np.abs(P - res.P).max() < 1e-8
+
+
+def test_kmeans_short_stream_is_an_error(engine):
+    from ptb_amd.engine import PtbGpuError
+    mol, H, S = small_system("water", (2, 2, 1), 0)
+    P = np.zeros_like(H)
+    with pytest.raises(PtbGpuError) as ex:
+        engine.sm_stupid_simple(mol.ndim, mol.nel, H, S, P, mol.n, mol.xyz, mol.aoat, 4, 2,
+                                kmeans_stream=np.array([0.5]))
+    assert ex.value.status == 1

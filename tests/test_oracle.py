@@ -29,8 +29,15 @@ def test_submatrix_density_matches_reference(fixture):
     from oracle import ptb_oracle as O
     d = _load(fixture)
     n = int(d["ndim"])
+    uniform = None
+    if "stream" in d:        # k-means: the compiler's RANDOM_NUMBER stream
+        stream = d["stream"]
+
+        def uniform(_nsm):
+            it = iter(stream)
+            return lambda: next(it)
     res = O.sm_stupid_simple(n, int(d["nel"]), d["H"], d["S"], int(d["n"]), d["xyz"], d["aoat"],
-                             int(d["cols"]), 2)
+                             int(d["cols"]), 2, uniform=uniform)
     assert np.abs(res.P - d["P"]).max() < 1e-12
     trace = d["mu_trace"]
     assert res.muiter == len(trace)
@@ -134,9 +141,27 @@ def test_iterative_inverse_sqrt_metric_stops_on_nan():
         O.purification_wrapper(pur, int(d["ndim"]), d["H"], d["S"])
 
 
+def test_kmeans_groups_match_reference_with_compiler_stream():
+    """KMPP (source/kmpp.f:24-179) fed the compiler's RANDOM_NUMBER stream
+    after initseed reproduces the reference's clusters exactly."""
+    from oracle import ptb_oracle as O
+    k = _load("kmeans.npz")
+    for tag, size, nmax in (("", (2, 2, 2), 4), ("_w32", (4, 4, 2), 8)):
+        mol, H, S = small_system("water", size, 0)
+        pat = O.pattern_dense(O.blowsym(H, mol.ndim), O.blowsym(S, mol.ndim))
+        for nsm in range(1, nmax + 1):
+            it = iter(k["stream"])
+            groups, eff = O.clustering_kmeans(mol.n, mol.ndim, mol.xyz, mol.aoat, nsm, pat,
+                                              lambda: next(it))
+            owner = np.full(mol.ndim, -1)
+            for g, cols in enumerate(groups):
+                owner[cols] = g
+            assert np.array_equal(owner, k[f"owner{tag}_{nsm}"])
+            assert abs(eff - float(k[f"effort{tag}_{nsm}"])) <= 1e-12 * eff
+
+
 def test_kmeans_effort_of_reference_groups():
-    """k-means groups depend on the compiler's RNG (parity unpinned); given the
-    reference's groups, the effort sum(m^2.5) must match."""
+    """Given the reference's groups, the effort sum(m^2.5) must match."""
     from oracle import ptb_oracle as O
     k = _load("kmeans.npz")
     mol, H, S = small_system("water", (2, 2, 2), 0)
