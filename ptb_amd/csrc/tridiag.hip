@@ -67,6 +67,31 @@ __device__ __forceinline__ double butterfly16(const double (&p)[16], int lane) {
     return v;
 }
 
+// The same for 8 values per lane: value q ends in the lanes with
+// (lane >> 3) & 7 == q (partners 32, 16, 8 halve, partners 1, 2, 4 finish).
+__device__ __forceinline__ double butterfly8(const double (&p)[8], int lane) {
+    double h4[4], h2[2];
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double send = b5 ? p[j] : p[j + 4];
+        const double keep = b5 ? p[j + 4] : p[j];
+        h4[j] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const double send = b4 ? h4[j] : h4[j + 2];
+        const double keep = b4 ? h4[j + 2] : h4[j];
+        h2[j] = keep + __shfl_xor(send, 16);
+    }
+    const double send = b3 ? h2[0] : h2[1];
+    double v = (b3 ? h2[1] : h2[0]) + __shfl_xor(send, 8);
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    return v;
+}
+
 __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict__ desc,
                                                     double *__restrict__ Aa,
                                                     double *__restrict__ Wa,
@@ -107,7 +132,7 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
             double part_s = 0.0;
             for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
                 double x = a[(int64_t)k * ld + i];
-#pragma unroll 8
+#pragma unroll 4
                 for (int t = 0; t < jj; ++t)
                     x -= a[(int64_t)(k0 + t) * ld + i] * c1[t] + W[(int64_t)t * ld + i] * c2[t];
                 vs[i] = x;
@@ -142,32 +167,32 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
                 tt[k] = tk;
             }
             __syncthreads();
-            // (d) pending dots q1[t] = W[:,t]^T v, q2[t] = V[:,k0+t]^T v, eight t at a time
-            for (int t0 = 0; t0 < jj; t0 += 8) {
-                double acc[16];
+            // (d) pending dots q1[t] = W[:,t]^T v, q2[t] = V[:,k0+t]^T v, four t at a time
+            for (int t0 = 0; t0 < jj; t0 += 4) {
+                double acc[8];
 #pragma unroll
-                for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+                for (int q = 0; q < 8; ++q) acc[q] = 0.0;
                 for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
                     const double v = vs[i];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) {
+                    for (int q = 0; q < 4; ++q) {
                         const int t = min(t0 + q, jj - 1);
                         acc[q] += W[(int64_t)t * ld + i] * v;
-                        acc[8 + q] += a[(int64_t)(k0 + t) * ld + i] * v;
+                        acc[4 + q] += a[(int64_t)(k0 + t) * ld + i] * v;
                     }
                 }
-#pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    const double sq = wave_sum(acc[q]);
-                    if (lane == 0) qred[wid * 16 + q] = sq;
+                {
+                    // wavefront sums: lanes with lane % 8 == 0 hold sum (lane >> 3) & 7
+                    const double sq = butterfly8(acc, lane);
+                    if ((lane & 7) == 0) qred[wid * 8 + ((lane >> 3) & 7)] = sq;
                 }
                 __syncthreads();
-                if ((int)threadIdx.x < 16) {
-                    const int q = threadIdx.x & 7;
+                if ((int)threadIdx.x < 8) {
+                    const int q = threadIdx.x & 3;
                     double sq = 0.0;
-                    for (int w = 0; w < kWaves; ++w) sq += qred[w * 16 + threadIdx.x];
+                    for (int w = 0; w < kWaves; ++w) sq += qred[w * 8 + threadIdx.x];
                     if (t0 + q < jj) {
-                        if (threadIdx.x < 8) q1[t0 + q] = sq;
+                        if (threadIdx.x < 4) q1[t0 + q] = sq;
                         else q2[t0 + q] = sq;
                     }
                 }
@@ -189,46 +214,34 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
                 // block by row block: block bi holds 4 (bi + 1) tiles (the
                 // last one ceil(n / 16)), so tile t lies in the block bi with
                 // 2 bi (bi + 1) <= t < 2 (bi + 1) (bi + 2).  Wavefront wid
-                // takes the tile pairs wid, wid + kWaves, ...
+                // takes the tiles wid, wid + kWaves, ...
                 const int nrb = (n + 63) / 64;
                 const int ntile = 2 * (nrb - 1) * nrb + (n + 15) / 16;
                 double *cpw = cp + wid * m;
-                for (int t0 = 2 * wid; t0 < ntile; t0 += 2 * kWaves) {
-                    double x[2][16];
-                    int irow[2], cbeg[2];
+                for (int t = wid; t < ntile; t += kWaves) {
+                    int bi = (int)((sqrtf(2.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+                    while (2 * (bi + 1) * (bi + 2) <= t) ++bi;
+                    while (bi > 0 && 2 * bi * (bi + 1) > t) --bi;
+                    const int cj = t - 2 * bi * (bi + 1);
+                    const int i = k + 1 + bi * 64 + lane;
+                    const int c0 = k + 1 + cj * 16;
+                    const bool ok = i < m;
+                    double x[16];
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const int t = t0 + h;
-                        int bi = (int)((sqrtf(2.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-                        while (2 * (bi + 1) * (bi + 2) <= t) ++bi;
-                        while (bi > 0 && 2 * bi * (bi + 1) > t) --bi;
-                        const int cj = t - 2 * bi * (bi + 1);
-                        const int i = k + 1 + bi * 64 + lane;
-                        const int c0 = k + 1 + cj * 16;
-                        const bool ok = t < ntile && i < m;
-                        irow[h] = ok ? i : -1;
-                        cbeg[h] = c0;
-#pragma unroll
-                        for (int q = 0; q < 16; ++q) {
-                            const int c = c0 + q;
-                            x[h][q] = (ok && c <= i) ? a[(int64_t)c * ld + i] : 0.0;
-                        }
+                    for (int q = 0; q < 16; ++q) {
+                        const int c = c0 + q;
+                        x[q] = (ok && c <= i) ? a[(int64_t)c * ld + i] : 0.0;
                     }
+                    const double vi = ok ? vs[i] : 0.0;
+                    double yrow = 0.0;
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        if (t0 + h >= ntile) break;
-                        const int i = irow[h], c0 = cbeg[h];
-                        const double vi = i >= 0 ? vs[i] : 0.0;
-                        double yrow = 0.0;
+                    for (int q = 0; q < 16; ++q) yrow += x[q] * vs[min(c0 + q, m - 1)];
 #pragma unroll
-                        for (int q = 0; q < 16; ++q) yrow += x[h][q] * vs[min(c0 + q, m - 1)];
-#pragma unroll
-                        for (int q = 0; q < 16; ++q) x[h][q] = (c0 + q < i) ? x[h][q] * vi : 0.0;
-                        const double colsum = butterfly16(x[h], lane);
-                        const int cq = c0 + ((lane >> 2) & 15);
-                        if ((lane & 3) == 0 && cq < m) cpw[cq] += colsum;
-                        if (i >= 0) cpw[i] += yrow;
-                    }
+                    for (int q = 0; q < 16; ++q) x[q] = (c0 + q < i) ? x[q] * vi : 0.0;
+                    const double colsum = butterfly16(x, lane);
+                    const int cq = c0 + ((lane >> 2) & 15);
+                    if ((lane & 3) == 0 && cq < m) cpw[cq] += colsum;
+                    if (ok) cpw[i] += yrow;
                 }
             }
             __syncthreads();
@@ -243,7 +256,7 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
             double pv = 0.0;
             for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
                 double p = praw[i];
-#pragma unroll 8
+#pragma unroll 4
                 for (int t = 0; t < jj; ++t)
                     p -= a[(int64_t)(k0 + t) * ld + i] * q1[t] + W[(int64_t)t * ld + i] * q2[t];
                 p *= tk;
