@@ -3,8 +3,8 @@
 // * k_gemm:     C = alpha op(A) op(B) + beta C for a variable-size batch whose
 //               shapes and pointers come from device descriptors (so a kernel
 //               earlier in the stream can size the next product).  64x64 output
-//               tile per workgroup, 16-deep k slices staged in LDS, 4x4
-//               accumulators per thread.
+//               tile per workgroup, 32-deep k slices staged in LDS with the
+//               next slice prefetched into registers.
 // * k_potrf_panel: one 32-column panel of the lower Cholesky S = L L^T: the
 //               diagonal block factored and inverted in LDS, the panel below as
 //               X = P L_kk^-T; the trailing update is a batched GEMM.  A
@@ -25,16 +25,21 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 // 64x64 output tile per 256-thread workgroup; each wavefront owns a 32x32
 // quarter as 2x2 blocks of v_mfma_f64_16x16x4.  MFMA operand layout: lane l
 // supplies A(l%16, l/16) of a 16x4 slice and B(l/16, l%16) of a 4x16 slice;
-// result value r of lane l is D(4r + l/16, l%16).
+// result value r of lane l is D(4r + l/16, l%16).  K advances 32 at a time:
+// the next 64x32 / 32x64 slices are loaded into registers (coalesced along
+// the contiguous dimension of each operand) while the 32 MFMAs of the current
+// slices run from LDS.
+constexpr int kKS = 32;
+
 template <bool TA, bool TB>
-__global__ __launch_bounds__(kT) void k_gemm(const GemmDesc *__restrict__ descs) {
+__global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ descs) {
     const GemmDesc g = descs[blockIdx.y];
     const int tilesM = (g.M + 63) / 64, tilesN = (g.N + 63) / 64;
     if (g.M <= 0 || g.N <= 0 || (int)blockIdx.x >= tilesM * tilesN) return;
     const int m0 = (blockIdx.x % tilesM) * 64, n0 = (blockIdx.x / tilesM) * 64;
     if (g.uplo && n0 > m0 + 63) return;  // tile strictly above the diagonal
-    __shared__ double As[16][68];
-    __shared__ double Bs[16][68];
+    __shared__ double As[kKS][68];
+    __shared__ double Bs[kKS][68];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
@@ -43,28 +48,51 @@ __global__ __launch_bounds__(kT) void k_gemm(const GemmDesc *__restrict__ descs)
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < g.K; k0 += 16) {
+    double ra[8], rb[8];
+    // element q of this thread's share: A_op(mm, kk) and B_op(kk, nn)
+    auto a_index = [&](int q, int &mm, int &kk) {
+        const int e = threadIdx.x + q * kT;
+        if (!TA) { mm = e & 63; kk = e >> 6; } else { kk = e & 31; mm = e >> 5; }
+    };
+    auto b_index = [&](int q, int &kk, int &nn) {
+        const int e = threadIdx.x + q * kT;
+        if (!TB) { kk = e & 31; nn = e >> 5; } else { nn = e & 63; kk = e >> 6; }
+    };
+    auto load = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int l = threadIdx.x + q * kT;
+        for (int q = 0; q < 8; ++q) {
             int mm, kk;
-            if (!TA) { mm = l & 63; kk = l >> 6; } else { kk = l & 15; mm = l >> 4; }
+            a_index(q, mm, kk);
             const int gm = m0 + mm, gk = k0 + kk;
-            double v = 0.0;
-            if (gm < g.M && gk < g.K)
-                v = TA ? g.A[(int64_t)gm * g.lda + gk] : g.A[(int64_t)gk * g.lda + gm];
-            As[kk][mm] = v;
-            int nn;
-            if (!TB) { kk = l & 15; nn = l >> 4; } else { nn = l & 63; kk = l >> 6; }
-            const int gn = n0 + nn, gk2 = k0 + kk;
-            double w = 0.0;
-            if (gn < g.N && gk2 < g.K)
-                w = TB ? g.B[(int64_t)gk2 * g.ldb + gn] : g.B[(int64_t)gn * g.ldb + gk2];
-            Bs[kk][nn] = w;
+            ra[q] = (gm < g.M && gk < g.K)
+                        ? (TA ? g.A[(int64_t)gm * g.lda + gk] : g.A[(int64_t)gk * g.lda + gm])
+                        : 0.0;
+            int kb, nn;
+            b_index(q, kb, nn);
+            const int gn = n0 + nn, gkb = k0 + kb;
+            rb[q] = (gn < g.N && gkb < g.K)
+                        ? (TB ? g.B[(int64_t)gkb * g.ldb + gn] : g.B[(int64_t)gn * g.ldb + gkb])
+                        : 0.0;
         }
-        __syncthreads();
+    };
+    auto store = [&]() {
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
+        for (int q = 0; q < 8; ++q) {
+            int mm, kk, kb, nn;
+            a_index(q, mm, kk);
+            b_index(q, kb, nn);
+            As[kk][mm] = ra[q];
+            Bs[kb][nn] = rb[q];
+        }
+    };
+    load(0);
+    store();
+    __syncthreads();
+    for (int k0 = 0; k0 < g.K; k0 += kKS) {
+        const bool more = k0 + kKS < g.K;
+        if (more) load(k0 + kKS);
+#pragma unroll
+        for (int ks = 0; ks < kKS / 4; ++ks) {
             const int kk = ks * 4 + lk;
             const double a0 = As[kk][wm + lr], a1 = As[kk][wm + 16 + lr];
             const double b0 = Bs[kk][wn + lr], b1 = Bs[kk][wn + 16 + lr];
@@ -74,6 +102,10 @@ __global__ __launch_bounds__(kT) void k_gemm(const GemmDesc *__restrict__ descs)
             acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
         }
         __syncthreads();
+        if (more) {
+            store();
+            __syncthreads();
+        }
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
