@@ -511,7 +511,7 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
         arena = std::max(arena, (size_t)(x.off + (int64_t)x.ld * x.m));
     }
     const size_t abytes = sizeof(double) * arena;
-    PTB_TRY(sytrd_batch(ctx, desc, ddesc, A, W, dvec, evec, tau));
+    PTB_TRY(sytrd_batch(ctx, desc, ddesc, A, dvec, evec, tau));
     PTB_TRY(zero_upper_batch(ctx, ddesc, nsm, max_m, A));              // A = V
     PTB_TRY(larft_all(ctx, desc, ddesc, A, tau, qf));
     // divide and conquer

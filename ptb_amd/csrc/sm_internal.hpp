@@ -84,7 +84,7 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
 
 // tridiag.hip
 int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
-                double *A, double *W, double *d, double *e, double *tau);
+                double *A, double *d, double *e, double *tau);
 
 // sm.hip
 int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,

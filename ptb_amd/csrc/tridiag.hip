@@ -14,7 +14,7 @@
 //
 // Output: d (diagonal), e (subdiagonal, e[k] couples k and k+1), tau, and v_k
 // stored in column k rows k+1.. of A (unit at row k+1).  W is a work panel
-// (m x 32, leading dimension ld) in a separate arena.
+// (m x 32, leading dimension ld) in a separate panel buffer.
 #include "sm_internal.hpp"
 
 namespace {
@@ -97,12 +97,16 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
                                                     double *__restrict__ Wa,
                                                     double *__restrict__ d_all,
                                                     double *__restrict__ e_all,
-                                                    double *__restrict__ tau_all, int panel) {
+                                                    double *__restrict__ tau_all,
+                                                    const int64_t *__restrict__ pwoff, int panel) {
     extern __shared__ double sm[];
     const SubDesc ds = desc[blockIdx.x];
     const int m = ds.m, ld = ds.ld;
     double *a = Aa + ds.off;
-    double *W = Wa + ds.off;  // columns 0..31 used
+    // panel buffer [W | V | W], ld x 96: W columns 0..31, a copy of the panel's
+    // reflectors in 32..63 and of W in 64..95, so that the trailing update
+    // A -= W V^T + V W^T is one product [W V] [V W]^T
+    double *W = Wa + pwoff[blockIdx.x];
     double *dd = d_all + ds.mapoff, *ee = e_all + ds.mapoff, *tt = tau_all + ds.mapoff;
     double *vs = sm;              // m   current Householder vector (rows > k)
     double *cp = vs + m;          // kWaves x m  per-wavefront partials of A22 v
@@ -119,6 +123,15 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
     if (k0 >= m - 2) return;
     {
         const int kb = min(kNB, m - 2 - k0);
+        if (kb < kNB) {  // the product reads all 64 columns: zero the unused ones
+            for (int l = threadIdx.x; l < (kNB - kb) * m; l += kT) {
+                const int jj = kb + l / m, i = l % m;
+                W[(int64_t)jj * ld + i] = 0.0;
+                W[(int64_t)(kNB + jj) * ld + i] = 0.0;
+                W[(int64_t)(2 * kNB + jj) * ld + i] = 0.0;
+            }
+            __syncthreads();
+        }
         for (int jj = 0; jj < kb; ++jj) {
             const int k = k0 + jj;
             const int n = m - k - 1;  // trailing size
@@ -161,6 +174,7 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
                 const double v = (i == k + 1) ? 1.0 : vs[i] * scale;
                 vs[i] = v;
                 a[(int64_t)k * ld + i] = v;
+                W[(int64_t)(kNB + jj) * ld + i] = v;
             }
             if (threadIdx.x == 0) {
                 ee[k] = beta;
@@ -265,8 +279,11 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
             }
             const double ptv = block_reduce(pv, red);
             const double acoef = -0.5 * tk * ptv;
-            for (int i = k + 1 + threadIdx.x; i < m; i += kT)
-                W[(int64_t)jj * ld + i] += acoef * vs[i];
+            for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
+                const double w = W[(int64_t)jj * ld + i] + acoef * vs[i];
+                W[(int64_t)jj * ld + i] = w;
+                W[(int64_t)(2 * kNB + jj) * ld + i] = w;
+            }
             for (int i = threadIdx.x; i <= k; i += kT) W[(int64_t)jj * ld + i] = 0.0;
             __syncthreads();
         }
@@ -302,9 +319,10 @@ __global__ void k_sytrd_finish(const SubDesc *__restrict__ desc, const double *_
 namespace ptb {
 
 // Panel by panel: the panel kernel reduces 32 columns of every submatrix, then
-// two batched GEMMs apply the rank-64 update A(s:,s:) -= V W^T + W V^T.
+// one batched GEMM applies the rank-64 update A(s:,s:) -= [W V] [V W]^T to the
+// lower triangle.
 int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
-                double *A, double *W, double *d, double *e, double *tau) {
+                double *A, double *d, double *e, double *tau) {
     ProfScope prof_(ctx, "sytrd");
     const int nsm = (int)desc.size();
     int max_m = 0;
@@ -312,7 +330,18 @@ int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
     const size_t shmem = sizeof(double) * ((1 + kWaves) * (size_t)max_m + kWaves + 4 * kNB + kWaves * 16);
     if (shmem > 160 * 1024) return fail(ctx, PTBGPU_E_ARG, "submatrix too large for sytrd (%d)", max_m);
     const int npanel = max_m > 2 ? (max_m - 2 + kNB - 1) / kNB : 0;
-    std::vector<GemmDesc> g1, g2;
+    std::vector<int64_t> pwoff(nsm);
+    int64_t pwtot = 0;
+    for (int si = 0; si < nsm; ++si) {
+        pwoff[si] = pwtot;
+        pwtot += (int64_t)desc[si].ld * 3 * kNB;
+    }
+    auto *PW = static_cast<double *>(ws(ctx, "sytrd_pw", sizeof(double) * std::max<int64_t>(pwtot, 1)));
+    auto *dpwoff = static_cast<int64_t *>(ws(ctx, "sytrd_pwoff", sizeof(int64_t) * nsm));
+    if (!PW || !dpwoff) return PTBGPU_E_NOMEM;
+    PTB_HIP(ctx, hipMemcpyAsync(dpwoff, pwoff.data(), sizeof(int64_t) * nsm, hipMemcpyHostToDevice,
+                                ctx->stream));
+    std::vector<GemmDesc> g1;
     std::vector<int> start(npanel + 1, 0);
     for (int p = 0; p < npanel; ++p) {
         start[p] = (int)g1.size();
@@ -323,36 +352,29 @@ int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
             const int kb = std::min(kNB, x.m - 2 - k0);
             const int s0 = k0 + kb, n = x.m - s0;
             if (n <= 0) continue;
-            double *a = A + x.off;
-            double *V = a + (int64_t)k0 * x.ld + s0, *Wp = W + x.off + s0;
-            double *C = a + (int64_t)s0 * x.ld + s0;
-            g1.push_back({V, Wp, C, n, n, kb, x.ld, x.ld, x.ld, -1.0, 1.0, 1, 0});
-            g2.push_back({Wp, V, C, n, n, kb, x.ld, x.ld, x.ld, -1.0, 1.0, 1, 0});
+            double *pw = PW + pwoff[si] + s0;
+            double *C = A + x.off + (int64_t)s0 * x.ld + s0;
+            // C -= [W V] [V W]^T (lower triangle)
+            g1.push_back({pw, pw + (int64_t)kNB * x.ld, C, n, n, 2 * kNB, x.ld, x.ld, x.ld, -1.0, 1.0,
+                          1, 0});
         }
     }
     start[npanel] = (int)g1.size();
     const size_t gb = sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1);
     auto *dg1 = static_cast<GemmDesc *>(ws(ctx, "sytrd_g1", gb));
-    auto *dg2 = static_cast<GemmDesc *>(ws(ctx, "sytrd_g2", gb));
-    if (!dg1 || !dg2) return PTBGPU_E_NOMEM;
-    if (!g1.empty()) {
+    if (!dg1) return PTBGPU_E_NOMEM;
+    if (!g1.empty())
         PTB_HIP(ctx, hipMemcpyAsync(dg1, g1.data(), sizeof(GemmDesc) * g1.size(),
                                     hipMemcpyHostToDevice, ctx->stream));
-        PTB_HIP(ctx, hipMemcpyAsync(dg2, g2.data(), sizeof(GemmDesc) * g2.size(),
-                                    hipMemcpyHostToDevice, ctx->stream));
-    }
     for (int p = 0; p < npanel; ++p) {
         {
             ProfScope kprof_(ctx, "sytrd_panel");
-            hipLaunchKernelGGL(k_sytrd_panel, dim3(nsm), dim3(kT), shmem, ctx->stream, ddesc, A, W,
-                               d, e, tau, p);
+            hipLaunchKernelGGL(k_sytrd_panel, dim3(nsm), dim3(kT), shmem, ctx->stream, ddesc, A, PW,
+                               d, e, tau, dpwoff, p);
             PTB_CHECK_LAUNCH(ctx);
         }
         const int c = start[p + 1] - start[p];
-        if (c > 0) {  // two launches: both products update the same trailing block
-            PTB_TRY(gemm_vbatch(ctx, false, true, dg1 + start[p], c, max_m, max_m, "sytrd"));
-            PTB_TRY(gemm_vbatch(ctx, false, true, dg2 + start[p], c, max_m, max_m, "sytrd"));
-        }
+        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dg1 + start[p], c, max_m, max_m, "sytrd"));
     }
     hipLaunchKernelGGL(k_sytrd_finish, dim3((nsm + 255) / 256), dim3(256), 0, ctx->stream, ddesc, A,
                        d, e, tau, nsm);
