@@ -10,8 +10,9 @@
 // lane r owning row r of the leaf's eigenvector block in LDS.
 // Merge of (Q1,D1),(Q2,D2): z = [Q1(last,:); sgn Q2(first,:)] / sqrt 2,
 // rho = 2|beta|; sort D; deflate (small rho|z_i| or close pairs rotated together);
-// solve the secular equation 1 + rho sum w_i^2/(d_i - lam) = 0 by bisection on
-// lam - d_origin (origin = nearer pole) so that d_i - lam is formed accurately;
+// solve the secular equation 1 + rho sum w_i^2/(d_i - lam) = 0 for
+// lam - d_origin (origin = nearer pole, so that d_i - lam is formed accurately)
+// with rational two-pole model steps inside a shrinking bracket;
 // Loewner vector zhat; U(i,j) = zhat_i/(d_i - lam_j) normalised; the merged
 // eigenvectors are Qnd U (a GEMM launched for all nodes of a tree level; this
 // kernel writes its descriptor).  Eigenvalues are kept in the same column order
@@ -164,7 +165,8 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     double *w = dl + npow;          // K weights
     double *tau = w + npow;         // K root offsets
     double *zh = tau + npow;        // K Loewner vector
-    int *ix = reinterpret_cast<int *>(zh + npow);  // npow sort permutation
+    double *cinv = zh + npow;       // K inverse column norms of U
+    int *ix = reinterpret_cast<int *>(cinv + npow);  // npow sort permutation
     int *slot = ix + npow;          // n output slot of sorted position
     int *org = slot + npow;         // K origin pole of each root
     __shared__ double red[kMT / PTB_WAVE];
@@ -310,28 +312,64 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
             lo = 0.0;
             hi = rho * s2;
         }
-        // safeguarded Newton on f(t) = 1 + rho sum w_i^2 / (delta_i - t), f increasing:
-        // a Newton point inside the bracket is taken, otherwise the midpoint;
-        // every evaluation also shrinks the bracket, so the iteration ends with
-        // the bracket at adjacent doubles or f == 0
+        // f(t) = 1 + rho sum w_i^2 / (delta_i - t) (delta_i = d_i - d_origin) is
+        // increasing between its poles.  Each step fits the two poles that
+        // bracket the root exactly, a + b1/(D1 - eta) + b2/(D2 - eta) matching
+        // f and the derivatives of the parts left (psi) and right (phi) of the
+        // root ("middle way" rational model, as LAPACK dlaed4), and takes the
+        // model's root when it lies inside the bracket, else a Newton step or
+        // the midpoint.  Every evaluation also shrinks the bracket; stop when
+        // |f| is at rounding level or the bracket reaches adjacent doubles.
         const double dorg = dl[o];
+        const bool last = (j == K - 1);
+        const double da = dl[j] - dorg, db = last ? 0.0 : dl[j + 1] - dorg;
         double t = 0.5 * (lo + hi);
-        for (int it = 0; it < 400; ++it) {
-            double f = 1.0, fp = 0.0;
-            for (int i = 0; i < K; ++i) {
+        for (int it = 0; it < 200; ++it) {
+            double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
+            for (int i = 0; i <= j; ++i) {
                 const double r = 1.0 / ((dl[i] - dorg) - t);
                 const double q = rho * w[i] * w[i] * r;
-                f += q;
-                fp += q * r;
+                psi += q;
+                dpsi += q * r;
             }
-            if (f == 0.0) {
+            for (int i = j + 1; i < K; ++i) {
+                const double r = 1.0 / ((dl[i] - dorg) - t);
+                const double q = rho * w[i] * w[i] * r;
+                phi += q;
+                dphi += q * r;
+            }
+            const double f = 1.0 + psi + phi;
+            if (f == 0.0 || fabs(f) <= 8.0 * kEps * (double)K * (1.0 + fabs(psi) + fabs(phi))) {
                 lo = hi = t;
                 break;
             }
             if (f < 0.0) lo = t;
             else hi = t;
-            double tn = t - f / fp;
-            if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+            const double d1 = da - t;  // distance to the left pole
+            double tn;
+            if (last) {
+                // one-pole model a + b1/(d1 - eta): eta = d1 + b1/a
+                const double b1 = d1 * d1 * dpsi;
+                const double a = f - b1 / d1;
+                tn = (a != 0.0) ? t + d1 + b1 / a : 0.5 * (lo + hi);
+            } else {
+                const double d2 = db - t;  // distance to the right pole
+                const double b1 = d1 * d1 * dpsi, b2 = d2 * d2 * dphi;
+                const double a = f - b1 / d1 - b2 / d2;
+                // a eta^2 - B eta + C = 0
+                const double B = a * (d1 + d2) + b1 + b2;
+                const double C = a * d1 * d2 + b1 * d2 + b2 * d1;
+                // the root between the poles is the small one, 2C / (B + sgn(B) sqrt),
+                // formed without cancellation; the other one is tried if it is not
+                const double sq = sqrt(fmax(B * B - 4.0 * a * C, 0.0));
+                const double den = B >= 0.0 ? B + sq : B - sq;
+                tn = (den != 0.0) ? t + 2.0 * C / den : t;
+                if (!(tn > lo && tn < hi) && a != 0.0) tn = t + den / (2.0 * a);
+            }
+            if (!(tn > lo && tn < hi)) {
+                tn = t - f / (dpsi + dphi);
+                if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+            }
             if (tn == t || tn == lo || tn == hi) break;
             t = tn;
         }
@@ -351,7 +389,9 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         zh[i] = copysign(sqrt(fmax(prod, 0.0) / rho), w[i]);
     }
     __syncthreads();
-    // U(:, j) = zhat ./ (d - lam_j), normalised; eigenvalues lam_j
+    // U(:, j) = zhat ./ (d - lam_j), normalised; eigenvalues lam_j.  Column
+    // norms first (one thread per column), then U written with consecutive
+    // threads on consecutive rows.
     for (int j = threadIdx.x; j < K; j += kMT) {
         const double dorg = dl[org[j]], tj = tau[j];
         double nrm = 0.0;
@@ -359,10 +399,13 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
             const double u = zh[i] / ((dl[i] - dorg) - tj);
             nrm += u * u;
         }
-        const double inv = 1.0 / sqrt(nrm);
-        double *Uc = X + (int64_t)(nd.lo + j) * ld + nd.lo;
-        for (int i = 0; i < K; ++i) Uc[i] = inv * zh[i] / ((dl[i] - dorg) - tj);
+        cinv[j] = 1.0 / sqrt(nrm);
         D[nd.lo + j] = dorg + tj;
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < K * K; l += kMT) {
+        const int i = l % K, j = l / K;
+        X[(int64_t)(nd.lo + j) * ld + nd.lo + i] = cinv[j] * zh[i] / ((dl[i] - dl[org[j]]) - tau[j]);
     }
     if (threadIdx.x == 0) {
         g->A = W + (int64_t)nd.lo * ld + nd.lo;
@@ -400,7 +443,7 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
     if (count <= 0) return PTBGPU_OK;
     int npow = 1;
     while (npow < max_n) npow <<= 1;
-    const size_t shmem = (size_t)npow * (6 * sizeof(double) + 3 * sizeof(int));
+    const size_t shmem = (size_t)npow * (7 * sizeof(double) + 3 * sizeof(int));
     if (shmem > 150 * 1024)
         return fail(ctx, PTBGPU_E_ARG, "merge of %d rows exceeds the LDS budget", max_n);
     {
