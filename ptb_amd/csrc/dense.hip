@@ -31,12 +31,23 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 // slices run from LDS.
 constexpr int kKS = 32;
 
+// Workgroups are dealt to the 8 XCDs round-robin by linear id, so the 1-D grid
+// is read as (xcd, slot): XCD x runs the products x, x+8, x+16, ... tile after
+// tile.  Every product's tiles share one XCD's L2, and products of any shape
+// spread evenly over the XCDs (a (tile, product) grid would put tile t of every
+// product on XCD t % 8, idling most of the chip for narrow products).
 template <bool TA, bool TB>
-__global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ descs) {
-    const GemmDesc g = descs[blockIdx.y];
+__global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ descs, int count,
+                                               int tiles) {
+    const int64_t lin = blockIdx.x;
+    const int64_t slot = lin >> 3;
+    const int gi = (int)((slot / tiles) * 8 + (lin & 7));
+    const int tile = (int)(slot % tiles);
+    if (gi >= count) return;
+    const GemmDesc g = descs[gi];
     const int tilesM = (g.M + 63) / 64, tilesN = (g.N + 63) / 64;
-    if (g.M <= 0 || g.N <= 0 || (int)blockIdx.x >= tilesM * tilesN) return;
-    const int m0 = (blockIdx.x % tilesM) * 64, n0 = (blockIdx.x / tilesM) * 64;
+    if (g.M <= 0 || g.N <= 0 || tile >= tilesM * tilesN) return;
+    const int m0 = (tile % tilesM) * 64, n0 = (tile / tilesM) * 64;
     if (g.uplo && n0 > m0 + 63) return;  // tile strictly above the diagonal
     __shared__ double As[kKS][68];
     __shared__ double Bs[kKS][68];
@@ -225,8 +236,8 @@ __global__ __launch_bounds__(kT) void k_transpose(const SubDesc *__restrict__ de
 
 __global__ __launch_bounds__(kT) void k_zero_upper(const SubDesc *__restrict__ desc,
                                                    double *__restrict__ Aa) {
-    const SubDesc d = desc[blockIdx.y];
-    const int c = blockIdx.x;
+    const SubDesc d = desc[blockIdx.x];
+    const int c = blockIdx.y;
     if (c >= d.m) return;
     double *col = Aa + d.off + (int64_t)c * d.ld;
     for (int r = threadIdx.x; r <= c; r += kT) col[r] = 0.0;
@@ -245,16 +256,16 @@ __global__ __launch_bounds__(kT) void k_larft_all(const SubDesc *__restrict__ de
                                                   const double *__restrict__ tau_all,
                                                   const int64_t *__restrict__ toff,
                                                   double *__restrict__ Ta) {
-    const SubDesc d = desc[blockIdx.y];
+    const SubDesc d = desc[blockIdx.x];
     const int nr = d.m - 2;
     const int nblk = nr > 0 ? (nr + 31) / 32 : 0;
-    const int b = blockIdx.x;
+    const int b = blockIdx.y;
     if (b >= nblk) return;
     const int b0 = b * 32;
     const int kb = min(32, nr - b0);
     const double *V = Va + d.off + (int64_t)b0 * d.ld;
     const double *tau = tau_all + d.mapoff + b0;
-    double *T = Ta + toff[blockIdx.y] + (int64_t)b * 1024;
+    double *T = Ta + toff[blockIdx.x] + (int64_t)b * 1024;
     __shared__ double Vs[64][33];
     __shared__ double G[32][33];
     __shared__ double Ts[32][33];
@@ -321,8 +332,8 @@ __global__ __launch_bounds__(64) void k_diag_inv(const SubDesc *__restrict__ des
                                                  const int64_t *__restrict__ doff,
                                                  const double *__restrict__ La,
                                                  double *__restrict__ dinv) {
-    const SubDesc d = desc[blockIdx.y];
-    const int b = blockIdx.x;
+    const SubDesc d = desc[blockIdx.x];
+    const int b = blockIdx.y;
     const int i0 = b * 64;
     if (i0 >= d.m) return;
     __shared__ double T[64][65];
@@ -347,7 +358,7 @@ __global__ __launch_bounds__(64) void k_diag_inv(const SubDesc *__restrict__ des
         for (int t = 0; t < r; ++t) sacc -= T[r][t] * x[t];
         x[r] = sacc / T[r][r];
     }
-    double *out = dinv + doff[blockIdx.y] + (int64_t)b * 4096 + (int64_t)c * 64;
+    double *out = dinv + doff[blockIdx.x] + (int64_t)b * 4096 + (int64_t)c * 64;
 #pragma unroll
     for (int r = 0; r < 64; ++r) out[r] = x[r];
 }
@@ -362,16 +373,16 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
     ProfScope sprof_(ctx, site ? (std::string("gemm_") + site).c_str() : "gemm_other");
     if (count <= 0 || max_m <= 0 || max_n <= 0) return PTBGPU_OK;
     const int tiles = ((max_m + 63) / 64) * ((max_n + 63) / 64);
-    for (int c0 = 0; c0 < count; c0 += 65535) {
-        const int cnt = std::min(65535, count - c0);
-        dim3 grid(tiles, cnt);
-        const GemmDesc *dd = ddesc + c0;
-        if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(kT), 0, ctx->stream, dd);
-        else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(kT), 0, ctx->stream, dd);
-        else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(kT), 0, ctx->stream, dd);
-        else hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(kT), 0, ctx->stream, dd);
-        PTB_CHECK_LAUNCH(ctx);
-    }
+    const int64_t groups8 = (count + 7) / 8;
+    const int64_t total = groups8 * 8 * tiles;
+    if (total > 0x7fffffff) return fail(ctx, PTBGPU_E_ARG, "GEMM batch too large (%lld workgroups)",
+                                        (long long)total);
+    dim3 grid((unsigned)total);
+    if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
+    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
+    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
+    else hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
+    PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
 
@@ -401,7 +412,7 @@ int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
                                 ctx->stream));
     {
         ProfScope kprof_(ctx, "diag_inv");
-        hipLaunchKernelGGL(k_diag_inv, dim3(max_nb, nsm), dim3(64), 0, ctx->stream, ddesc, ddoff, L,
+        hipLaunchKernelGGL(k_diag_inv, dim3(nsm, max_nb), dim3(64), 0, ctx->stream, ddesc, ddoff, L,
                            dinv);
         PTB_CHECK_LAUNCH(ctx);
     }
@@ -503,7 +514,7 @@ int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_
 
 int zero_upper_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, double *A) {
     ProfScope prof_(ctx, "zero_upper");
-    hipLaunchKernelGGL(k_zero_upper, dim3(max_m, nsm), dim3(kT), 0, ctx->stream, ddesc, A);
+    hipLaunchKernelGGL(k_zero_upper, dim3(nsm, max_m), dim3(kT), 0, ctx->stream, ddesc, A);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
@@ -530,7 +541,7 @@ int larft_all(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
     PTB_HIP(ctx, hipMemcpyAsync(dtoff, qf.toff.data(), sizeof(int64_t) * nsm, hipMemcpyHostToDevice,
                                 ctx->stream));
     if (maxblk > 0) {
-        hipLaunchKernelGGL(k_larft_all, dim3(maxblk, nsm), dim3(kT), 0, ctx->stream, ddesc, V, tau,
+        hipLaunchKernelGGL(k_larft_all, dim3(nsm, maxblk), dim3(kT), 0, ctx->stream, ddesc, V, tau,
                            dtoff, qf.T);
         PTB_CHECK_LAUNCH(ctx);
     }

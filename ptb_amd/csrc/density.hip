@@ -39,13 +39,13 @@ __global__ __launch_bounds__(kT) void k_xall_build(const SubDesc *__restrict__ d
                                                    const int64_t *__restrict__ xboff,
                                                    double *__restrict__ XA,
                                                    double *__restrict__ XB) {
-    const SubDesc d = desc[blockIdx.y];
-    const int g = d.gbeg + blockIdx.x;
+    const SubDesc d = desc[blockIdx.x];
+    const int g = d.gbeg + blockIdx.y;
     if (g >= d.gend) return;
     const int row = oloc[g];
     const double *L = La + d.off;
-    double *xa = XA + xoff[blockIdx.y] + (int64_t)blockIdx.x * d.ld;
-    double *xb = XB + xboff[blockIdx.y] + (int64_t)blockIdx.x * d.ld;
+    double *xa = XA + xoff[blockIdx.x] + (int64_t)blockIdx.y * d.ld;
+    double *xb = XB + xboff[blockIdx.x] + (int64_t)blockIdx.y * d.ld;
     for (int t = threadIdx.x; t < d.m; t += kT) {
         xa[t] = t <= row ? L[(int64_t)t * d.ld + row] : 0.0;
         xb[t] = t == row ? 1.0 : 0.0;
@@ -60,11 +60,11 @@ __global__ __launch_bounds__(kT) void k_fe_proj(const SubDesc *__restrict__ desc
                                                 const int64_t *__restrict__ xoff,
                                                 const double *__restrict__ F,
                                                 double *__restrict__ fe_all) {
-    const SubDesc d = desc[blockIdx.y];
-    const int k = blockIdx.x * kT + threadIdx.x;
+    const SubDesc d = desc[blockIdx.x];
+    const int k = blockIdx.y * kT + threadIdx.x;
     if (k >= d.m) return;
     const int c = d.gend - d.gbeg;
-    const double *f = F + xoff[blockIdx.y] + k;
+    const double *f = F + xoff[blockIdx.x] + k;
     double s = 0.0;
     if (!(d.flags & 1))
         for (int g = 0; g < c; ++g) s += f[(int64_t)g * d.ld] * f[(int64_t)(c + g) * d.ld];
@@ -227,7 +227,7 @@ int xall_build(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_own, cons
                double *XB) {
     ProfScope prof_(ctx, "fe_rows");
     if (max_own <= 0) return PTBGPU_OK;
-    hipLaunchKernelGGL(k_xall_build, dim3(max_own, nsm), dim3(kT), 0, ctx->stream, ddesc, doloc, L,
+    hipLaunchKernelGGL(k_xall_build, dim3(nsm, max_own), dim3(kT), 0, ctx->stream, ddesc, doloc, L,
                        dxoff, dxboff, XA, XB);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
@@ -236,7 +236,7 @@ int xall_build(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_own, cons
 int fe_proj(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const int64_t *dxoff,
             const double *F, double *fe) {
     ProfScope prof_(ctx, "fe_finish");
-    hipLaunchKernelGGL(k_fe_proj, dim3((max_m + kT - 1) / kT, nsm), dim3(kT), 0, ctx->stream,
+    hipLaunchKernelGGL(k_fe_proj, dim3(nsm, (max_m + kT - 1) / kT), dim3(kT), 0, ctx->stream,
                        ddesc, dxoff, F, fe);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;

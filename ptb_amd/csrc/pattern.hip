@@ -43,11 +43,11 @@ __global__ __launch_bounds__(kThreads) void k_group_union(const uint32_t *__rest
                                                           uint32_t *__restrict__ uni,
                                                           int *__restrict__ mcount) {
     __shared__ int red[16];
-    const int s = blockIdx.y;
+    const int s = blockIdx.x;
     const int g0 = gptr[s], g1 = gptr[s + 1];
     int pc = 0;
-    for (int64_t w = (int64_t)blockIdx.x * kThreads + threadIdx.x; w < words;
-         w += (int64_t)gridDim.x * kThreads) {
+    for (int64_t w = (int64_t)blockIdx.y * kThreads + threadIdx.x; w < words;
+         w += (int64_t)gridDim.y * kThreads) {
         uint32_t acc = 0;
         for (int g = g0; g < g1; ++g) acc |= bits[(int64_t)gcols[g] * words + w];
         uni[(int64_t)s * words + w] = acc;
@@ -191,7 +191,7 @@ int group_union(ptbgpu_ctx *ctx, const uint32_t *bits, int64_t words, const int 
     ProfScope prof_(ctx, "group_union");
     PTB_HIP(ctx, hipMemsetAsync(dmcount, 0, sizeof(int) * nsm, ctx->stream));
     int bx = (int)std::min<int64_t>((words + kThreads - 1) / kThreads, 64);
-    hipLaunchKernelGGL(k_group_union, dim3(std::max(bx, 1), nsm), dim3(kThreads), 0, ctx->stream,
+    hipLaunchKernelGGL(k_group_union, dim3(nsm, std::max(bx, 1)), dim3(kThreads), 0, ctx->stream,
                        bits, words, dgptr, dgcols, uni, dmcount);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
