@@ -1,7 +1,7 @@
 // Householder reduction of each K = L^-1 H L^-T to tridiagonal form, K = Q T Q^T,
 // Q = H_0 ... H_{m-3},  H_k = I - tau_k v_k v_k^T  (LAPACK dsytrd, lower).
 //
-// One 512-thread workgroup per submatrix, blocked by 32 columns (dlatrd scheme):
+// One 256-thread workgroup per submatrix, blocked by 32 columns (dlatrd scheme):
 // inside a panel the trailing matrix stays as it was at the panel start and the
 // pending rank-2 terms are applied on the fly as V (W^T v) + W (V^T v); at the
 // end of the panel one rank-64 update refreshes the lower triangle of the
@@ -19,7 +19,7 @@
 
 namespace {
 
-constexpr int kT = 512;
+constexpr int kT = 256;
 constexpr int kNB = 32;
 constexpr int kWaves = kT / PTB_WAVE;
 
