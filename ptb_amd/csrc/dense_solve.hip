@@ -63,7 +63,7 @@ std::vector<SubDesc> one_desc(int64_t n) {
     SubDesc d{};
     d.off = 0;
     d.m = (int)n;
-    d.ld = (int)((n + 3) & ~3);
+    d.ld = (int)((n + 15) & ~15);
     d.mapoff = 0;
     d.gbeg = 0;
     d.gend = (int)n;

@@ -316,7 +316,7 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     for (int s = 0; s < nsm; ++s) {
         SubDesc &d = desc[s];
         d.m = mcount[s];
-        d.ld = (d.m + 3) & ~3;
+        d.ld = (d.m + 15) & ~15;
         d.off = arena;
         d.mapoff = (int)rows;
         d.gbeg = gptr[s];

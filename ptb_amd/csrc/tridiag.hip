@@ -224,37 +224,42 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
                 for (int w = 0; w < kWaves; ++w) cp[w * m + i] = 0.0;
             __syncthreads();
             {
-                // lower-triangle tiles of 64 rows x 16 columns, numbered row
-                // block by row block: block bi holds 4 (bi + 1) tiles (the
-                // last one ceil(n / 16)), so tile t lies in the block bi with
+                // lower-triangle tiles of 64 rows x 16 columns over rows and
+                // columns base .. m-1, base = (k+1) rounded down to 16 so that
+                // every 64-row column segment starts on a 128-byte line (rows
+                // and columns below k+1 are masked); numbered row block by row
+                // block: block bi holds 4 (bi + 1) tiles (the last one
+                // ceil(nb / 16)), so tile t lies in the block bi with
                 // 2 bi (bi + 1) <= t < 2 (bi + 1) (bi + 2).  Wavefront wid
                 // takes the tiles wid, wid + kWaves, ...
-                const int nrb = (n + 63) / 64;
-                const int ntile = 2 * (nrb - 1) * nrb + (n + 15) / 16;
+                const int base = (k + 1) & ~15;
+                const int nb = m - base;
+                const int nrb = (nb + 63) / 64;
+                const int ntile = 2 * (nrb - 1) * nrb + (nb + 15) / 16;
                 double *cpw = cp + wid * m;
                 for (int t = wid; t < ntile; t += kWaves) {
                     int bi = (int)((sqrtf(2.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
                     while (2 * (bi + 1) * (bi + 2) <= t) ++bi;
                     while (bi > 0 && 2 * bi * (bi + 1) > t) --bi;
                     const int cj = t - 2 * bi * (bi + 1);
-                    const int i = k + 1 + bi * 64 + lane;
-                    const int c0 = k + 1 + cj * 16;
-                    const bool ok = i < m;
+                    const int i = base + bi * 64 + lane;
+                    const int c0 = base + cj * 16;
+                    const bool ok = i < m && i > k;
                     double x[16];
 #pragma unroll
                     for (int q = 0; q < 16; ++q) {
                         const int c = c0 + q;
-                        x[q] = (ok && c <= i) ? a[(int64_t)c * ld + i] : 0.0;
+                        x[q] = (ok && c <= i && c > k) ? a[(int64_t)c * ld + i] : 0.0;
                     }
                     const double vi = ok ? vs[i] : 0.0;
                     double yrow = 0.0;
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) yrow += x[q] * vs[min(c0 + q, m - 1)];
+                    for (int q = 0; q < 16; ++q) yrow += x[q] * vs[max(k + 1, min(c0 + q, m - 1))];
 #pragma unroll
                     for (int q = 0; q < 16; ++q) x[q] = (c0 + q < i) ? x[q] * vi : 0.0;
                     const double colsum = butterfly16(x, lane);
                     const int cq = c0 + ((lane >> 2) & 15);
-                    if ((lane & 3) == 0 && cq < m) cpw[cq] += colsum;
+                    if ((lane & 3) == 0 && cq < m && cq > k) cpw[cq] += colsum;
                     if (ok) cpw[i] += yrow;
                 }
             }
