@@ -75,6 +75,21 @@ def roofline_model(name, ms, ndim):
     return None
 
 
+def traffic_per_launch(kernel, workload):
+    """Scaled HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    --pmc summary of this workload (FETCH_SIZE + WRITE_SIZE, each scaled by the
+    counter's reading for a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_hbm_traffic.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("bytes_per_launch="):
+                    return float(line.split("=", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(mol, H, S, cols, budget_s=20.0):
     """The oracle (numpy/scipy restatement of source/sm.f90) timed on a bounded
     sample of the same workload: every submatrix's gather + dsygvd + fe +
@@ -228,8 +243,12 @@ def main():
         else:
             achieved = units / (ms * 1e-3) / 1e12
             peak = FP64_PEAK_TFS
-        roof = {"kernel": name, "bound": bound, "achieved": round(achieved, 2), "peak": peak,
-                "unit": unit, "frac": round(achieved / peak, 4), "traffic": None,
+        kname = "k_" + name
+        traffic = traffic_per_launch(kname, args.config)
+        roof = {"kernel": kname, "bound": bound, "achieved": round(achieved, 2), "peak": peak,
+                "unit": unit, "frac": round(achieved / peak, 4),
+                "traffic": None if traffic is None else round(traffic, 0),
+                "algorithmic_bytes_per_launch": round(units / max(cnt, 1), 0),
                 "ms_per_launch": round(per_launch_ms, 3), "share_of_build": round(ms / r["ms_per_step"], 3)}
         break
     out = {

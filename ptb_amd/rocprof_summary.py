@@ -1,0 +1,108 @@
+"""Short text summaries of rocprofv3 CSV output for this library's kernels.
+
+    python -m ptb_amd.rocprof_summary stats DIR [--top N]
+        kernel table from DIR/*_kernel_stats.csv (rocprofv3 --kernel-trace --stats)
+    python -m ptb_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR --kernel NAME --copy-mib MIB
+        HBM bytes per dispatch of kernel NAME from two rocprofv3 --pmc runs
+        (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch), each scaled by what the
+        same counter reads for the plain MIB-MiB device copy in the same run.
+
+Prints markdown on stdout; the traffic mode also prints one line
+`bytes_per_launch=<float>` that bench.py reads back from the committed summary.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import os
+import re
+import sys
+
+
+def short_name(name: str) -> str:
+    """Kernel name without namespace, template arguments and parameter list."""
+    s = re.sub(r"\(anonymous namespace\)::", "", name)
+    s = re.sub(r"^void ", "", s)
+    head = s.split("(")[0]
+    return head if len(head) < 60 else head[:57] + "..."
+
+
+def _one(pattern: str) -> str:
+    files = sorted(glob.glob(pattern))
+    if not files:
+        sys.exit(f"no file matches {pattern}")
+    return files[0]
+
+
+def stats(d: str, top: int) -> None:
+    rows = list(csv.DictReader(open(_one(os.path.join(d, "*kernel_stats.csv")))))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    print("| kernel | calls | total ms | avg us | min us | max us | % |")
+    print("|---|---|---|---|---|---|---|")
+    for r in rows[:top]:
+        print(f"| `{short_name(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} "
+              f"| {float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} "
+              f"| {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
+    print(f"\nall kernels: {total / 1e6:.2f} ms in {sum(int(r['Calls']) for r in rows)} dispatches")
+
+
+def _counter(d: str, counter: str):
+    """{dispatch id: (kernel name, value in KiB)} for one counter."""
+    out = {}
+    for r in csv.DictReader(open(_one(os.path.join(d, "*counter_collection.csv")))):
+        if r["Counter_Name"] != counter:
+            continue
+        did = int(r["Dispatch_Id"])
+        name, val = r["Kernel_Name"], float(r["Counter_Value"])
+        if did in out:  # one row per agent/instance: sum them
+            val += out[did][1]
+        out[did] = (name, val)
+    return out
+
+
+def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_mib: int) -> None:
+    expect_kib = copy_mib * 1024.0
+    scales = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = _counter(d, counter)
+        copies = [v for n, v in rows.values() if "copyBuffer" in n and v > 0.25 * expect_kib]
+        if not copies:
+            sys.exit(f"{counter}: no calibration copy of {copy_mib} MiB in {d}")
+        calib = max(copies)
+        mine = [v for n, v in rows.values() if short_name(n) == kernel]
+        if not mine:
+            sys.exit(f"{counter}: kernel {kernel} not found in {d}")
+        scales[counter] = (expect_kib / calib, calib, sum(mine) / len(mine), len(mine))
+    print(f"| counter | copy reads (KiB) | scale | {kernel} KiB/dispatch (raw) | scaled GB/dispatch | dispatches |")
+    print("|---|---|---|---|---|---|")
+    total = 0.0
+    for counter, (scale, calib, avg, cnt) in scales.items():
+        b = avg * scale * 1024.0
+        total += b
+        print(f"| {counter} | {calib:.0f} (expect {expect_kib:.0f}) | {scale:.3f} | {avg:.0f} "
+              f"| {b / 1e9:.4f} | {cnt} |")
+    print(f"\nHBM traffic per dispatch of {kernel} (fetch + write, scaled): {total / 1e9:.4f} GB\n")
+    print(f"bytes_per_launch={total:.6e}")
+
+
+def main(argv=None) -> None:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="mode", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    s.add_argument("--top", type=int, default=25)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("--kernel", required=True)
+    t.add_argument("--copy-mib", type=int, required=True)
+    a = ap.parse_args(argv)
+    if a.mode == "stats":
+        stats(a.dir, a.top)
+    else:
+        traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_mib)
+
+
+if __name__ == "__main__":
+    main()
