@@ -36,36 +36,38 @@ WORKLOADS = {
 
 # bracketed names: whole stages (their GEMMs included) and single kernels
 PHASES = ["potrf", "trsm", "trsm_h", "trsm_k", "trsm_e", "trsm_u", "sytrd", "apply_q",
-          "apply_q_x", "apply_q_o", "gemm_sytrd", "gemm_potrf", "gemm_trsm", "gemm_dc", "gemm_q",
-          "gemm_fe", "gemm_other"]
+          "apply_q_x", "apply_q_o", "apply_q_r1", "apply_q_r2", "gemm_sytrd", "gemm_potrf",
+          "gemm_trsm", "gemm_dc", "gemm_q", "gemm_fe", "gemm_refine", "gemm_other", "refine_all"]
 KERNELS = ["pattern", "group_union", "group_map", "gather", "potrf_panel", "diag_inv",
            "transpose", "sytrd_panel", "zero_upper", "dc_leaf", "dc_merge", "gemm", "larft",
-           "fe_rows", "fe_finish", "bisect", "occ_compact", "dmat_scatter"]
+           "fe_rows", "fe_finish", "bisect", "occ_compact", "dmat_scatter", "to_f32", "promote",
+           "refine"]
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def sytrd_bytes(ms):
+def sytrd_bytes(ms, elem=8):
     """Algorithmic HBM bytes of the Householder tridiagonalisation of one build
     (all panel launches): the symmetric trailing matrix (one triangle) read
-    once per reflector, 8 * sum_k (m-k)(m-k+1)/2 per submatrix."""
+    once per reflector, elem * sum_k (m-k)(m-k+1)/2 per submatrix (elem = 8
+    for FP64 storage, 4 for the float storage of the mixed-precision solve)."""
     tot = 0.0
     for m in ms:
         n = np.arange(m - 1, 1, -1, dtype=np.float64)  # trailing sizes m-1 .. 2
-        tot += float(np.sum(n * (n + 1) / 2.0)) * 8.0
+        tot += float(np.sum(n * (n + 1) / 2.0)) * elem
     return tot
 
 
-def roofline_model(name, ms, ndim):
+def roofline_model(name, ms, ndim, mixed=False):
     """(bound, algorithmic units of one build summed over the kernel's launches,
     unit); divided by the kernel's bracketed time of one build this is the
     per-launch figure averaged over its launches."""
     m = np.asarray(ms, dtype=np.float64)
     npk = ndim * (ndim + 1) / 2.0
     if name == "sytrd_panel":
-        return "hbm", sytrd_bytes(ms), "GB/s"
+        return "hbm", sytrd_bytes(ms, 4 if mixed else 8), "GB/s"
     if name == "gather":   # dense Hsm and Ssm written + packed reads counted once
         return "hbm", float(np.sum(16.0 * m * m)) + 16.0 * npk, "GB/s"
     if name == "pattern":
@@ -141,7 +143,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="h2o_512", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -230,7 +232,7 @@ def main():
     for name, (ms, cnt) in sorted(r["kernels"].items(), key=lambda kv: -kv[1][0]):
         if name not in KERNELS:
             continue
-        model = roofline_model(name, r["m"], n)
+        model = roofline_model(name, r["m"], n, mixed=main_prec == "mixed")
         if model is None:
             continue
         bound, units, unit = model
@@ -244,7 +246,7 @@ def main():
             achieved = units / (ms * 1e-3) / 1e12
             peak = FP64_PEAK_TFS
         kname = "k_" + name
-        traffic = traffic_per_launch(kname, args.config)
+        traffic = traffic_per_launch(kname, args.config + ("_mixed" if main_prec == "mixed" else ""))
         roof = {"kernel": kname, "bound": bound, "achieved": round(achieved, 2), "peak": peak,
                 "unit": unit, "frac": round(achieved / peak, 4),
                 "traffic": None if traffic is None else round(traffic, 0),
@@ -262,7 +264,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if main_prec == "fp64" else "bf16x3+f64",
+        "dtype": "f64" if main_prec == "fp64" else "f32+f64",
         "data": "synthetic (deterministic geometry; Gaussian/extended-Hueckel surrogate H,S)",
         "config": {"workload": f"{args.config} (512 H2O, 1536 atoms, ndim {n}), filter {wl['filter']}, "
                                f"atoms_columns, submatrix_sygv",

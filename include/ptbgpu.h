@@ -46,7 +46,9 @@ enum ptbgpu_status {
 /* arithmetic of the per-submatrix solve */
 enum ptbgpu_precision {
     PTBGPU_FP64 = 0,   /* plain FP64 drop-in */
-    PTBGPU_MIXED = 1   /* low-precision MFMA solve + FP64 refinement */
+    PTBGPU_MIXED = 1   /* tridiagonalisation on float storage + FP64 refinement of
+                          the occupied eigenvectors (Cholesky, K, divide and
+                          conquer, fe, mu and the density stay FP64) */
 };
 
 typedef struct ptbgpu_ctx ptbgpu_ctx;
@@ -56,7 +58,9 @@ typedef struct {
     int64_t strict;       /* 1: a non-SPD overlap block fails the call with
                              PTBGPU_E_NOT_SPD; 0: count it in the report and go on */
     int64_t verbose;      /* 1: print the reference's progress lines */
-    int64_t refine_steps; /* MIXED only: FP64 refinement steps, 0 = default */
+    int64_t refine_steps; /* MIXED only: FP64 refinement sweeps, 0 = default (1);
+                             negative: none (the float-storage accuracy, for
+                             accuracy studies) */
     /* submatrix_columns 4 (k-means): the uniform random numbers k-means++ draws,
      * restarted for every candidate as the reference's initseed does
      * (source/sm.f90:20-32,459-470): stream[0..nstream), nstream >= 64 covers

@@ -36,7 +36,12 @@ constexpr int kKS = 32;
 // tile.  Every product's tiles share one XCD's L2, and products of any shape
 // spread evenly over the XCDs (a (tile, product) grid would put tile t of every
 // product on XCD t % 8, idling most of the chip for narrow products).
-template <bool TA, bool TB>
+//
+// T is the storage type of A, B and C (the descriptor's pointers are read as
+// T*, leading dimensions in elements of T); products and sums are FP64 either
+// way, so float storage halves the traffic of the FP32 tridiagonalisation's
+// updates without changing the arithmetic.
+template <bool TA, bool TB, typename T>
 __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ descs, int count,
                                                int tiles) {
     const int64_t lin = blockIdx.x;
@@ -45,6 +50,9 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
     const int tile = (int)(slot % tiles);
     if (gi >= count) return;
     const GemmDesc g = descs[gi];
+    const T *__restrict__ gA = reinterpret_cast<const T *>(g.A);
+    const T *__restrict__ gB = reinterpret_cast<const T *>(g.B);
+    T *gC = reinterpret_cast<T *>(g.C);
     const int tilesM = (g.M + 63) / 64, tilesN = (g.N + 63) / 64;
     if (g.M <= 0 || g.N <= 0 || tile >= tilesM * tilesN) return;
     const int m0 = (tile % tilesM) * 64, n0 = (tile / tilesM) * 64;
@@ -76,13 +84,13 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
             a_index(q, mm, kk);
             const int gm = m0 + mm, gk = k0 + kk;
             ra[q] = (gm < g.M && gk < g.K)
-                        ? (TA ? g.A[(int64_t)gm * g.lda + gk] : g.A[(int64_t)gk * g.lda + gm])
+                        ? (double)(TA ? gA[(int64_t)gm * g.lda + gk] : gA[(int64_t)gk * g.lda + gm])
                         : 0.0;
             int kb, nn;
             b_index(q, kb, nn);
             const int gn = n0 + nn, gkb = k0 + kb;
             rb[q] = (gn < g.N && gkb < g.K)
-                        ? (TB ? g.B[(int64_t)gkb * g.ldb + gn] : g.B[(int64_t)gn * g.ldb + gkb])
+                        ? (double)(TB ? gB[(int64_t)gkb * g.ldb + gn] : gB[(int64_t)gn * g.ldb + gkb])
                         : 0.0;
         }
     };
@@ -128,9 +136,9 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
             for (int r = 0; r < 4; ++r) {
                 const int gm = m0 + wm + 16 * i + 4 * r + lk;
                 if (gm >= g.M || (g.uplo && gm < gn)) continue;
-                double *c = g.C + (int64_t)gn * g.ldc + gm;
+                T *c = gC + (int64_t)gn * g.ldc + gm;
                 const double v = acc[i][j][r];
-                *c = g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * *c;
+                *c = (T)(g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * (double)*c);
             }
         }
 }
@@ -241,6 +249,47 @@ __global__ __launch_bounds__(kT) void k_zero_upper(const SubDesc *__restrict__ d
     if (c >= d.m) return;
     double *col = Aa + d.off + (int64_t)c * d.ld;
     for (int r = threadIdx.x; r <= c; r += kT) col[r] = 0.0;
+}
+
+// Float copy of every arena block (column c of submatrix s per workgroup row):
+// the input of the FP32 tridiagonalisation.  grid: (submatrix, column)
+__global__ __launch_bounds__(kT) void k_to_f32(const SubDesc *__restrict__ desc,
+                                               const double *__restrict__ Aa,
+                                               float *__restrict__ Fa) {
+    const SubDesc d = desc[blockIdx.x];
+    const int c = blockIdx.y;
+    if (c >= d.m) return;
+    const double *src = Aa + d.off + (int64_t)c * d.ld;
+    float *dst = Fa + d.off + (int64_t)c * d.ld;
+    for (int r = threadIdx.x; r < d.m; r += kT) dst[r] = (float)src[r];
+}
+
+// Reflectors of the FP32 tridiagonalisation in FP64: V(:, c) = the stored
+// v_c (rows > c, unit at c+1), zero above, and tau_c = 2 / (v_c^T v_c) so that
+// every H_c = I - tau_c v_c v_c^T is orthogonal to FP64 rounding (tau_c = 0,
+// no reflection, stays 0).  One wavefront per column.
+// grid: (submatrix, ceil(max_m / 4)), 256 threads
+__global__ __launch_bounds__(kT) void k_promote_reflectors(const SubDesc *__restrict__ desc,
+                                                           const float *__restrict__ Fa,
+                                                           double *__restrict__ Va,
+                                                           double *__restrict__ tau_all) {
+    const SubDesc d = desc[blockIdx.x];
+    const int c = blockIdx.y * (kT / PTB_WAVE) + threadIdx.x / PTB_WAVE;
+    const int lane = threadIdx.x % PTB_WAVE;
+    if (c >= d.m) return;
+    const float *src = Fa + d.off + (int64_t)c * d.ld;
+    double *dst = Va + d.off + (int64_t)c * d.ld;
+    double ss = 0.0;
+    for (int r = lane; r < d.m; r += PTB_WAVE) {
+        const double v = r > c ? (double)src[r] : 0.0;
+        dst[r] = v;
+        ss += v * v;
+    }
+    ss = wave_sum(ss);
+    if (lane == 0 && c < d.m - 2) {
+        double *t = tau_all + d.mapoff + c;
+        if (*t != 0.0) *t = 2.0 / ss;
+    }
 }
 
 // T factor of every reflector block of every submatrix in one launch (grid:
@@ -368,7 +417,7 @@ __global__ __launch_bounds__(64) void k_diag_inv(const SubDesc *__restrict__ des
 namespace ptb {
 
 int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int count, int max_m,
-                int max_n, const char *site) {
+                int max_n, const char *site, bool f32) {
     ProfScope prof_(ctx, "gemm");
     ProfScope sprof_(ctx, site ? (std::string("gemm_") + site).c_str() : "gemm_other");
     if (count <= 0 || max_m <= 0 || max_n <= 0) return PTBGPU_OK;
@@ -378,10 +427,13 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
     if (total > 0x7fffffff) return fail(ctx, PTBGPU_E_ARG, "GEMM batch too large (%lld workgroups)",
                                         (long long)total);
     dim3 grid((unsigned)total);
-    if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
-    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
-    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
-    else hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
+    if (f32) {  // float storage: only the tridiagonal update's form is used
+        if (ta || !tb) return fail(ctx, PTBGPU_E_ARG, "float-storage GEMM supports A B^T only");
+        hipLaunchKernelGGL((k_gemm<false, true, float>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
+    } else if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
+    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
+    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
+    else hipLaunchKernelGGL((k_gemm<true, true, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
@@ -515,6 +567,24 @@ int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_
 int zero_upper_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, double *A) {
     ProfScope prof_(ctx, "zero_upper");
     hipLaunchKernelGGL(k_zero_upper, dim3(nsm, max_m), dim3(kT), 0, ctx->stream, ddesc, A);
+    PTB_CHECK_LAUNCH(ctx);
+    return PTBGPU_OK;
+}
+
+int to_f32_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *A,
+                 float *F) {
+    ProfScope prof_(ctx, "to_f32");
+    hipLaunchKernelGGL(k_to_f32, dim3(nsm, max_m), dim3(kT), 0, ctx->stream, ddesc, A, F);
+    PTB_CHECK_LAUNCH(ctx);
+    return PTBGPU_OK;
+}
+
+int promote_reflectors(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const float *F,
+                       double *V, double *tau) {
+    ProfScope prof_(ctx, "promote");
+    const int per = kT / PTB_WAVE;
+    hipLaunchKernelGGL(k_promote_reflectors, dim3(nsm, (max_m + per - 1) / per), dim3(kT), 0,
+                       ctx->stream, ddesc, F, V, tau);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }

@@ -129,7 +129,8 @@ __global__ __launch_bounds__(kT) void k_occ_compact(const SubDesc *__restrict__ 
                                                     const double *__restrict__ Za,
                                                     double *__restrict__ Zo,
                                                     double *__restrict__ wocc,
-                                                    int *__restrict__ nocc) {
+                                                    int *__restrict__ nocc,
+                                                    int *__restrict__ oidx) {
     extern __shared__ int occk[];  // m
     __shared__ int wtot[kT / PTB_WAVE];
     __shared__ int base;
@@ -151,6 +152,7 @@ __global__ __launch_bounds__(kT) void k_occ_compact(const SubDesc *__restrict__ 
         if (w != 0.0) {
             occk[off + before] = k;
             wocc[d.mapoff + off + before] = w;
+            if (oidx) oidx[d.mapoff + off + before] = k;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -169,6 +171,91 @@ __global__ __launch_bounds__(kT) void k_occ_compact(const SubDesc *__restrict__ 
         double *dst = Y + (int64_t)t * d.ld;
         for (int r = threadIdx.x; r < d.m; r += kT) dst[r] = src[r];
     }
+}
+
+// ---- FP64 refinement of the occupied eigenvectors (mixed precision) -------
+// With float storage in the tridiagonalisation, T and Q satisfy
+// Q^T K Q = T + D with D a dense perturbation of float rounding size, and the
+// exact eigenpairs (theta, Z) of T are first-order accurate for K.  One
+// refinement sweep, in the basis of Z, for the occupied columns z_j (compacted
+// in Zo, state index oidx[j]):
+//   R = Q^T K Q Zo - Zo diag(theta_o)          (= D Zo, FP64)
+//   C = Z^T R,  E(i, j) = C(i, j) / (theta_j - theta_i) for unoccupied i, 0 else
+//   Zo <- (Zo + Z E) (3 I - G) / 2,  G = Zo^T Zo   (first-order Loewdin)
+//   theta_j <- theta_j + C(oidx_j, j)              (the Rayleigh quotient)
+// Rotations among occupied states leave the density unchanged, so only the
+// occupied-unoccupied couplings are removed; the error drops from O(|D|) to
+// O(|D|^2 / gap).
+
+// R(:, j) -= theta_{oidx_j} Zo(:, j).  grid: (submatrix, occupied column)
+__global__ __launch_bounds__(kT) void k_refine_resid(const SubDesc *__restrict__ desc,
+                                                     const int *__restrict__ nocc,
+                                                     const int *__restrict__ oidx,
+                                                     const double *__restrict__ eig_all,
+                                                     const double *__restrict__ Zo,
+                                                     double *__restrict__ R) {
+    const SubDesc d = desc[blockIdx.x];
+    const int j = blockIdx.y;
+    if (j >= nocc[blockIdx.x]) return;
+    const double th = eig_all[d.mapoff + oidx[d.mapoff + j]];
+    const double *z = Zo + d.off + (int64_t)j * d.ld;
+    double *r = R + d.off + (int64_t)j * d.ld;
+    for (int i = threadIdx.x; i < d.m; i += kT) r[i] -= th * z[i];
+}
+
+// C -> E in place and the refined eigenvalues thn[mapoff + j].
+// grid: (submatrix, occupied column)
+__global__ __launch_bounds__(kT) void k_refine_corr(const SubDesc *__restrict__ desc,
+                                                    const int *__restrict__ nocc,
+                                                    const int *__restrict__ oidx,
+                                                    const double *__restrict__ eig_all,
+                                                    const double *__restrict__ mu_p,
+                                                    double *__restrict__ C,
+                                                    double *__restrict__ thn) {
+    const SubDesc d = desc[blockIdx.x];
+    const int j = blockIdx.y;
+    if (j >= nocc[blockIdx.x]) return;
+    const double mu = mu_p[0];
+    const double *eig = eig_all + d.mapoff;
+    const int kj = oidx[d.mapoff + j];
+    const double th = eig[kj];
+    double *c = C + d.off + (int64_t)j * d.ld;
+    for (int i = threadIdx.x; i < d.m; i += kT) {
+        const double x = c[i];
+        if (i == kj) thn[d.mapoff + j] = th + x;
+        const double gap = th - eig[i];
+        c[i] = (occ_of(eig[i], mu) == 0.0 && fabs(gap) > 1e-12) ? x / gap : 0.0;
+    }
+}
+
+// G <- (3 I - G) / 2 on the leading nocc x nocc block.
+// grid: (submatrix, column)
+__global__ __launch_bounds__(kT) void k_refine_shift(const SubDesc *__restrict__ desc,
+                                                     const int *__restrict__ nocc,
+                                                     double *__restrict__ G) {
+    const SubDesc d = desc[blockIdx.x];
+    const int j = blockIdx.y, n = nocc[blockIdx.x];
+    if (j >= n) return;
+    double *g = G + d.off + (int64_t)j * d.ld;
+    for (int i = threadIdx.x; i < n; i += kT) g[i] = (i == j ? 1.5 : 0.0) - 0.5 * g[i];
+}
+
+// Z(:, oidx_j) = Xo(:, j), eig[oidx_j] = thn[j].  grid: (submatrix, occupied column)
+__global__ __launch_bounds__(kT) void k_refine_scatter(const SubDesc *__restrict__ desc,
+                                                       const int *__restrict__ nocc,
+                                                       const int *__restrict__ oidx,
+                                                       const double *__restrict__ Xo,
+                                                       const double *__restrict__ thn,
+                                                       double *__restrict__ Z,
+                                                       double *__restrict__ eig_all) {
+    const SubDesc d = desc[blockIdx.x];
+    const int j = blockIdx.y;
+    if (j >= nocc[blockIdx.x]) return;
+    const int kj = oidx[d.mapoff + j];
+    const double *x = Xo + d.off + (int64_t)j * d.ld;
+    double *z = Z + d.off + (int64_t)kj * d.ld;
+    for (int i = threadIdx.x; i < d.m; i += kT) z[i] = x[i];
+    if (threadIdx.x == 0) eig_all[d.mapoff + kj] = thn[d.mapoff + j];
 }
 
 // P(:, j) = U_o diag(w) U_o(loc_j, :)^T for the owned columns j, with U_o the
@@ -252,10 +339,50 @@ int bisect_mu(ptbgpu_ctx *ctx, const double *e, const double *fe, int64_t total,
 }
 
 int occ_compact(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *eig,
-                const double *dmu, const double *Z, double *Zo, double *wocc, int *nocc) {
+                const double *dmu, const double *Z, double *Zo, double *wocc, int *nocc,
+                int *oidx) {
     ProfScope prof_(ctx, "occ_compact");
     hipLaunchKernelGGL(k_occ_compact, dim3(nsm), dim3(kT), sizeof(int) * (size_t)max_m,
-                       ctx->stream, ddesc, eig, dmu, Z, Zo, wocc, nocc);
+                       ctx->stream, ddesc, eig, dmu, Z, Zo, wocc, nocc, oidx);
+    PTB_CHECK_LAUNCH(ctx);
+    return PTBGPU_OK;
+}
+
+int refine_resid(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, const int *nocc,
+                 const int *oidx, const double *eig, const double *Zo, double *R) {
+    ProfScope prof_(ctx, "refine");
+    if (max_nocc <= 0) return PTBGPU_OK;
+    hipLaunchKernelGGL(k_refine_resid, dim3(nsm, max_nocc), dim3(kT), 0, ctx->stream, ddesc, nocc,
+                       oidx, eig, Zo, R);
+    PTB_CHECK_LAUNCH(ctx);
+    return PTBGPU_OK;
+}
+
+int refine_corr(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, const int *nocc,
+                const int *oidx, const double *eig, const double *dmu, double *C, double *thn) {
+    ProfScope prof_(ctx, "refine");
+    if (max_nocc <= 0) return PTBGPU_OK;
+    hipLaunchKernelGGL(k_refine_corr, dim3(nsm, max_nocc), dim3(kT), 0, ctx->stream, ddesc, nocc,
+                       oidx, eig, dmu, C, thn);
+    PTB_CHECK_LAUNCH(ctx);
+    return PTBGPU_OK;
+}
+
+int refine_shift(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, const int *nocc,
+                 double *G) {
+    ProfScope prof_(ctx, "refine");
+    if (max_nocc <= 0) return PTBGPU_OK;
+    hipLaunchKernelGGL(k_refine_shift, dim3(nsm, max_nocc), dim3(kT), 0, ctx->stream, ddesc, nocc, G);
+    PTB_CHECK_LAUNCH(ctx);
+    return PTBGPU_OK;
+}
+
+int refine_scatter(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, const int *nocc,
+                   const int *oidx, const double *Xo, const double *thn, double *Z, double *eig) {
+    ProfScope prof_(ctx, "refine");
+    if (max_nocc <= 0) return PTBGPU_OK;
+    hipLaunchKernelGGL(k_refine_scatter, dim3(nsm, max_nocc), dim3(kT), 0, ctx->stream, ddesc, nocc,
+                       oidx, Xo, thn, Z, eig);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }

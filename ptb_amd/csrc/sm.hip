@@ -271,6 +271,69 @@ int kmeans_groups(ptbgpu_ctx *ctx, int64_t ndim, int64_t n, const double *xyz,
     return PTBGPU_OK;
 }
 
+// One FP64 refinement sweep of the eigenvectors occupied at mu = bis[0] (the
+// mixed-precision solve, whose Q and T come from a float-storage
+// tridiagonalisation of K; A holds K in FP64, both triangles):
+//   Zo = occupied columns of Z;  R = Q^T K Q Zo - Zo diag(theta_o);  C = Z^T R;
+//   E = C / (theta_j - theta_i) on unoccupied rows;  Zo += Z E;
+//   Zo <- Zo (3 I - Zo^T Zo) / 2;  theta_j += C(j, j)
+// and the refined columns and eigenvalues are written back into Z and eig.
+// Uses W, X and R as scratch.
+int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                    const QFactor &qf, const double *K, double *Z, double *W, double *X, double *R,
+                    double *eig, const double *bis, double *wocc, int *nocc, int *oidx,
+                    double *thn) {
+    hipStream_t st = ctx->stream;
+    const int nsm = (int)desc.size();
+    int max_m = 0;
+    for (const auto &d : desc) max_m = std::max(max_m, d.m);
+    PTB_TRY(ptb::occ_compact(ctx, ddesc, nsm, max_m, eig, bis, Z, W, wocc, nocc, oidx));  // W = Zo
+    PTB_TRY(ptb::occ_compact(ctx, ddesc, nsm, max_m, eig, bis, Z, X, wocc, nocc, nullptr));
+    std::vector<int> hn(nsm);
+    PTB_HIP(ctx, hipMemcpyAsync(hn.data(), nocc, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
+    PTB_HIP(ctx, hipStreamSynchronize(st));
+    int max_n = 0;
+    std::vector<int64_t> off(nsm);
+    for (int s = 0; s < nsm; ++s) {
+        max_n = std::max(max_n, hn[s]);
+        off[s] = desc[s].off;
+    }
+    if (max_n == 0) return PTBGPU_OK;
+    ptb::ProfScope prof_(ctx, "refine_all");
+    PTB_TRY(ptb::apply_q(ctx, desc, qf, X, off, hn, false, "r1"));  // X = Q Zo
+    std::vector<GemmDesc> g(nsm);
+    auto run = [&](const char *name, bool ta, int mm, int nn, auto make) -> int {
+        for (int s = 0; s < nsm; ++s) g[s] = make(desc[s], hn[s]);
+        GemmDesc *dg;
+        PTB_TRY(upload(ctx, name, g, &dg));
+        return ptb::gemm_vbatch(ctx, ta, false, dg, nsm, mm, nn, "refine");
+    };
+    // R = K (Q Zo)
+    PTB_TRY(run("rf_g1", false, max_m, max_n, [&](const SubDesc &d, int n) {
+        return GemmDesc{K + d.off, X + d.off, R + d.off, d.m, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
+    }));
+    PTB_TRY(ptb::apply_q(ctx, desc, qf, R, off, hn, true, "r2"));  // R = Q^T K Q Zo
+    PTB_TRY(ptb::refine_resid(ctx, ddesc, nsm, max_n, nocc, oidx, eig, W, R));
+    // X = Z^T R, then E in place and the Rayleigh quotients
+    PTB_TRY(run("rf_g2", true, max_m, max_n, [&](const SubDesc &d, int n) {
+        return GemmDesc{Z + d.off, R + d.off, X + d.off, d.m, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
+    }));
+    PTB_TRY(ptb::refine_corr(ctx, ddesc, nsm, max_n, nocc, oidx, eig, bis, X, thn));
+    // W = Zo + Z E
+    PTB_TRY(run("rf_g3", false, max_m, max_n, [&](const SubDesc &d, int n) {
+        return GemmDesc{Z + d.off, X + d.off, W + d.off, d.m, n, d.m, d.ld, d.ld, d.ld, 1.0, 1.0, 0, 0};
+    }));
+    // R = (3 I - W^T W) / 2, X = W R
+    PTB_TRY(run("rf_g4", true, max_n, max_n, [&](const SubDesc &d, int n) {
+        return GemmDesc{W + d.off, W + d.off, R + d.off, n, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
+    }));
+    PTB_TRY(ptb::refine_shift(ctx, ddesc, nsm, max_n, nocc, R));
+    PTB_TRY(run("rf_g5", false, max_m, max_n, [&](const SubDesc &d, int n) {
+        return GemmDesc{W + d.off, R + d.off, X + d.off, d.m, n, n, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
+    }));
+    return ptb::refine_scatter(ctx, ddesc, nsm, max_n, nocc, oidx, X, thn, Z, eig);
+}
+
 int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const double *dH,
                       const double *dS, double *dP, int64_t natoms, const double *xyz,
                       const int64_t *aoat, int64_t cols, int64_t mode, const ptbgpu_opts &opts,
@@ -389,8 +452,22 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, Z, A, false));  // A = (L^-1 H)^T
     PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, A, Z, false, "k"));        // Z = K
     PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, Z, A, true));   // A = sym(K)
+    // mixed precision: the tridiagonalisation on float storage, its reflectors
+    // in V, and K (FP64) kept in A for the refinement below
+    const bool mixed = opts.precision == PTBGPU_MIXED;
+    float *K32 = nullptr;
+    double *V = nullptr, *R = nullptr, *thn = nullptr;
+    int *oidx = nullptr;
+    if (mixed) {
+        K32 = static_cast<float *>(ptb::ws(ctx, "sm_K32", sizeof(float) * (size_t)arena));
+        V = static_cast<double *>(ptb::ws(ctx, "sm_V", abytes));
+        R = static_cast<double *>(ptb::ws(ctx, "sm_R", abytes));
+        thn = static_cast<double *>(ptb::ws(ctx, "sm_thn", sizeof(double) * rows));
+        oidx = static_cast<int *>(ptb::ws(ctx, "sm_oidx", sizeof(int) * rows));
+        if (!K32 || !V || !R || !thn || !oidx) return PTBGPU_E_NOMEM;
+    }
     QFactor qf;
-    PTB_TRY(ptb::eig_tri_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot, qf));
+    PTB_TRY(ptb::eig_tri_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot, qf, K32, V));
     tm.mark(4, st);
     // ---- fe from the owned rows: F = Z^T Q^T [L(O,:)^T, L^-1 e_O]
     std::vector<int> hflags(nsm);
@@ -398,13 +475,13 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     PTB_TRY(ptb::xall_build(ctx, ddesc, nsm, max_own, doloc, B, dxoff, dxboff, XA, XB));
     PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, XB, XA, false, "e", &lrhs));  // L^-1 e_O
     PTB_TRY(ptb::apply_q(ctx, desc, qf, XA, xoff, c2, true, "x"));           // Q^T XA
+    GemmDesc *dgf;
     {
         std::vector<GemmDesc> gf(nsm);
         for (int s = 0; s < nsm; ++s) {
             const SubDesc &d = desc[s];
             gf[s] = {Z + d.off, XA + xoff[s], F + xoff[s], d.m, c2[s], d.m, d.ld, d.ld, d.ld, 1.0, 0.0};
         }
-        GemmDesc *dgf;
         PTB_TRY(upload(ctx, "sm_gf", gf, &dgf));
         PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgf, nsm, max_m, 2 * max_own, "fe"));
     }
@@ -425,6 +502,17 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     tm.mark(5, st);
     // ---- mu bisection
     PTB_TRY(ptb::bisect_mu(ctx, dvec, fe, rows, nelref, bis));
+    const int sweeps = !mixed || opts.refine_steps < 0 ? 0 : (int)std::max<int64_t>(1, opts.refine_steps);
+    if (sweeps > 0) {
+        // FP64 refinement of the states occupied at this mu, then fe and the
+        // bisection again with the refined eigenpairs
+        for (int sweep = 0; sweep < sweeps; ++sweep)
+            PTB_TRY(refine_occupied(ctx, desc, ddesc, qf, A, Z, W, X, R, dvec, bis, wocc, nocc,
+                                    oidx, thn));
+        PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgf, nsm, max_m, 2 * max_own, "fe"));
+        PTB_TRY(ptb::fe_proj(ctx, ddesc, nsm, max_m, dxoff, F, fe));
+        PTB_TRY(ptb::bisect_mu(ctx, dvec, fe, rows, nelref, bis));
+    }
     tm.mark(6, st);
     // ---- final density: occupied columns only, Y_o = Q Z_o, U_o = L^-T Y_o
     PTB_TRY(ptb::occ_compact(ctx, ddesc, nsm, max_m, dvec, bis, Z, W, wocc, nocc));
@@ -501,7 +589,7 @@ namespace ptb {
 // vectors and qf the blocked form of Q (apply_q multiplies by Q or Q^T).
 int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
-                  double *tau, double *rot, QFactor &qf) {
+                  double *tau, double *rot, QFactor &qf, float *K32, double *V) {
     hipStream_t st = ctx->stream;
     const int nsm = (int)desc.size();
     int max_m = 0;
@@ -511,9 +599,16 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
         arena = std::max(arena, (size_t)(x.off + (int64_t)x.ld * x.m));
     }
     const size_t abytes = sizeof(double) * arena;
-    PTB_TRY(sytrd_batch(ctx, desc, ddesc, A, dvec, evec, tau));
-    PTB_TRY(zero_upper_batch(ctx, ddesc, nsm, max_m, A));              // A = V
-    PTB_TRY(larft_all(ctx, desc, ddesc, A, tau, qf));
+    if (K32 && V) {  // float storage, FP64 reflectors into V, A untouched
+        PTB_TRY(to_f32_batch(ctx, ddesc, nsm, max_m, A, K32));
+        PTB_TRY(sytrd_batch_f32(ctx, desc, ddesc, K32, dvec, evec, tau));
+        PTB_TRY(promote_reflectors(ctx, ddesc, nsm, max_m, K32, V, tau));
+        PTB_TRY(larft_all(ctx, desc, ddesc, V, tau, qf));
+    } else {
+        PTB_TRY(sytrd_batch(ctx, desc, ddesc, A, dvec, evec, tau));
+        PTB_TRY(zero_upper_batch(ctx, ddesc, nsm, max_m, A));              // A = V
+        PTB_TRY(larft_all(ctx, desc, ddesc, A, tau, qf));
+    }
     // divide and conquer
     std::vector<DcNode> leaves;
     std::vector<std::vector<DcNode>> levels;

@@ -66,8 +66,9 @@ int gather(ptbgpu_ctx *ctx, const double *dH, const double *dS, const SubDesc *d
            int max_m, const int *dmap, double *Hd, double *Sd);
 
 // dense.hip
+// f32: A, B, C are float arrays behind the descriptor's pointers (A B^T only)
 int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int count, int max_m,
-                int max_n, const char *site = nullptr);
+                int max_n, const char *site = nullptr, bool f32 = false);
 int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                 double *A, int *flags);
 int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m,
@@ -78,6 +79,13 @@ int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
 int zero_upper_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, double *A);
 int larft_all(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
               const double *V, const double *tau, QFactor &qf);
+// float copy of the arena blocks (same offsets and leading dimensions)
+int to_f32_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *A,
+                 float *F);
+// FP64 reflectors (zero above the subdiagonal) and orthogonal tau from the
+// float output of sytrd_batch_f32
+int promote_reflectors(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const float *F,
+                       double *V, double *tau);
 int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf, double *Y,
             const std::vector<int64_t> &yoff, const std::vector<int> &ncols, bool trans,
             const char *tag);
@@ -85,11 +93,17 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
 // tridiag.hip
 int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                 double *A, double *d, double *e, double *tau);
+// the same on float storage (FP64 arithmetic): d, e, tau in FP64, reflectors in A
+int sytrd_batch_f32(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                    float *A, double *d, double *e, double *tau);
 
 // sm.hip
+// With K32 and V given, the tridiagonalisation runs on float storage (K32) and
+// the FP64 reflectors go to V; A (the FP64 matrix) is left intact.
 int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
-                  double *tau, double *rot, QFactor &qf);
+                  double *tau, double *rot, QFactor &qf, float *K32 = nullptr,
+                  double *V = nullptr);
 int eig_sym_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
                   double *tau, double *rot);
@@ -115,7 +129,17 @@ int fe_proj(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const int
 int bisect_mu(ptbgpu_ctx *ctx, const double *e, const double *fe, int64_t total, int64_t nelref,
               double *dout);
 int occ_compact(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *eig,
-                const double *dmu, const double *Z, double *Zo, double *wocc, int *nocc);
+                const double *dmu, const double *Z, double *Zo, double *wocc, int *nocc,
+                int *oidx = nullptr);
+// FP64 refinement of the occupied eigenvectors of Q^T K Q (density.hip)
+int refine_resid(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, const int *nocc,
+                 const int *oidx, const double *eig, const double *Zo, double *R);
+int refine_corr(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, const int *nocc,
+                const int *oidx, const double *eig, const double *dmu, double *C, double *thn);
+int refine_shift(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, const int *nocc,
+                 double *G);
+int refine_scatter(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, const int *nocc,
+                   const int *oidx, const double *Xo, const double *thn, double *Z, double *eig);
 int dmat_compact(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, int max_own,
                  const int *dgcols, const int *doloc, const int *dmap, const double *wocc,
                  const int *nocc, const double *U, double *P);

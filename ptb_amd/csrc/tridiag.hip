@@ -15,6 +15,11 @@
 // Output: d (diagonal), e (subdiagonal, e[k] couples k and k+1), tau, and v_k
 // stored in column k rows k+1.. of A (unit at row k+1).  W is a work panel
 // (m x 32, leading dimension ld) in a separate panel buffer.
+//
+// The matrix and the panel buffer are stored as T: double for the plain
+// solve, float for the mixed-precision one (half the bytes of the GEMV).
+// Sums, the reflectors and the LDS vectors are FP64 in both cases; only the
+// stored values are rounded to T.
 #include "sm_internal.hpp"
 
 namespace {
@@ -92,9 +97,10 @@ __device__ __forceinline__ double butterfly8(const double (&p)[8], int lane) {
     return v;
 }
 
+template <typename T>
 __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict__ desc,
-                                                    double *__restrict__ Aa,
-                                                    double *__restrict__ Wa,
+                                                    T *__restrict__ Aa,
+                                                    T *__restrict__ Wa,
                                                     double *__restrict__ d_all,
                                                     double *__restrict__ e_all,
                                                     double *__restrict__ tau_all,
@@ -102,11 +108,11 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
     extern __shared__ double sm[];
     const SubDesc ds = desc[blockIdx.x];
     const int m = ds.m, ld = ds.ld;
-    double *a = Aa + ds.off;
+    T *a = Aa + ds.off;
     // panel buffer [W | V | W], ld x 96: W columns 0..31, a copy of the panel's
     // reflectors in 32..63 and of W in 64..95, so that the trailing update
     // A -= W V^T + V W^T is one product [W V] [V W]^T
-    double *W = Wa + pwoff[blockIdx.x];
+    T *W = Wa + pwoff[blockIdx.x];
     double *dd = d_all + ds.mapoff, *ee = e_all + ds.mapoff, *tt = tau_all + ds.mapoff;
     double *vs = sm;              // m   current Householder vector (rows > k)
     double *cp = vs + m;          // kWaves x m  per-wavefront partials of A22 v
@@ -126,9 +132,9 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
         if (kb < kNB) {  // the product reads all 64 columns: zero the unused ones
             for (int l = threadIdx.x; l < (kNB - kb) * m; l += kT) {
                 const int jj = kb + l / m, i = l % m;
-                W[(int64_t)jj * ld + i] = 0.0;
-                W[(int64_t)(kNB + jj) * ld + i] = 0.0;
-                W[(int64_t)(2 * kNB + jj) * ld + i] = 0.0;
+                W[(int64_t)jj * ld + i] = T(0);
+                W[(int64_t)(kNB + jj) * ld + i] = T(0);
+                W[(int64_t)(2 * kNB + jj) * ld + i] = T(0);
             }
             __syncthreads();
         }
@@ -173,8 +179,8 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
             for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
                 const double v = (i == k + 1) ? 1.0 : vs[i] * scale;
                 vs[i] = v;
-                a[(int64_t)k * ld + i] = v;
-                W[(int64_t)(kNB + jj) * ld + i] = v;
+                a[(int64_t)k * ld + i] = (T)v;
+                W[(int64_t)(kNB + jj) * ld + i] = (T)v;
             }
             if (threadIdx.x == 0) {
                 ee[k] = beta;
@@ -213,54 +219,88 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
                 __syncthreads();
             }
             // (e) praw = A22 v from the lower triangle only (A22 is symmetric
-            // and only its lower triangle is kept up to date), tile by tile
-            // (64 rows, one per lane, x 16 columns).  A tile's row sums and
-            // its transposed (strictly lower) products, summed over the
-            // wavefront by a butterfly, are added to the wavefront's own
-            // partial vector cp[wid][:]; praw = sum of the partial vectors in
-            // fixed order.
+            // and only its lower triangle is kept up to date).  Vertical
+            // strips of 16 columns c0 .. c0+15 (c0 = base + 16 s, base = k+1
+            // rounded down to 16, so that column segments start on 64-byte
+            // lines) over the rows c0 .. m-1 in blocks of 64 (one row per
+            // lane); wavefront wid takes the strips wid, wid + kWaves, ...
+            // Per element A(i, c) one load and two FMAs: the row sum
+            // y_i += A(i, c) v_c and the column partial z_c += A(i, c) v_i,
+            // kept in registers down the strip.  The row sums of a block go
+            // to the wavefront's partial vector cp[wid][:]; the 16 column
+            // partials are summed over the wavefront once per strip by a
+            // butterfly.  praw = sum of the partial vectors in fixed order.
             for (int i = k + 1 + threadIdx.x; i < m; i += kT)
 #pragma unroll
                 for (int w = 0; w < kWaves; ++w) cp[w * m + i] = 0.0;
             __syncthreads();
             {
-                // lower-triangle tiles of 64 rows x 16 columns over rows and
-                // columns base .. m-1, base = (k+1) rounded down to 16 so that
-                // every 64-row column segment starts on a 128-byte line (rows
-                // and columns below k+1 are masked); numbered row block by row
-                // block: block bi holds 4 (bi + 1) tiles (the last one
-                // ceil(nb / 16)), so tile t lies in the block bi with
-                // 2 bi (bi + 1) <= t < 2 (bi + 1) (bi + 2).  Wavefront wid
-                // takes the tiles wid, wid + kWaves, ...
                 const int base = (k + 1) & ~15;
-                const int nb = m - base;
-                const int nrb = (nb + 63) / 64;
-                const int ntile = 2 * (nrb - 1) * nrb + (nb + 15) / 16;
+                const int nstrip = (m - base + 15) / 16;
                 double *cpw = cp + wid * m;
-                for (int t = wid; t < ntile; t += kWaves) {
-                    int bi = (int)((sqrtf(2.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-                    while (2 * (bi + 1) * (bi + 2) <= t) ++bi;
-                    while (bi > 0 && 2 * bi * (bi + 1) > t) --bi;
-                    const int cj = t - 2 * bi * (bi + 1);
-                    const int i = base + bi * 64 + lane;
-                    const int c0 = base + cj * 16;
-                    const bool ok = i < m && i > k;
-                    double x[16];
+                const unsigned uld = (unsigned)ld;
+                // loads of one 64 x 16 block through 32-bit offsets from the
+                // submatrix block, all in flight together (no branch per
+                // element); the last row block and the last strip clamp to
+                // in-bounds addresses (their excess is masked below)
+                auto fetch = [&](int r, int c0, T (&raw)[16]) {
+                    const unsigned ic = (unsigned)min(r + lane, m - 1);
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) raw[q] = a[(unsigned)min(c0 + q, m - 1) * uld + ic];
+                };
+                // with float storage the next block's loads are issued before
+                // this block's arithmetic
+                constexpr bool kAhead = sizeof(T) == 4;
+                for (int sidx = wid; sidx < nstrip; sidx += kWaves) {
+                    const int c0 = base + sidx * 16;
+                    double vc[16], zc[16];
 #pragma unroll
                     for (int q = 0; q < 16; ++q) {
                         const int c = c0 + q;
-                        x[q] = (ok && c <= i && c > k) ? a[(int64_t)c * ld + i] : 0.0;
+                        vc[q] = (c > k && c < m) ? vs[c] : 0.0;
+                        zc[q] = 0.0;
                     }
-                    const double vi = ok ? vs[i] : 0.0;
-                    double yrow = 0.0;
+                    const bool cols_in = c0 > k && c0 + 16 <= m;  // no column of the strip masked
+                    T raw[16];
+                    fetch(c0, c0, raw);
+                    for (int r = c0; r < m; r += 64) {
+                        T rawn[16];
+                        if (kAhead) fetch(min(r + 64, m - 1), c0, rawn);  // past the end: clamped, unused
+                        const int i = r + lane;
+                        double yrow = 0.0;
+                        if (cols_in && r > c0 && r + 64 <= m) {
+                            // interior block: strictly below the diagonal, all rows
+                            // and columns inside the trailing matrix
+                            const double vi = vs[i];
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) yrow += x[q] * vs[max(k + 1, min(c0 + q, m - 1))];
+                            for (int q = 0; q < 16; ++q) {
+                                const double x = (double)raw[q];
+                                yrow += x * vc[q];
+                                zc[q] += x * vi;
+                            }
+                            cpw[i] += yrow;
+                        } else {
+                            const bool ok = i < m && i > k;
+                            const double vi = ok ? vs[i] : 0.0;
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) x[q] = (c0 + q < i) ? x[q] * vi : 0.0;
-                    const double colsum = butterfly16(x, lane);
+                            for (int q = 0; q < 16; ++q) {
+                                const int c = c0 + q;
+                                const double x = (ok && c <= i && c > k) ? (double)raw[q] : 0.0;
+                                yrow += x * vc[q];
+                                if (c < i) zc[q] += x * vi;
+                            }
+                            if (ok) cpw[i] += yrow;
+                        }
+                        if (kAhead) {
+#pragma unroll
+                            for (int q = 0; q < 16; ++q) raw[q] = rawn[q];
+                        } else if (r + 64 < m) {
+                            fetch(r + 64, c0, raw);
+                        }
+                    }
+                    const double colsum = butterfly16(zc, lane);
                     const int cq = c0 + ((lane >> 2) & 15);
                     if ((lane & 3) == 0 && cq < m && cq > k) cpw[cq] += colsum;
-                    if (ok) cpw[i] += yrow;
                 }
             }
             __syncthreads();
@@ -279,31 +319,32 @@ __global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict
                 for (int t = 0; t < jj; ++t)
                     p -= a[(int64_t)(k0 + t) * ld + i] * q1[t] + W[(int64_t)t * ld + i] * q2[t];
                 p *= tk;
-                W[(int64_t)jj * ld + i] = p;
+                W[(int64_t)jj * ld + i] = (T)p;
                 pv += p * vs[i];
             }
             const double ptv = block_reduce(pv, red);
             const double acoef = -0.5 * tk * ptv;
             for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
                 const double w = W[(int64_t)jj * ld + i] + acoef * vs[i];
-                W[(int64_t)jj * ld + i] = w;
-                W[(int64_t)(2 * kNB + jj) * ld + i] = w;
+                W[(int64_t)jj * ld + i] = (T)w;
+                W[(int64_t)(2 * kNB + jj) * ld + i] = (T)w;
             }
-            for (int i = threadIdx.x; i <= k; i += kT) W[(int64_t)jj * ld + i] = 0.0;
+            for (int i = threadIdx.x; i <= k; i += kT) W[(int64_t)jj * ld + i] = T(0);
             __syncthreads();
         }
     }
 }
 
 // d, e of the last two rows once every panel and trailing update is done
-__global__ void k_sytrd_finish(const SubDesc *__restrict__ desc, const double *__restrict__ Aa,
+template <typename T>
+__global__ void k_sytrd_finish(const SubDesc *__restrict__ desc, const T *__restrict__ Aa,
                                double *__restrict__ d_all, double *__restrict__ e_all,
                                double *__restrict__ tau_all, int nsm) {
     const int si = blockIdx.x * blockDim.x + threadIdx.x;
     if (si >= nsm) return;
     const SubDesc ds = desc[si];
     const int m = ds.m, ld = ds.ld;
-    const double *a = Aa + ds.off;
+    const T *a = Aa + ds.off;
     double *dd = d_all + ds.mapoff, *ee = e_all + ds.mapoff, *tt = tau_all + ds.mapoff;
     if (m == 1) {
         dd[0] = a[0];
@@ -326,8 +367,11 @@ namespace ptb {
 // Panel by panel: the panel kernel reduces 32 columns of every submatrix, then
 // one batched GEMM applies the rank-64 update A(s:,s:) -= [W V] [V W]^T to the
 // lower triangle.
-int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
-                double *A, double *d, double *e, double *tau) {
+namespace {
+template <typename T>
+int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc, T *A,
+               double *d, double *e, double *tau) {
+    constexpr bool f32 = sizeof(T) == 4;
     ProfScope prof_(ctx, "sytrd");
     const int nsm = (int)desc.size();
     int max_m = 0;
@@ -341,7 +385,7 @@ int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
         pwoff[si] = pwtot;
         pwtot += (int64_t)desc[si].ld * 3 * kNB;
     }
-    auto *PW = static_cast<double *>(ws(ctx, "sytrd_pw", sizeof(double) * std::max<int64_t>(pwtot, 1)));
+    auto *PW = static_cast<T *>(ws(ctx, f32 ? "sytrd_pw32" : "sytrd_pw", sizeof(T) * std::max<int64_t>(pwtot, 1)));
     auto *dpwoff = static_cast<int64_t *>(ws(ctx, "sytrd_pwoff", sizeof(int64_t) * nsm));
     if (!PW || !dpwoff) return PTBGPU_E_NOMEM;
     PTB_HIP(ctx, hipMemcpyAsync(dpwoff, pwoff.data(), sizeof(int64_t) * nsm, hipMemcpyHostToDevice,
@@ -357,16 +401,19 @@ int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
             const int kb = std::min(kNB, x.m - 2 - k0);
             const int s0 = k0 + kb, n = x.m - s0;
             if (n <= 0) continue;
-            double *pw = PW + pwoff[si] + s0;
-            double *C = A + x.off + (int64_t)s0 * x.ld + s0;
-            // C -= [W V] [V W]^T (lower triangle)
-            g1.push_back({pw, pw + (int64_t)kNB * x.ld, C, n, n, 2 * kNB, x.ld, x.ld, x.ld, -1.0, 1.0,
+            T *pw = PW + pwoff[si] + s0;
+            T *C = A + x.off + (int64_t)s0 * x.ld + s0;
+            // C -= [W V] [V W]^T (lower triangle); float storage goes through
+            // the descriptor's pointers unchanged (the kernel reads them as T)
+            g1.push_back({reinterpret_cast<const double *>(pw),
+                          reinterpret_cast<const double *>(pw + (int64_t)kNB * x.ld),
+                          reinterpret_cast<double *>(C), n, n, 2 * kNB, x.ld, x.ld, x.ld, -1.0, 1.0,
                           1, 0});
         }
     }
     start[npanel] = (int)g1.size();
     const size_t gb = sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1);
-    auto *dg1 = static_cast<GemmDesc *>(ws(ctx, "sytrd_g1", gb));
+    auto *dg1 = static_cast<GemmDesc *>(ws(ctx, f32 ? "sytrd_g1f" : "sytrd_g1", gb));
     if (!dg1) return PTBGPU_E_NOMEM;
     if (!g1.empty())
         PTB_HIP(ctx, hipMemcpyAsync(dg1, g1.data(), sizeof(GemmDesc) * g1.size(),
@@ -374,17 +421,28 @@ int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
     for (int p = 0; p < npanel; ++p) {
         {
             ProfScope kprof_(ctx, "sytrd_panel");
-            hipLaunchKernelGGL(k_sytrd_panel, dim3(nsm), dim3(kT), shmem, ctx->stream, ddesc, A, PW,
+            hipLaunchKernelGGL(k_sytrd_panel<T>, dim3(nsm), dim3(kT), shmem, ctx->stream, ddesc, A, PW,
                                d, e, tau, dpwoff, p);
             PTB_CHECK_LAUNCH(ctx);
         }
         const int c = start[p + 1] - start[p];
-        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dg1 + start[p], c, max_m, max_m, "sytrd"));
+        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dg1 + start[p], c, max_m, max_m, "sytrd", f32));
     }
-    hipLaunchKernelGGL(k_sytrd_finish, dim3((nsm + 255) / 256), dim3(256), 0, ctx->stream, ddesc, A,
+    hipLaunchKernelGGL(k_sytrd_finish<T>, dim3((nsm + 255) / 256), dim3(256), 0, ctx->stream, ddesc, A,
                        d, e, tau, nsm);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
+}
+}  // namespace
+
+int sytrd_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                double *A, double *d, double *e, double *tau) {
+    return sytrd_impl(ctx, desc, ddesc, A, d, e, tau);
+}
+
+int sytrd_batch_f32(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                    float *A, double *d, double *e, double *tau) {
+    return sytrd_impl(ctx, desc, ddesc, A, d, e, tau);
 }
 
 }  // namespace ptb
