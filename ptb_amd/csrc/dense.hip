@@ -77,21 +77,23 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
         const int e = threadIdx.x + q * kT;
         if (!TB) { kk = e & 31; nn = e >> 5; } else { nn = e & 63; kk = e >> 6; }
     };
+    // element offsets are 32-bit (every operand block of the batch is below
+    // 2^32 elements from its descriptor pointer), which keeps the 16 addresses
+    // of a slice in half the registers of 64-bit pointers
+    const unsigned ulda = (unsigned)g.lda, uldb = (unsigned)g.ldb;
     auto load = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             int mm, kk;
             a_index(q, mm, kk);
             const int gm = m0 + mm, gk = k0 + kk;
-            ra[q] = (gm < g.M && gk < g.K)
-                        ? (double)(TA ? gA[(int64_t)gm * g.lda + gk] : gA[(int64_t)gk * g.lda + gm])
-                        : 0.0;
+            const unsigned oa = TA ? (unsigned)gm * ulda + (unsigned)gk : (unsigned)gk * ulda + (unsigned)gm;
+            ra[q] = (gm < g.M && gk < g.K) ? (double)gA[oa] : 0.0;
             int kb, nn;
             b_index(q, kb, nn);
             const int gn = n0 + nn, gkb = k0 + kb;
-            rb[q] = (gn < g.N && gkb < g.K)
-                        ? (double)(TB ? gB[(int64_t)gkb * g.ldb + gn] : gB[(int64_t)gn * g.ldb + gkb])
-                        : 0.0;
+            const unsigned ob = TB ? (unsigned)gkb * uldb + (unsigned)gn : (unsigned)gn * uldb + (unsigned)gkb;
+            rb[q] = (gn < g.N && gkb < g.K) ? (double)gB[ob] : 0.0;
         }
     };
     auto store = [&]() {
