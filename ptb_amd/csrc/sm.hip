@@ -25,7 +25,7 @@
 
 namespace {
 
-constexpr int kLeafMax = 64;
+constexpr int kLeafMax = 16;
 
 struct Timer {
     hipEvent_t ev[PTBGPU_T_NSLOT + 1];

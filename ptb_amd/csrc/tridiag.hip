@@ -97,8 +97,10 @@ __device__ __forceinline__ double butterfly8(const double (&p)[8], int lane) {
     return v;
 }
 
+// Float storage runs three workgroups per CU (the FP64 instance would spill
+// at that register budget and runs two).
 template <typename T>
-__global__ __launch_bounds__(kT, 2) void k_sytrd_panel(const SubDesc *__restrict__ desc,
+__global__ __launch_bounds__(kT, sizeof(T) == 4 ? 3 : 2) void k_sytrd_panel(const SubDesc *__restrict__ desc,
                                                     T *__restrict__ Aa,
                                                     T *__restrict__ Wa,
                                                     double *__restrict__ d_all,
