@@ -156,6 +156,29 @@ int ptbgpu_sm_density_dev(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const d
                           int64_t submatrix_mode, const ptbgpu_opts *opts,
                           ptbgpu_report *rep);
 
+/* Many independent molecules in one build (no counterpart in the reference,
+ * which builds one molecule per call): molecule i has ndim[i], nelref[i],
+ * packed H[i], S[i] (filtered) and P[i], natoms[i], xyz[i] (3 x natoms,
+ * bohr) and aoat[i] (ndim entries, 1..natoms); every molecule gets the same
+ * column grouping and options and its own chemical potential, exactly as
+ * sm_stupid_simple would give it alone.  The submatrices of all molecules share
+ * one batched eigensolver pass; the O(N^2) phases run per molecule.  reps
+ * (nmol entries, may be NULL) carry each molecule's mu, nel, passes and
+ * submatrix statistics; their phase times are the batch's.  ptbgpu_sm_spectra
+ * afterwards returns the submatrices of all molecules in order (nsm and sum_m
+ * summed over the batch).  The _dev variant takes device pointers. */
+int ptbgpu_sm_density_batch(ptbgpu_ctx *ctx, int64_t nmol, const int64_t *ndim,
+                            const int64_t *nelref, const double *const *H, const double *const *S,
+                            double *const *P, const int64_t *natoms, const double *const *xyz,
+                            const int64_t *const *aoat, int64_t submatrix_columns,
+                            int64_t submatrix_mode, const ptbgpu_opts *opts, ptbgpu_report *reps);
+int ptbgpu_sm_density_batch_dev(ptbgpu_ctx *ctx, int64_t nmol, const int64_t *ndim,
+                                const int64_t *nelref, const double *const *dH,
+                                const double *const *dS, double *const *dP, const int64_t *natoms,
+                                const double *const *xyz, const int64_t *const *aoat,
+                                int64_t submatrix_columns, int64_t submatrix_mode,
+                                const ptbgpu_opts *opts, ptbgpu_report *reps);
+
 /* Per-submatrix dimensions, eigenvalues and fe weights of the last build on
  * this context (the quantities the reference prints per submatrix and sums in
  * the mu bisection, source/sm.f90:298-310,340).  Arrays sized from the report:

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kT) void k_occ_compact(const SubDesc *__restrict__ 
     __shared__ int wtot[kT / PTB_WAVE];
     __shared__ int base;
     const SubDesc d = desc[blockIdx.x];
-    const double mu = mu_p[0];
+    const double mu = mu_p[(size_t)d.mol * kBisStride];
     const double *eig = eig_all + d.mapoff;
     const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
     if (threadIdx.x == 0) base = 0;
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kT) void k_refine_corr(const SubDesc *__restrict__ 
     const SubDesc d = desc[blockIdx.x];
     const int j = blockIdx.y;
     if (j >= nocc[blockIdx.x]) return;
-    const double mu = mu_p[0];
+    const double mu = mu_p[(size_t)d.mol * kBisStride];
     const double *eig = eig_all + d.mapoff;
     const int kj = oidx[d.mapoff + j];
     const double th = eig[kj];

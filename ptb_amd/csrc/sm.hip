@@ -334,10 +334,25 @@ int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
     return ptb::refine_scatter(ctx, ddesc, nsm, max_n, nocc, oidx, X, thn, Z, eig);
 }
 
-int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const double *dH,
-                      const double *dS, double *dP, int64_t natoms, const double *xyz,
-                      const int64_t *aoat, int64_t cols, int64_t mode, const ptbgpu_opts &opts,
-                      ptbgpu_report &rep) {
+// One molecule of a batch: its packed H, S (filtered) and P on the device and
+// the host arrays the column grouping reads.
+struct MolIn {
+    int64_t ndim, nelref, natoms;
+    const double *dH, *dS;
+    double *dP;
+    const double *xyz;
+    const int64_t *aoat;
+};
+
+// The density build for a batch of independent molecules.  The O(N^2) phases
+// (pattern, row unions, index maps, gather, bisection, scatter) run per
+// molecule on the molecule's own packed matrices; the eigendecomposition, fe
+// and back-transformation phases run once over the submatrices of every
+// molecule together.  Each submatrix carries its molecule index (SubDesc::mol),
+// which selects the chemical potential its occupations use.  A single
+// molecule is a batch of one.
+int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int64_t cols,
+                            int64_t mode, const ptbgpu_opts &opts, ptbgpu_report *reps) {
     hipStream_t st = ctx->stream;
     Timer tm;
     tm.init(st);
@@ -347,17 +362,36 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
                          "(the reference stops, source/sm.f90:249-251,312-314)", (long long)mode);
     if (opts.precision != PTBGPU_FP64 && opts.precision != PTBGPU_MIXED)
         return ptb::fail(ctx, PTBGPU_E_ARG, "unknown precision %lld", (long long)opts.precision);
-    // ---- coupling pattern (the k-means candidates are costed on it)
-    const int64_t words = (ndim + 31) / 32;
-    auto *bits = static_cast<uint32_t *>(ptb::ws(ctx, "sm_bits", sizeof(uint32_t) * ndim * words));
+    const int nmol = (int)mols.size();
+    if (nmol < 1) return ptb::fail(ctx, PTBGPU_E_ARG, "empty batch");
+    // ---- coupling pattern of every molecule (the k-means candidates are costed on it)
+    std::vector<int64_t> words(nmol), bitoff(nmol + 1, 0);
+    for (int i = 0; i < nmol; ++i) {
+        words[i] = (mols[i].ndim + 31) / 32;
+        bitoff[i + 1] = bitoff[i] + mols[i].ndim * words[i];
+    }
+    auto *bits = static_cast<uint32_t *>(ptb::ws(ctx, "sm_bits", sizeof(uint32_t) * bitoff[nmol]));
     if (!bits) return PTBGPU_E_NOMEM;
-    PTB_TRY(ptb::pattern_bits(ctx, dH, dS, ndim, words, bits));
-    // ---- combination
-    std::vector<int> gptr, gcols;
-    if (cols == 4) {
-        PTB_TRY(kmeans_groups(ctx, ndim, natoms, xyz, aoat, bits, words, opts, gptr, gcols));
-    } else if (column_groups(cols, ndim, aoat, gptr, gcols) != PTBGPU_OK) {
-        return ptb::fail(ctx, PTBGPU_E_MODE, "submatrix_columns %lld not implemented", (long long)cols);
+    for (int i = 0; i < nmol; ++i)
+        PTB_TRY(ptb::pattern_bits(ctx, mols[i].dH, mols[i].dS, mols[i].ndim, words[i],
+                                  bits + bitoff[i]));
+    // ---- combination: groups of every molecule; gcols holds molecule-local
+    // column indices, gptr positions in the concatenated gcols
+    std::vector<int> gptr{0}, gcols, sbeg(nmol + 1, 0);
+    for (int i = 0; i < nmol; ++i) {
+        const MolIn &mo = mols[i];
+        std::vector<int> gp, gc;
+        if (cols == 4) {
+            PTB_TRY(kmeans_groups(ctx, mo.ndim, mo.natoms, mo.xyz, mo.aoat, bits + bitoff[i],
+                                  words[i], opts, gp, gc));
+        } else if (column_groups(cols, mo.ndim, mo.aoat, gp, gc) != PTBGPU_OK) {
+            return ptb::fail(ctx, PTBGPU_E_MODE, "submatrix_columns %lld not implemented",
+                             (long long)cols);
+        }
+        const int base = (int)gcols.size();
+        for (size_t g = 1; g < gp.size(); ++g) gptr.push_back(base + gp[g]);
+        gcols.insert(gcols.end(), gc.begin(), gc.end());
+        sbeg[i + 1] = (int)gptr.size() - 1;
     }
     const int nsm = (int)gptr.size() - 1;
     int *dgptr, *dgcols;
@@ -365,40 +399,54 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     PTB_TRY(upload(ctx, "sm_gcols", gcols, &dgcols));
     tm.mark(1, st);
     // ---- stats: unions, dimensions
-    auto *uni = static_cast<uint32_t *>(ptb::ws(ctx, "sm_union", sizeof(uint32_t) * nsm * words));
+    std::vector<int64_t> unioff(nmol + 1, 0);
+    for (int i = 0; i < nmol; ++i) unioff[i + 1] = unioff[i] + (int64_t)(sbeg[i + 1] - sbeg[i]) * words[i];
+    auto *uni = static_cast<uint32_t *>(ptb::ws(ctx, "sm_union", sizeof(uint32_t) * unioff[nmol]));
     auto *dcount = static_cast<int *>(ptb::ws(ctx, "sm_count", sizeof(int) * nsm));
     if (!uni || !dcount) return PTBGPU_E_NOMEM;
-    PTB_TRY(ptb::group_union(ctx, bits, words, dgptr, dgcols, nsm, uni, dcount));
+    for (int i = 0; i < nmol; ++i)
+        PTB_TRY(ptb::group_union(ctx, bits + bitoff[i], words[i], dgptr + sbeg[i], dgcols,
+                                 sbeg[i + 1] - sbeg[i], uni + unioff[i], dcount + sbeg[i]));
     std::vector<int> mcount(nsm);
     PTB_HIP(ctx, hipMemcpyAsync(mcount.data(), dcount, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
     PTB_HIP(ctx, hipStreamSynchronize(st));
     std::vector<SubDesc> desc(nsm);
+    std::vector<int64_t> rbeg(nmol + 1, 0);
+    std::vector<int> mol_max_m(nmol, 0);
     int64_t arena = 0, rows = 0;
     int max_m = 0, max_own = 0;
-    double sum_m3 = 0.0;
-    for (int s = 0; s < nsm; ++s) {
-        SubDesc &d = desc[s];
-        d.m = mcount[s];
-        d.ld = (d.m + 15) & ~15;
-        d.off = arena;
-        d.mapoff = (int)rows;
-        d.gbeg = gptr[s];
-        d.gend = gptr[s + 1];
-        d.flags = 0;
-        arena += (int64_t)d.ld * d.m;
-        rows += d.m;
-        max_m = std::max(max_m, d.m);
-        max_own = std::max(max_own, d.gend - d.gbeg);
-        sum_m3 += (double)d.m * d.m * d.m;
+    for (int i = 0; i < nmol; ++i) {
+        rbeg[i] = rows;
+        for (int s = sbeg[i]; s < sbeg[i + 1]; ++s) {
+            SubDesc &d = desc[s];
+            d.m = mcount[s];
+            d.ld = (d.m + 15) & ~15;
+            d.off = arena;
+            d.mapoff = (int)rows;
+            d.gbeg = gptr[s];
+            d.gend = gptr[s + 1];
+            d.flags = 0;
+            d.mol = i;
+            arena += (int64_t)d.ld * d.m;
+            rows += d.m;
+            max_m = std::max(max_m, d.m);
+            mol_max_m[i] = std::max(mol_max_m[i], d.m);
+            max_own = std::max(max_own, d.gend - d.gbeg);
+        }
     }
+    rbeg[nmol] = rows;
     if (max_m > 2048)
         return ptb::fail(ctx, PTBGPU_E_ARG, "submatrix dimension %d exceeds 2048", max_m);
+    if (rows > INT32_MAX)
+        return ptb::fail(ctx, PTBGPU_E_ARG, "batch too large (%lld submatrix rows)", (long long)rows);
     SubDesc *ddesc;
     PTB_TRY(upload(ctx, "sm_desc", desc, &ddesc));
     auto *dmap = static_cast<int *>(ptb::ws(ctx, "sm_map", sizeof(int) * rows));
-    auto *doloc = static_cast<int *>(ptb::ws(ctx, "sm_oloc", sizeof(int) * ndim));
+    auto *doloc = static_cast<int *>(ptb::ws(ctx, "sm_oloc", sizeof(int) * std::max<size_t>(gcols.size(), 1)));
     if (!dmap || !doloc) return PTBGPU_E_NOMEM;
-    PTB_TRY(ptb::group_map(ctx, uni, words, ddesc, nsm, dmap, dgptr, dgcols, doloc));
+    for (int i = 0; i < nmol; ++i)
+        PTB_TRY(ptb::group_map(ctx, uni + unioff[i], words[i], ddesc + sbeg[i], sbeg[i + 1] - sbeg[i],
+                               dmap, dgptr + sbeg[i], dgcols, doloc));
     tm.mark(2, st);
     // ---- construction
     const size_t abytes = sizeof(double) * (size_t)arena;
@@ -407,14 +455,14 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     auto *Z = static_cast<double *>(ptb::ws(ctx, "sm_Z", abytes));
     auto *W = static_cast<double *>(ptb::ws(ctx, "sm_W", abytes));
     auto *X = static_cast<double *>(ptb::ws(ctx, "sm_X", abytes));
-    auto *vec = static_cast<double *>(ptb::ws(ctx, "sm_vec", sizeof(double) * rows * 10 + 4096));
+    auto *vec = static_cast<double *>(ptb::ws(ctx, "sm_vec", sizeof(double) * rows * 9));
+    auto *bis = static_cast<double *>(ptb::ws(ctx, "sm_bis", sizeof(double) * kBisStride * nmol));
     auto *flags = static_cast<int *>(ptb::ws(ctx, "sm_flags", sizeof(int) * nsm));
     auto *nocc = static_cast<int *>(ptb::ws(ctx, "sm_nocc", sizeof(int) * nsm));
-    if (!A || !B || !Z || !W || !X || !vec || !flags || !nocc) return PTBGPU_E_NOMEM;
+    if (!A || !B || !Z || !W || !X || !vec || !bis || !flags || !nocc) return PTBGPU_E_NOMEM;
     double *dvec = vec, *evec = vec + rows, *tau = vec + 2 * rows, *fe = vec + 3 * rows;
     double *rot = vec + 4 * rows;   // 4 * rows
     double *wocc = vec + 8 * rows;  // rows
-    double *bis = vec + 9 * rows;   // 4 + 200
     // owned-row projection buffers: XA = [L(O,:)^T, L^-1 e_O] (m x 2c), XB the
     // unit right-hand sides (m x c), F = Z^T Q^T XA (m x 2c)
     std::vector<int64_t> xoff(nsm), xboff(nsm);
@@ -443,9 +491,11 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     PTB_TRY(upload(ctx, "sm_xoff", xoff, &dxoff));
     PTB_TRY(upload(ctx, "sm_xboff", xboff, &dxboff));
     if (!XA || !XB || !F) return PTBGPU_E_NOMEM;
-    PTB_TRY(ptb::gather(ctx, dH, dS, ddesc, nsm, max_m, dmap, A, B));
+    for (int i = 0; i < nmol; ++i)
+        PTB_TRY(ptb::gather(ctx, mols[i].dH, mols[i].dS, ddesc + sbeg[i], sbeg[i + 1] - sbeg[i],
+                            mol_max_m[i], dmap, A, B));
     tm.mark(3, st);
-    // ---- eigendecomposition
+    // ---- eigendecomposition (every submatrix of every molecule)
     PTB_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int) * nsm, st));
     PTB_TRY(ptb::potrf_batch(ctx, desc, ddesc, B, flags));
     PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, A, Z, false, "h"));        // Z = L^-1 H
@@ -486,22 +536,30 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
         PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgf, nsm, max_m, 2 * max_own, "fe"));
     }
     PTB_HIP(ctx, hipStreamSynchronize(st));
-    int64_t not_spd = 0;
+    std::vector<int64_t> not_spd(nmol, 0);
+    int64_t not_spd_all = 0;
     for (int s = 0; s < nsm; ++s)
         if (hflags[s] & 1) {
-            ++not_spd;
+            ++not_spd[desc[s].mol];
+            ++not_spd_all;
             desc[s].flags |= 1;
         }
-    if (not_spd) {
+    if (not_spd_all) {
         if (opts.strict)
             return ptb::fail(ctx, PTBGPU_E_NOT_SPD, "%lld overlap submatrices are not positive definite",
-                             (long long)not_spd);
+                             (long long)not_spd_all);
         PTB_HIP(ctx, hipMemcpyAsync(ddesc, desc.data(), sizeof(SubDesc) * nsm, hipMemcpyHostToDevice, st));
     }
     PTB_TRY(ptb::fe_proj(ctx, ddesc, nsm, max_m, dxoff, F, fe));
     tm.mark(5, st);
-    // ---- mu bisection
-    PTB_TRY(ptb::bisect_mu(ctx, dvec, fe, rows, nelref, bis));
+    // ---- mu bisection, per molecule over its own states
+    auto bisect_all = [&]() -> int {
+        for (int i = 0; i < nmol; ++i)
+            PTB_TRY(ptb::bisect_mu(ctx, dvec + rbeg[i], fe + rbeg[i], rbeg[i + 1] - rbeg[i],
+                                   mols[i].nelref, bis + (size_t)i * kBisStride));
+        return PTBGPU_OK;
+    };
+    PTB_TRY(bisect_all());
     const int sweeps = !mixed || opts.refine_steps < 0 ? 0 : (int)std::max<int64_t>(1, opts.refine_steps);
     if (sweeps > 0) {
         // FP64 refinement of the states occupied at this mu, then fe and the
@@ -511,7 +569,7 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
                                     oidx, thn));
         PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgf, nsm, max_m, 2 * max_own, "fe"));
         PTB_TRY(ptb::fe_proj(ctx, ddesc, nsm, max_m, dxoff, F, fe));
-        PTB_TRY(ptb::bisect_mu(ctx, dvec, fe, rows, nelref, bis));
+        PTB_TRY(bisect_all());
     }
     tm.mark(6, st);
     // ---- final density: occupied columns only, Y_o = Q Z_o, U_o = L^-T Y_o
@@ -533,35 +591,45 @@ int sm_density_device(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     }
     PTB_TRY(ptb::apply_q(ctx, desc, qf, W, aoff, hnocc, false, "o"));        // W = Q Z_o
     PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, W, X, true, "u", &urhs));    // X = L^-T Q Z_o
-    PTB_HIP(ctx, hipMemsetAsync(dP, 0, sizeof(double) * (size_t)(ndim * (ndim + 1) / 2), st));
-    PTB_TRY(ptb::dmat_compact(ctx, ddesc, nsm, max_nocc, max_own, dgcols, doloc, dmap, wocc, nocc,
-                              X, dP));
+    for (int i = 0; i < nmol; ++i) {
+        const int64_t n = mols[i].ndim;
+        PTB_HIP(ctx, hipMemsetAsync(mols[i].dP, 0, sizeof(double) * (size_t)(n * (n + 1) / 2), st));
+        PTB_TRY(ptb::dmat_compact(ctx, ddesc + sbeg[i], sbeg[i + 1] - sbeg[i], max_nocc, max_own,
+                                  dgcols, doloc, dmap, wocc, nocc + sbeg[i], X, mols[i].dP));
+    }
     tm.mark(7, st);
-    std::vector<double> hb(4 + 200);
+    std::vector<double> hb((size_t)kBisStride * nmol);
     PTB_HIP(ctx, hipMemcpyAsync(hb.data(), bis, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, st));
     PTB_HIP(ctx, hipStreamSynchronize(st));
     tm.mark(8, st);
     PTB_HIP(ctx, hipStreamSynchronize(st));
-    rep.mu = hb[0];
-    rep.nel = hb[1];
-    rep.muiter = (int64_t)hb[2];
-    rep.nsm = nsm;
-    rep.sum_m = rows;
-    rep.max_m = max_m;
-    rep.sum_m3 = sum_m3;
-    rep.not_spd = not_spd;
     const int slots[] = {PTBGPU_T_COMBINATION, PTBGPU_T_STATS, PTBGPU_T_CONSTRUCTION, PTBGPU_T_EIGEN,
                          PTBGPU_T_FE, PTBGPU_T_MUITER, PTBGPU_T_FINAL};
-    for (int i = 0; i < 7; ++i) rep.t_phase[slots[i]] += 1e-3 * tm.between(i, i + 1);
-    if (opts.verbose) {
-        printf(" Stupid simple implementation of the submatrix method: %lld %lld\n", (long long)cols,
-               (long long)mode);
-        printf(" number of submatrices= %d\n average sm dim= %lld\n", nsm, (long long)(rows / nsm));
-        for (int i = 0; i < rep.muiter; ++i)
-            printf(" muiter %d mu %.17g %.17g of %lld\n", i + 1, hb[4 + 2 * i], hb[5 + 2 * i],
-                   (long long)nelref);
-        printf(" iteration of chemical potential converged\n");
-        fflush(stdout);
+    for (int i = 0; i < nmol; ++i) {
+        const double *b = hb.data() + (size_t)i * kBisStride;
+        ptbgpu_report &rep = reps[i];
+        rep.mu = b[0];
+        rep.nel = b[1];
+        rep.muiter = (int64_t)b[2];
+        rep.nsm = sbeg[i + 1] - sbeg[i];
+        rep.sum_m = rbeg[i + 1] - rbeg[i];
+        rep.max_m = mol_max_m[i];
+        rep.sum_m3 = 0.0;
+        for (int s = sbeg[i]; s < sbeg[i + 1]; ++s) rep.sum_m3 += (double)desc[s].m * desc[s].m * desc[s].m;
+        rep.not_spd = not_spd[i];
+        // the phases are shared by the batch: every report carries the batch's times
+        for (int t = 0; t < 7; ++t) rep.t_phase[slots[t]] += 1e-3 * tm.between(t, t + 1);
+        if (opts.verbose) {
+            printf(" Stupid simple implementation of the submatrix method: %lld %lld\n", (long long)cols,
+                   (long long)mode);
+            printf(" number of submatrices= %lld\n average sm dim= %lld\n", (long long)rep.nsm,
+                   (long long)(rep.sum_m / std::max<int64_t>(rep.nsm, 1)));
+            for (int k = 0; k < rep.muiter; ++k)
+                printf(" muiter %d mu %.17g %.17g of %lld\n", k + 1, b[4 + 2 * k], b[5 + 2 * k],
+                       (long long)mols[i].nelref);
+            printf(" iteration of chemical potential converged\n");
+            fflush(stdout);
+        }
     }
     return PTBGPU_OK;
 }
@@ -667,11 +735,93 @@ int ptbgpu_sm_density_dev(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const d
     ptbgpu_report r;
     memset(&r, 0, sizeof(r));
     auto t0 = std::chrono::steady_clock::now();
-    int st = sm_density_device(ctx, ndim, nelref, dH, dS, dP, n, xyz, aoat, submatrix_columns,
-                               submatrix_mode, o, r);
+    std::vector<MolIn> mols{{ndim, nelref, n, dH, dS, dP, xyz, aoat}};
+    int st = sm_density_batch_device(ctx, mols, submatrix_columns, submatrix_mode, o, &r);
     r.t_phase[PTBGPU_T_TOTAL] =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (rep) *rep = r;
+    return st;
+}
+
+int ptbgpu_sm_density_batch_dev(ptbgpu_ctx *ctx, int64_t nmol, const int64_t *ndim,
+                                const int64_t *nelref, const double *const *dH,
+                                const double *const *dS, double *const *dP, const int64_t *natoms,
+                                const double *const *xyz, const int64_t *const *aoat,
+                                int64_t submatrix_columns, int64_t submatrix_mode,
+                                const ptbgpu_opts *opts, ptbgpu_report *reps) {
+    if (!ctx) return PTBGPU_E_ARG;
+    if (nmol < 1 || !ndim || !nelref || !dH || !dS || !dP || !natoms || !xyz || !aoat)
+        return ptb::fail(ctx, PTBGPU_E_ARG, "bad arguments to sm_density_batch");
+    std::vector<MolIn> mols(nmol);
+    for (int64_t i = 0; i < nmol; ++i) {
+        PTB_TRY(check_sm_args(ctx, ndim[i], nelref[i], dH[i], dS[i], dP[i], natoms[i], aoat[i]));
+        mols[i] = {ndim[i], nelref[i], natoms[i], dH[i], dS[i], dP[i], xyz[i], aoat[i]};
+    }
+    PTB_HIP(ctx, hipSetDevice(ctx->device));
+    ptbgpu_opts o = opts ? *opts : ptbgpu_opts{PTBGPU_FP64, 0, 0, 0, nullptr, 0};
+    std::vector<ptbgpu_report> r(nmol);
+    memset(r.data(), 0, sizeof(ptbgpu_report) * nmol);
+    auto t0 = std::chrono::steady_clock::now();
+    int st = sm_density_batch_device(ctx, mols, submatrix_columns, submatrix_mode, o, r.data());
+    const double tot = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (auto &x : r) x.t_phase[PTBGPU_T_TOTAL] = tot;
+    if (reps) memcpy(reps, r.data(), sizeof(ptbgpu_report) * nmol);
+    return st;
+}
+
+int ptbgpu_sm_density_batch(ptbgpu_ctx *ctx, int64_t nmol, const int64_t *ndim,
+                            const int64_t *nelref, const double *const *H, const double *const *S,
+                            double *const *P, const int64_t *natoms, const double *const *xyz,
+                            const int64_t *const *aoat, int64_t submatrix_columns,
+                            int64_t submatrix_mode, const ptbgpu_opts *opts, ptbgpu_report *reps) {
+    if (!ctx) return PTBGPU_E_ARG;
+    if (nmol < 1 || !ndim || !H || !S || !P)
+        return ptb::fail(ctx, PTBGPU_E_ARG, "bad arguments to sm_density_batch");
+    PTB_HIP(ctx, hipSetDevice(ctx->device));
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<size_t> off(nmol + 1, 0);
+    for (int64_t i = 0; i < nmol; ++i) {
+        if (ndim[i] < 1 || ndim[i] > (int64_t)1 << 30 || !H[i] || !S[i] || !P[i])
+            return ptb::fail(ctx, PTBGPU_E_ARG, "bad arguments to sm_density_batch (molecule %lld)",
+                             (long long)i);
+        off[i + 1] = off[i] + (size_t)(ndim[i] * (ndim[i] + 1) / 2);
+    }
+    const size_t bytes = sizeof(double) * off[nmol];
+    auto *dH = static_cast<double *>(ptb::ws(ctx, "io_h", bytes));
+    auto *dS = static_cast<double *>(ptb::ws(ctx, "io_s", bytes));
+    auto *dP = static_cast<double *>(ptb::ws(ctx, "io_p", bytes));
+    if (!dH || !dS || !dP) return PTBGPU_E_NOMEM;
+    std::vector<const double *> ph(nmol), ps(nmol);
+    std::vector<double *> pp(nmol);
+    for (int64_t i = 0; i < nmol; ++i) {
+        const size_t b = sizeof(double) * (off[i + 1] - off[i]);
+        PTB_HIP(ctx, hipMemcpyAsync(dH + off[i], H[i], b, hipMemcpyHostToDevice, ctx->stream));
+        PTB_HIP(ctx, hipMemcpyAsync(dS + off[i], S[i], b, hipMemcpyHostToDevice, ctx->stream));
+        ph[i] = dH + off[i];
+        ps[i] = dS + off[i];
+        pp[i] = dP + off[i];
+    }
+    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double t_up = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::vector<ptbgpu_report> r(nmol);
+    int st = ptbgpu_sm_density_batch_dev(ctx, nmol, ndim, nelref, ph.data(), ps.data(), pp.data(),
+                                         natoms, xyz, aoat, submatrix_columns, submatrix_mode, opts,
+                                         r.data());
+    if (st == PTBGPU_OK) {
+        auto t1 = std::chrono::steady_clock::now();
+        for (int64_t i = 0; i < nmol; ++i)
+            PTB_HIP(ctx, hipMemcpyAsync(P[i], dP + off[i], sizeof(double) * (off[i + 1] - off[i]),
+                                        hipMemcpyDeviceToHost, ctx->stream));
+        PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const double t_down = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+        const double tot = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (auto &x : r) {
+            x.t_phase[PTBGPU_T_ALLOC] = t_up;
+            x.t_phase[PTBGPU_T_DEALLOC] = t_down;
+            x.t_phase[PTBGPU_T_TOTAL] = tot;
+        }
+    }
+    if (reps) memcpy(reps, r.data(), sizeof(ptbgpu_report) * nmol);
     return st;
 }
 

@@ -14,7 +14,13 @@ struct SubDesc {
     int gbeg;   // owned columns: gcols[gbeg..gend)
     int gend;
     int flags;  // bit 0: overlap block not positive definite
+    int mol;    // molecule of a batch (its chemical potential: mu[mol * kBisStride])
+    int pad;
 };
+
+// Per-molecule bisection record: [0] mu, [1] nel, [2] logged passes, [3]
+// converged flag, then (mu, nel) of up to 100 passes.
+constexpr int kBisStride = 4 + 200;
 
 // One product C = alpha op(A) op(B) + beta C of a variable-size batch.  With
 // uplo = 1 only the lower triangle of C (row >= column) is formed: tiles above

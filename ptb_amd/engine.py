@@ -107,6 +107,11 @@ def _lib():
     for name in ("ptbgpu_sm_density", "ptbgpu_sm_density_dev"):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = sm_args
+    batch_args = [VP, I64, VP, VP, VP, VP, VP, VP, VP, VP, I64, I64, ctypes.POINTER(Opts),
+                  VP]
+    for name in ("ptbgpu_sm_density_batch", "ptbgpu_sm_density_batch_dev"):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = batch_args
     if hasattr(lib, "ptbgpu_sm_spectra"):
         lib.ptbgpu_sm_spectra.argtypes = [VP, I64, I64, VP, VP, VP]
     for name in ("ptbgpu_get_nel", "ptbgpu_band_energy", "ptbgpu_idempotency"):
@@ -280,6 +285,43 @@ class Engine:
             self.ctx, ndim, nelref, _ptr(dH), _ptr(dS), _ptr(dP), n, _ptr(x), _ptr(a),
             submatrix_columns, submatrix_mode, ctypes.byref(opts), ctypes.byref(rep)))
         return rep.as_dict()
+
+    def sm_stupid_simple_batch(self, mols, submatrix_columns, submatrix_mode, precision="fp64",
+                               strict=False, verbose=False, refine_steps=0, kmeans_stream=None,
+                               device=False) -> list:
+        """Many independent molecules in one build (ptbgpu_sm_density_batch):
+        ``mols`` is a list of (ndim, nelref, H, S, P, n, xyz, aoat) with packed
+        numpy arrays (``device=False``; P is overwritten) or torch CUDA tensors
+        (``device=True``).  Returns one report dict per molecule."""
+        k = len(mols)
+        ndim = np.array([m[0] for m in mols], dtype=np.int64)
+        nel = np.array([m[1] for m in mols], dtype=np.int64)
+        nat = np.array([m[5] for m in mols], dtype=np.int64)
+        keep = []
+        ptrs = {name: (VP * k)() for name in ("H", "S", "P", "xyz", "aoat")}
+        for i, (nd, _, H, S, P, n, xyz, aoat) in enumerate(mols):
+            npk = nd * (nd + 1) // 2
+            if not device:
+                _f64(H, npk, "H")
+                _f64(S, npk, "S")
+                _f64(P, npk, "P")
+            x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(n, 3)
+            a = np.ascontiguousarray(aoat, dtype=np.int64)
+            if a.size != nd:
+                raise ValueError("aoat must have ndim entries")
+            keep += [x, a]
+            ptrs["H"][i] = _ptr(H).value
+            ptrs["S"][i] = _ptr(S).value
+            ptrs["P"][i] = _ptr(P).value
+            ptrs["xyz"][i] = x.ctypes.data
+            ptrs["aoat"][i] = a.ctypes.data
+        reps = (Report * k)()
+        opts = self._opts(precision, strict, verbose, refine_steps, kmeans_stream)
+        fn = self.lib.ptbgpu_sm_density_batch_dev if device else self.lib.ptbgpu_sm_density_batch
+        self._check(fn(self.ctx, k, _ptr(ndim), _ptr(nel), ptrs["H"], ptrs["S"], ptrs["P"],
+                       _ptr(nat), ptrs["xyz"], ptrs["aoat"], submatrix_columns, submatrix_mode,
+                       ctypes.byref(opts), reps))
+        return [r.as_dict() for r in reps]
 
     def sm_spectra(self, nsm: int, sum_m: int):
         """(m per submatrix, eigenvalues, fe) of the last build, split per submatrix."""
