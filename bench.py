@@ -77,15 +77,17 @@ def roofline_model(name, ms, ndim, mixed=False):
     return None
 
 
-def traffic_per_launch(kernel, workload):
-    """Scaled HBM bytes per dispatch of `kernel` from the committed rocprofv3
-    --pmc summary of this workload (FETCH_SIZE + WRITE_SIZE, each scaled by the
-    counter's reading for a plain copy of known size), or None."""
+def traffic_per_launch(kernel, workload, mixed=False):
+    """Scaled HBM bytes per dispatch of `kernel` (its float-storage instance for
+    the mixed-precision solve) from the committed rocprofv3 --pmc summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, each scaled by the counter's
+    reading for a plain copy of known size), or None."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_hbm_traffic.md")
+    key = "bytes_per_launch_float=" if mixed else "bytes_per_launch="
     try:
         with open(path) as fh:
             for line in fh:
-                if line.startswith("bytes_per_launch="):
+                if line.startswith(key):
                     return float(line.split("=", 1)[1])
     except OSError:
         return None
@@ -321,7 +323,7 @@ def main():
             achieved = units / (ms * 1e-3) / 1e12
             peak = FP64_PEAK_TFS
         kname = "k_" + name
-        traffic = traffic_per_launch(kname, args.config + ("_mixed" if main_prec == "mixed" else ""))
+        traffic = traffic_per_launch(kname, args.config, mixed=main_prec == "mixed")
         roof = {"kernel": kname, "bound": bound, "achieved": round(achieved, 2), "peak": peak,
                 "unit": unit, "frac": round(achieved / peak, 4),
                 "traffic": None if traffic is None else round(traffic, 0),
