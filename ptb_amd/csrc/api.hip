@@ -19,6 +19,7 @@ int fail(ptbgpu_ctx *ctx, int status, const char *fmt, ...) {
 void *ws(ptbgpu_ctx *ctx, const char *name, size_t bytes) {
     DevBuf &b = ctx->bufs[name];
     if (b.bytes >= bytes && b.ptr) return b.ptr;
+    ctx->shadow.erase(name);  // new memory: no table is known to be in it
     if (b.ptr) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipFree(b.ptr);
@@ -34,6 +35,23 @@ void *ws(ptbgpu_ctx *ctx, const char *name, size_t bytes) {
     }
     b.bytes = want;
     return b.ptr;
+}
+
+void *upload_table(ptbgpu_ctx *ctx, const char *name, const void *src, size_t bytes) {
+    void *d = ws(ctx, name, bytes ? bytes : 1);
+    if (!d || bytes == 0) return d;
+    auto it = ctx->shadow.find(name);
+    if (it != ctx->shadow.end() && it->second.size() == bytes &&
+        memcmp(it->second.data(), src, bytes) == 0)
+        return d;
+    if (hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        ctx->shadow.erase(name);
+        fail(ctx, PTBGPU_E_HIP, "upload of '%s' failed: %s", name, hipGetErrorString(hipGetLastError()));
+        return nullptr;
+    }
+    const char *c = static_cast<const char *>(src);
+    ctx->shadow[name].assign(c, c + bytes);
+    return d;
 }
 
 void *hws(ptbgpu_ctx *ctx, const char *name, size_t bytes) {

@@ -31,6 +31,7 @@ struct ptbgpu_ctx {
     std::map<std::string, ProfRecord> prof;  // kernel name -> bracketing events
     std::map<std::string, DevBuf> bufs;      // named device workspace, grows on demand
     std::map<std::string, DevBuf> host_bufs; // pinned host staging
+    std::map<std::string, std::vector<char>> shadow;  // last upload_table contents per buffer
     int num_cus = 0;
 };
 
@@ -41,6 +42,11 @@ int fail(ptbgpu_ctx *ctx, int status, const char *fmt, ...);
 
 // Named device workspace of at least `bytes` (contents undefined after growth).
 void *ws(ptbgpu_ctx *ctx, const char *name, size_t bytes);
+// Named device table of `bytes` filled from host memory: the copy is skipped
+// when the buffer still holds exactly these bytes from the previous call
+// (descriptor tables of repeated builds with the same structure).  Returns the
+// device pointer, or nullptr with the error recorded.
+void *upload_table(ptbgpu_ctx *ctx, const char *name, const void *src, size_t bytes);
 // Named pinned host buffer of at least `bytes`.
 void *hws(ptbgpu_ctx *ctx, const char *name, size_t bytes);
 

@@ -460,10 +460,8 @@ int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
         max_m = std::max(max_m, desc[s].m);
     }
     auto *dinv = static_cast<double *>(ws(ctx, "trsm_dinv", sizeof(double) * tot));
-    auto *ddoff = static_cast<int64_t *>(ws(ctx, "trsm_doff", sizeof(int64_t) * nsm));
+    auto *ddoff = static_cast<int64_t *>(upload_table(ctx, "trsm_doff", doff.data(), sizeof(int64_t) * nsm));
     if (!dinv || !ddoff) return PTBGPU_E_NOMEM;
-    PTB_HIP(ctx, hipMemcpyAsync(ddoff, doff.data(), sizeof(int64_t) * nsm, hipMemcpyHostToDevice,
-                                ctx->stream));
     {
         ProfScope kprof_(ctx, "diag_inv");
         hipLaunchKernelGGL(k_diag_inv, dim3(nsm, max_nb), dim3(64), 0, ctx->stream, ddesc, ddoff, L,
@@ -502,14 +500,9 @@ int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
     s1[max_nb] = (int)g1.size();
     s2[max_nb] = (int)g2.size();
     std::string n1 = std::string("trsm_g1_") + tag, n2 = std::string("trsm_g2_") + tag;
-    auto *dg1 = static_cast<GemmDesc *>(ws(ctx, n1.c_str(), sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1)));
-    auto *dg2 = static_cast<GemmDesc *>(ws(ctx, n2.c_str(), sizeof(GemmDesc) * std::max<size_t>(g2.size(), 1)));
+    auto *dg1 = static_cast<GemmDesc *>(upload_table(ctx, n1.c_str(), g1.data(), sizeof(GemmDesc) * g1.size()));
+    auto *dg2 = static_cast<GemmDesc *>(upload_table(ctx, n2.c_str(), g2.data(), sizeof(GemmDesc) * g2.size()));
     if (!dg1 || !dg2) return PTBGPU_E_NOMEM;
-    if (!g1.empty())
-        PTB_HIP(ctx, hipMemcpyAsync(dg1, g1.data(), sizeof(GemmDesc) * g1.size(),
-                                    hipMemcpyHostToDevice, ctx->stream));
-    PTB_HIP(ctx, hipMemcpyAsync(dg2, g2.data(), sizeof(GemmDesc) * g2.size(), hipMemcpyHostToDevice,
-                                ctx->stream));
     for (int q = 0; q < max_nb; ++q) {
         const int c1 = s1[q + 1] - s1[q], c2 = s2[q + 1] - s2[q];
         if (c1 > 0) PTB_TRY(gemm_vbatch(ctx, trans, false, dg1 + s1[q], c1, 64, max_m, "trsm"));
@@ -539,11 +532,8 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
         }
     }
     start[npanel] = (int)g.size();
-    auto *dg = static_cast<GemmDesc *>(ws(ctx, "potrf_g", sizeof(GemmDesc) * std::max<size_t>(g.size(), 1)));
+    auto *dg = static_cast<GemmDesc *>(upload_table(ctx, "potrf_g", g.data(), sizeof(GemmDesc) * g.size()));
     if (!dg) return PTBGPU_E_NOMEM;
-    if (!g.empty())
-        PTB_HIP(ctx, hipMemcpyAsync(dg, g.data(), sizeof(GemmDesc) * g.size(), hipMemcpyHostToDevice,
-                                    ctx->stream));
     for (int p = 0; p < npanel; ++p) {
         {
             ProfScope kprof_(ctx, "potrf_panel");
@@ -608,10 +598,8 @@ int larft_all(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
     qf.V = V;
     qf.maxblk = maxblk;
     qf.T = static_cast<double *>(ws(ctx, "q_T", sizeof(double) * std::max<int64_t>(tot, 1)));
-    auto *dtoff = static_cast<int64_t *>(ws(ctx, "q_toff", sizeof(int64_t) * nsm));
+    auto *dtoff = static_cast<int64_t *>(upload_table(ctx, "q_toff", qf.toff.data(), sizeof(int64_t) * nsm));
     if (!qf.T || !dtoff) return PTBGPU_E_NOMEM;
-    PTB_HIP(ctx, hipMemcpyAsync(dtoff, qf.toff.data(), sizeof(int64_t) * nsm, hipMemcpyHostToDevice,
-                                ctx->stream));
     if (maxblk > 0) {
         hipLaunchKernelGGL(k_larft_all, dim3(nsm, maxblk), dim3(kT), 0, ctx->stream, ddesc, V, tau,
                            dtoff, qf.T);
@@ -669,15 +657,12 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
     }
     start[qf.maxblk] = (int)g1.size();
     const std::string t(tag);
-    GemmDesc *dg1 = static_cast<GemmDesc *>(ws(ctx, ("q_g1_" + t).c_str(), sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1)));
-    GemmDesc *dg2 = static_cast<GemmDesc *>(ws(ctx, ("q_g2_" + t).c_str(), sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1)));
-    GemmDesc *dg3 = static_cast<GemmDesc *>(ws(ctx, ("q_g3_" + t).c_str(), sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1)));
-    if (!dg1 || !dg2 || !dg3) return PTBGPU_E_NOMEM;
     if (g1.empty()) return PTBGPU_OK;
     const size_t gb = sizeof(GemmDesc) * g1.size();
-    PTB_HIP(ctx, hipMemcpyAsync(dg1, g1.data(), gb, hipMemcpyHostToDevice, ctx->stream));
-    PTB_HIP(ctx, hipMemcpyAsync(dg2, g2.data(), gb, hipMemcpyHostToDevice, ctx->stream));
-    PTB_HIP(ctx, hipMemcpyAsync(dg3, g3.data(), gb, hipMemcpyHostToDevice, ctx->stream));
+    GemmDesc *dg1 = static_cast<GemmDesc *>(upload_table(ctx, ("q_g1_" + t).c_str(), g1.data(), gb));
+    GemmDesc *dg2 = static_cast<GemmDesc *>(upload_table(ctx, ("q_g2_" + t).c_str(), g2.data(), gb));
+    GemmDesc *dg3 = static_cast<GemmDesc *>(upload_table(ctx, ("q_g3_" + t).c_str(), g3.data(), gb));
+    if (!dg1 || !dg2 || !dg3) return PTBGPU_E_NOMEM;
     for (int q = 0; q < qf.maxblk; ++q) {
         const int c = start[q + 1] - start[q];
         if (c == 0) continue;

@@ -8,9 +8,9 @@
 // full back-transformation is needed for fe; owned rows accumulate in the
 // reference's order.
 //
-// The bisection runs in one 1024-thread workgroup over every (eigenvalue, fe)
-// pair of the batch with fixed-order reductions, so it is deterministic; it
-// reproduces the reference's decisions: mu = (mua+mub)/2 from [-10, 10],
+// The bisection runs per molecule of a batch, every pass one launch over all
+// (eigenvalue, fe) pairs (per-block partials, then a fixed-order sum), so it is
+// deterministic; it reproduces the reference's decisions: mu = (mua+mub)/2 from [-10, 10],
 // occupations 2/1/0 (tie e == mu -> 1), stop on |mua-mub| < 1e-3 or
 // |nel-nelref| < 1e-5 with both literals rounded to single precision, then one
 // more pass at the same mu.
@@ -75,47 +75,90 @@ __device__ __forceinline__ double occ_of(double e, double mu) {
     return e < mu ? 2.0 : (e == mu ? 1.0 : 0.0);
 }
 
-constexpr int kBT = 1024;
+// Bisection state of one molecule.
+struct BisState {
+    double mua, mub, mu;
+    int iter, logged, finaliter, done;
+};
 
-// out: [0] mu, [1] nel at mu, [2] number of logged passes, then the trace
-// (mu, nel) of every logged pass from out[4]
-__global__ __launch_bounds__(kBT) void k_bisect(const double *__restrict__ e,
-                                                const double *__restrict__ fe, int64_t total,
-                                                double nelref, double *__restrict__ out) {
-    __shared__ double red[kBT / PTB_WAVE];
-    const float tol_mu = 1e-3f, tol_nel = 1e-5f;  // REAL(4) literals of the reference
-    double mua = -10.0, mub = 10.0, mu = 0.0, nel = 0.0;
-    bool finaliter = false;
-    int logged = 0;
-    for (int muiter = 1; muiter <= 100; ++muiter) {
-        mu = (mua + mub) / 2.0;
-        double part = 0.0;
-        for (int64_t i = threadIdx.x; i < total; i += kBT) part += fe[i] * occ_of(e[i], mu);
-        part = wave_sum(part);
-        const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
-        __syncthreads();
-        if (lane == 0) red[wid] = part;
-        __syncthreads();
-        nel = 0.0;
-        for (int w = 0; w < kBT / PTB_WAVE; ++w) nel += red[w];
-        if (finaliter) break;
-        if (threadIdx.x == 0) {
-            out[4 + 2 * logged] = mu;
-            out[5 + 2 * logged] = nel;
-        }
-        ++logged;
-        if (fabs(mua - mub) < (double)tol_mu || fabs(nel - nelref) < (double)tol_nel) finaliter = true;
-        if (!finaliter) {
-            if (nel > nelref) mub = mu;
-            else mua = mu;
-        }
-    }
+__global__ void k_bisect_init(BisState *__restrict__ st, int nmol) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nmol) st[i] = BisState{-10.0, 10.0, 0.0, 0, 0, 0, 0};
+}
+
+// One pass, part 1: partial sums of fe(k) occ(e_k, mu) over chunk b of
+// molecule i's states.  grid: (nblk, nmol)
+__global__ __launch_bounds__(kT) void k_nel_partial(const double *__restrict__ e,
+                                                    const double *__restrict__ fe,
+                                                    const int64_t *__restrict__ rbeg,
+                                                    const BisState *__restrict__ st, int nblk,
+                                                    double *__restrict__ partial) {
+    const int i = blockIdx.y;
+    if (st[i].done) return;
+    const double mu = st[i].mu;
+    const int64_t lo = rbeg[i], n = rbeg[i + 1] - lo;
+    const int64_t chunk = (n + nblk - 1) / nblk;
+    const int64_t b0 = lo + (int64_t)blockIdx.x * chunk, b1 = min(lo + n, b0 + chunk);
+    double part = 0.0;
+    for (int64_t k = b0 + threadIdx.x; k < b1; k += kT) part += fe[k] * occ_of(e[k], mu);
+    __shared__ double red[kT / PTB_WAVE];
+    part = wave_sum(part);
+    if (threadIdx.x % PTB_WAVE == 0) red[threadIdx.x / PTB_WAVE] = part;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        out[0] = mu;
-        out[1] = nel;
-        out[2] = (double)logged;
-        out[3] = finaliter ? 1.0 : 0.0;
+        double t = 0.0;
+        for (int w = 0; w < kT / PTB_WAVE; ++w) t += red[w];
+        partial[(int64_t)i * nblk + blockIdx.x] = t;
     }
+}
+
+// One pass, part 2 (one thread per molecule): nel = the partials summed in
+// fixed order, then the reference's step (source/sm.f90:365-383): a pass after
+// convergence is the final one (its nel is reported); otherwise log (mu, nel),
+// test |mua-mub| < 1e-3 or |nel-nelref| < 1e-5 (REAL(4) literals), move the
+// bracket and bisect.  At most 100 passes, as the reference's loop.
+// out (per molecule, kBisStride): [0] mu, [1] nel, [2] logged passes,
+// [3] converged, then (mu, nel) of every logged pass.
+__global__ void k_bisect_step(BisState *__restrict__ st, const double *__restrict__ partial,
+                              int nblk, const int64_t *__restrict__ nelref, int nmol,
+                              double *__restrict__ out_all, int *__restrict__ ndone) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nmol) return;
+    BisState s = st[i];
+    if (s.done) return;
+    const float tol_mu = 1e-3f, tol_nel = 1e-5f;  // REAL(4) literals of the reference
+    double *out = out_all + (size_t)i * kBisStride;
+    double nel = 0.0;
+    for (int b = 0; b < nblk; ++b) nel += partial[(int64_t)i * nblk + b];
+    const double ref = (double)nelref[i];
+    s.iter += 1;
+    bool finish = false;
+    if (s.finaliter) {
+        finish = true;
+    } else {
+        out[4 + 2 * s.logged] = s.mu;
+        out[5 + 2 * s.logged] = nel;
+        s.logged += 1;
+        if (fabs(s.mua - s.mub) < (double)tol_mu || fabs(nel - ref) < (double)tol_nel) {
+            s.finaliter = 1;
+        } else if (nel > ref) {
+            s.mub = s.mu;
+        } else {
+            s.mua = s.mu;
+        }
+        finish = s.iter >= 100;
+    }
+    if (finish) {
+        out[0] = s.mu;
+        out[1] = nel;
+        out[2] = (double)s.logged;
+        out[3] = s.finaliter ? 1.0 : 0.0;
+        s.done = 1;
+        atomicAdd(ndone, 1);
+    } else {
+        s.mu = (s.mua + s.mub) / 2.0;
+    }
+    st[i] = s;
 }
 
 // Occupied states of each submatrix at the final mu, in ascending state
@@ -329,12 +372,35 @@ int fe_proj(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const int
     return PTBGPU_OK;
 }
 
-int bisect_mu(ptbgpu_ctx *ctx, const double *e, const double *fe, int64_t total, int64_t nelref,
-              double *dout) {
+int bisect_batch(ptbgpu_ctx *ctx, const double *e, const double *fe, const int64_t *drbeg,
+                 const int64_t *dnelref, int nmol, int64_t max_rows, double *dout) {
     ProfScope prof_(ctx, "bisect");
-    hipLaunchKernelGGL(k_bisect, dim3(1), dim3(kBT), 0, ctx->stream, e, fe, total, (double)nelref,
-                       dout);
+    auto *st = static_cast<BisState *>(ws(ctx, "bis_state", sizeof(BisState) * nmol));
+    auto *ndone = static_cast<int *>(ws(ctx, "bis_done", sizeof(int)));
+    const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(256, (max_rows + 4095) / 4096));
+    auto *partial = static_cast<double *>(ws(ctx, "bis_partial", sizeof(double) * nblk * nmol));
+    if (!st || !ndone || !partial) return PTBGPU_E_NOMEM;
+    const int tb = (nmol + 63) / 64;
+    hipLaunchKernelGGL(k_bisect_init, dim3(tb), dim3(64), 0, ctx->stream, st, nmol);
     PTB_CHECK_LAUNCH(ctx);
+    PTB_HIP(ctx, hipMemsetAsync(ndone, 0, sizeof(int), ctx->stream));
+    // passes in chunks (a pass after a molecule is done returns at once);
+    // 101 passes cover the reference's 100 and the final one
+    int passes = 0, chunk = 24, done = 0;
+    while (passes < 101 && done < nmol) {
+        const int n = std::min(chunk, 101 - passes);
+        for (int p = 0; p < n; ++p) {
+            hipLaunchKernelGGL(k_nel_partial, dim3(nblk, nmol), dim3(kT), 0, ctx->stream, e, fe,
+                               drbeg, st, nblk, partial);
+            hipLaunchKernelGGL(k_bisect_step, dim3(tb), dim3(64), 0, ctx->stream, st, partial,
+                               nblk, dnelref, nmol, dout, ndone);
+        }
+        PTB_CHECK_LAUNCH(ctx);
+        passes += n;
+        PTB_HIP(ctx, hipMemcpyAsync(&done, ndone, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        chunk = 8;
+    }
     return PTBGPU_OK;
 }
 

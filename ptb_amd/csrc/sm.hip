@@ -205,12 +205,8 @@ void build_tree(int s, int lo, int hi, std::vector<DcNode> &leaves,
 
 template <typename T>
 int upload(ptbgpu_ctx *ctx, const char *name, const std::vector<T> &v, T **out) {
-    *out = static_cast<T *>(ptb::ws(ctx, name, sizeof(T) * std::max<size_t>(v.size(), 1)));
-    if (!*out) return PTBGPU_E_NOMEM;
-    if (!v.empty())
-        PTB_HIP(ctx, hipMemcpyAsync(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice,
-                                    ctx->stream));
-    return PTBGPU_OK;
+    *out = static_cast<T *>(ptb::upload_table(ctx, name, v.data(), sizeof(T) * v.size()));
+    return *out ? PTBGPU_OK : PTBGPU_E_NOMEM;
 }
 
 // k-means column groups (submatrix_columns 4, source/sm.f90:129-157): for
@@ -548,16 +544,22 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         if (opts.strict)
             return ptb::fail(ctx, PTBGPU_E_NOT_SPD, "%lld overlap submatrices are not positive definite",
                              (long long)not_spd_all);
-        PTB_HIP(ctx, hipMemcpyAsync(ddesc, desc.data(), sizeof(SubDesc) * nsm, hipMemcpyHostToDevice, st));
+        PTB_TRY(upload(ctx, "sm_desc", desc, &ddesc));
     }
     PTB_TRY(ptb::fe_proj(ctx, ddesc, nsm, max_m, dxoff, F, fe));
     tm.mark(5, st);
     // ---- mu bisection, per molecule over its own states
+    int64_t max_rows = 0;
+    std::vector<int64_t> nelrefs(nmol);
+    for (int i = 0; i < nmol; ++i) {
+        max_rows = std::max(max_rows, rbeg[i + 1] - rbeg[i]);
+        nelrefs[i] = mols[i].nelref;
+    }
+    int64_t *drbeg, *dnelref;
+    PTB_TRY(upload(ctx, "sm_rbeg", rbeg, &drbeg));
+    PTB_TRY(upload(ctx, "sm_nelref", nelrefs, &dnelref));
     auto bisect_all = [&]() -> int {
-        for (int i = 0; i < nmol; ++i)
-            PTB_TRY(ptb::bisect_mu(ctx, dvec + rbeg[i], fe + rbeg[i], rbeg[i + 1] - rbeg[i],
-                                   mols[i].nelref, bis + (size_t)i * kBisStride));
-        return PTBGPU_OK;
+        return ptb::bisect_batch(ctx, dvec, fe, drbeg, dnelref, nmol, max_rows, bis);
     };
     PTB_TRY(bisect_all());
     const int sweeps = !mixed || opts.refine_steps < 0 ? 0 : (int)std::max<int64_t>(1, opts.refine_steps);

@@ -132,8 +132,11 @@ int xall_build(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_own, cons
                double *XB);
 int fe_proj(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const int64_t *dxoff,
             const double *F, double *fe);
-int bisect_mu(ptbgpu_ctx *ctx, const double *e, const double *fe, int64_t total, int64_t nelref,
-              double *dout);
+// mu bisection of nmol molecules at once: molecule i owns the states
+// [rbeg[i], rbeg[i+1]) of e and fe (device arrays); its record goes to
+// dout + i * kBisStride
+int bisect_batch(ptbgpu_ctx *ctx, const double *e, const double *fe, const int64_t *drbeg,
+                 const int64_t *dnelref, int nmol, int64_t max_rows, double *dout);
 int occ_compact(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *eig,
                 const double *dmu, const double *Z, double *Zo, double *wocc, int *nocc,
                 int *oidx = nullptr);

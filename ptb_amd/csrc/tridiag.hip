@@ -388,10 +388,8 @@ int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc 
         pwtot += (int64_t)desc[si].ld * 3 * kNB;
     }
     auto *PW = static_cast<T *>(ws(ctx, f32 ? "sytrd_pw32" : "sytrd_pw", sizeof(T) * std::max<int64_t>(pwtot, 1)));
-    auto *dpwoff = static_cast<int64_t *>(ws(ctx, "sytrd_pwoff", sizeof(int64_t) * nsm));
+    auto *dpwoff = static_cast<int64_t *>(upload_table(ctx, "sytrd_pwoff", pwoff.data(), sizeof(int64_t) * nsm));
     if (!PW || !dpwoff) return PTBGPU_E_NOMEM;
-    PTB_HIP(ctx, hipMemcpyAsync(dpwoff, pwoff.data(), sizeof(int64_t) * nsm, hipMemcpyHostToDevice,
-                                ctx->stream));
     std::vector<GemmDesc> g1;
     std::vector<int> start(npanel + 1, 0);
     for (int p = 0; p < npanel; ++p) {
@@ -414,12 +412,9 @@ int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc 
         }
     }
     start[npanel] = (int)g1.size();
-    const size_t gb = sizeof(GemmDesc) * std::max<size_t>(g1.size(), 1);
-    auto *dg1 = static_cast<GemmDesc *>(ws(ctx, f32 ? "sytrd_g1f" : "sytrd_g1", gb));
+    auto *dg1 = static_cast<GemmDesc *>(upload_table(ctx, f32 ? "sytrd_g1f" : "sytrd_g1", g1.data(),
+                                                     sizeof(GemmDesc) * g1.size()));
     if (!dg1) return PTBGPU_E_NOMEM;
-    if (!g1.empty())
-        PTB_HIP(ctx, hipMemcpyAsync(dg1, g1.data(), sizeof(GemmDesc) * g1.size(),
-                                    hipMemcpyHostToDevice, ctx->stream));
     for (int p = 0; p < npanel; ++p) {
         {
             ProfScope kprof_(ctx, "sytrd_panel");
