@@ -511,6 +511,10 @@ int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
     return PTBGPU_OK;
 }
 
+// 64-column blocks: factor panel 2b, apply its rank-32 update to the 32
+// columns of panel 2b+1 only, factor panel 2b+1, then one rank-64 update of the
+// trailing matrix (lower triangle), so the trailing matrix is read and
+// written once per 64 columns.
 int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                 double *A, int *flags) {
     ProfScope prof_(ctx, "potrf");
@@ -518,30 +522,48 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
     int max_m = 0;
     for (const auto &x : desc) max_m = std::max(max_m, x.m);
     const int npanel = (max_m + 31) / 32;
-    std::vector<GemmDesc> g;
-    std::vector<int> start(npanel + 1, 0);
-    for (int p = 0; p < npanel; ++p) {
-        start[p] = (int)g.size();
+    const int nblk = (max_m + 63) / 64;
+    std::vector<GemmDesc> gs, gb;
+    std::vector<int> ss(nblk + 1, 0), sb(nblk + 1, 0);
+    for (int b = 0; b < nblk; ++b) {
+        ss[b] = (int)gs.size();
+        sb[b] = (int)gb.size();
         for (int si = 0; si < nsm; ++si) {
             const SubDesc &x = desc[si];
-            const int k0 = p * 32, s0 = k0 + 32, n = x.m - s0;
-            if (n <= 0) continue;
             double *a = A + x.off;
-            double *P = a + (int64_t)k0 * x.ld + s0;
-            g.push_back({P, P, a + (int64_t)s0 * x.ld + s0, n, n, 32, x.ld, x.ld, x.ld, -1.0, 1.0, 1, 0});
+            const int k0 = 64 * b, s1 = k0 + 32, s2 = k0 + 64;
+            if (s1 < x.m) {  // panel 2b+1 -= P P(s1:s1+w, :)^T
+                const int n = x.m - s1, w = std::min(32, n);
+                double *P = a + (int64_t)k0 * x.ld + s1;
+                gs.push_back({P, P, a + (int64_t)s1 * x.ld + s1, n, w, 32, x.ld, x.ld, x.ld, -1.0, 1.0,
+                              1, 0});
+            }
+            if (s2 < x.m) {  // trailing matrix -= [P P'] [P P']^T
+                const int n = x.m - s2;
+                double *P = a + (int64_t)k0 * x.ld + s2;
+                gb.push_back({P, P, a + (int64_t)s2 * x.ld + s2, n, n, 64, x.ld, x.ld, x.ld, -1.0, 1.0,
+                              1, 0});
+            }
         }
     }
-    start[npanel] = (int)g.size();
-    auto *dg = static_cast<GemmDesc *>(upload_table(ctx, "potrf_g", g.data(), sizeof(GemmDesc) * g.size()));
-    if (!dg) return PTBGPU_E_NOMEM;
-    for (int p = 0; p < npanel; ++p) {
-        {
-            ProfScope kprof_(ctx, "potrf_panel");
-            hipLaunchKernelGGL(k_potrf_panel, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, A, flags, p);
-            PTB_CHECK_LAUNCH(ctx);
-        }
-        const int c = start[p + 1] - start[p];
-        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dg + start[p], c, max_m, max_m, "potrf"));
+    ss[nblk] = (int)gs.size();
+    sb[nblk] = (int)gb.size();
+    auto *dgs = static_cast<GemmDesc *>(upload_table(ctx, "potrf_gs", gs.data(), sizeof(GemmDesc) * gs.size()));
+    auto *dgb = static_cast<GemmDesc *>(upload_table(ctx, "potrf_gb", gb.data(), sizeof(GemmDesc) * gb.size()));
+    if (!dgs || !dgb) return PTBGPU_E_NOMEM;
+    auto panel = [&](int p) -> int {
+        ProfScope kprof_(ctx, "potrf_panel");
+        hipLaunchKernelGGL(k_potrf_panel, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, A, flags, p);
+        PTB_CHECK_LAUNCH(ctx);
+        return PTBGPU_OK;
+    };
+    for (int b = 0; b < nblk; ++b) {
+        PTB_TRY(panel(2 * b));
+        int c = ss[b + 1] - ss[b];
+        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dgs + ss[b], c, max_m, 32, "potrf"));
+        if (2 * b + 1 < npanel) PTB_TRY(panel(2 * b + 1));
+        c = sb[b + 1] - sb[b];
+        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dgb + sb[b], c, max_m, max_m, "potrf"));
     }
     return PTBGPU_OK;
 }
