@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 3 : 2) void k_sytrd_panel(cons
     double *c2 = c1 + kNB;        // kNB  V[k][k0+t]
     double *q1 = c2 + kNB;        // kNB  W[:,t]^T v
     double *q2 = q1 + kNB;        // kNB  V[:,k0+t]^T v
-    double *qred = q2 + kNB;      // kWaves x 16
+    double *qred = q2 + kNB;      // kWaves x 8 (float storage: wavefront parts of the dots)
     const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
 
     const int k0 = panel * kNB;
@@ -189,36 +189,66 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 3 : 2) void k_sytrd_panel(cons
                 tt[k] = tk;
             }
             __syncthreads();
-            // (d) pending dots q1[t] = W[:,t]^T v, q2[t] = V[:,k0+t]^T v, four t at a time
-            for (int t0 = 0; t0 < jj; t0 += 4) {
-                double acc[8];
+            // (d) pending dots q1[t] = W[:,t]^T v, q2[t] = V[:,k0+t]^T v,
+            // eight sums per butterfly (lanes with lane % 8 == 0 end with
+            // sum (lane >> 3) & 7).  FP64 storage: wavefront wid takes the
+            // groups of four t from 4 wid, 4 (wid + kWaves), ... over all rows,
+            // with no barrier (q1 and q2 are read in (f), after the barriers
+            // of (e)).  Float storage (three workgroups per CU, fewer
+            // registers): the workgroup takes one group at a time and sums
+            // the wavefronts' parts through LDS.
+            if constexpr (sizeof(T) == 8) {
+                for (int t0 = 4 * wid; t0 < jj; t0 += 4 * kWaves) {
+                    double acc[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) acc[q] = 0.0;
-                for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
-                    const double v = vs[i];
+                    for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+#pragma unroll 2
+                    for (int i = k + 1 + lane; i < m; i += PTB_WAVE) {
+                        const double v = vs[i];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int t = min(t0 + q, jj - 1);
-                        acc[q] += W[(int64_t)t * ld + i] * v;
-                        acc[4 + q] += a[(int64_t)(k0 + t) * ld + i] * v;
+                        for (int q = 0; q < 4; ++q) {
+                            const int t = min(t0 + q, jj - 1);
+                            acc[q] += W[(int64_t)t * ld + i] * v;
+                            acc[4 + q] += a[(int64_t)(k0 + t) * ld + i] * v;
+                        }
+                    }
+                    const double sq = butterfly8(acc, lane);
+                    if ((lane & 7) == 0) {
+                        const int q = (lane >> 3) & 7, t = t0 + (q & 3);
+                        if (t < jj) {
+                            if (q < 4) q1[t] = sq;
+                            else q2[t] = sq;
+                        }
                     }
                 }
-                {
-                    // wavefront sums: lanes with lane % 8 == 0 hold sum (lane >> 3) & 7
+            } else {
+                for (int t0 = 0; t0 < jj; t0 += 4) {
+                    double acc[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+                    for (int i = k + 1 + threadIdx.x; i < m; i += kT) {
+                        const double v = vs[i];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int t = min(t0 + q, jj - 1);
+                            acc[q] += W[(int64_t)t * ld + i] * v;
+                            acc[4 + q] += a[(int64_t)(k0 + t) * ld + i] * v;
+                        }
+                    }
                     const double sq = butterfly8(acc, lane);
                     if ((lane & 7) == 0) qred[wid * 8 + ((lane >> 3) & 7)] = sq;
-                }
-                __syncthreads();
-                if ((int)threadIdx.x < 8) {
-                    const int q = threadIdx.x & 3;
-                    double sq = 0.0;
-                    for (int w = 0; w < kWaves; ++w) sq += qred[w * 8 + threadIdx.x];
-                    if (t0 + q < jj) {
-                        if (threadIdx.x < 4) q1[t0 + q] = sq;
-                        else q2[t0 + q] = sq;
+                    __syncthreads();
+                    if ((int)threadIdx.x < 8) {
+                        const int q = threadIdx.x & 3;
+                        double t8 = 0.0;
+                        for (int w = 0; w < kWaves; ++w) t8 += qred[w * 8 + threadIdx.x];
+                        if (t0 + q < jj) {
+                            if (threadIdx.x < 4) q1[t0 + q] = t8;
+                            else q2[t0 + q] = t8;
+                        }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
             }
             // (e) praw = A22 v from the lower triangle only (A22 is symmetric
             // and only its lower triangle is kept up to date).  Vertical
@@ -255,11 +285,15 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 3 : 2) void k_sytrd_panel(cons
                 constexpr bool kAhead = sizeof(T) == 4;
                 for (int sidx = wid; sidx < nstrip; sidx += kWaves) {
                     const int c0 = base + sidx * 16;
+                    // v of the strip's columns: in registers for FP64 storage;
+                    // for float storage (three workgroups per CU, fewer
+                    // registers) read from LDS per block (broadcast reads)
+                    constexpr bool kVcReg = sizeof(T) == 8;
                     double vc[16], zc[16];
 #pragma unroll
                     for (int q = 0; q < 16; ++q) {
                         const int c = c0 + q;
-                        vc[q] = (c > k && c < m) ? vs[c] : 0.0;
+                        if (kVcReg) vc[q] = (c > k && c < m) ? vs[c] : 0.0;
                         zc[q] = 0.0;
                     }
                     const bool cols_in = c0 > k && c0 + 16 <= m;  // no column of the strip masked
@@ -277,7 +311,7 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 3 : 2) void k_sytrd_panel(cons
 #pragma unroll
                             for (int q = 0; q < 16; ++q) {
                                 const double x = (double)raw[q];
-                                yrow += x * vc[q];
+                                yrow += x * (kVcReg ? vc[q] : vs[c0 + q]);
                                 zc[q] += x * vi;
                             }
                             cpw[i] += yrow;
@@ -288,7 +322,7 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 3 : 2) void k_sytrd_panel(cons
                             for (int q = 0; q < 16; ++q) {
                                 const int c = c0 + q;
                                 const double x = (ok && c <= i && c > k) ? (double)raw[q] : 0.0;
-                                yrow += x * vc[q];
+                                yrow += x * (kVcReg ? vc[q] : vs[min(max(c, k + 1), m - 1)]);
                                 if (c < i) zc[q] += x * vi;
                             }
                             if (ok) cpw[i] += yrow;
@@ -378,7 +412,7 @@ int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc 
     const int nsm = (int)desc.size();
     int max_m = 0;
     for (const auto &x : desc) max_m = std::max(max_m, x.m);
-    const size_t shmem = sizeof(double) * ((1 + kWaves) * (size_t)max_m + kWaves + 4 * kNB + kWaves * 16);
+    const size_t shmem = sizeof(double) * ((1 + kWaves) * (size_t)max_m + kWaves + 4 * kNB + kWaves * 8);
     if (shmem > 160 * 1024) return fail(ctx, PTBGPU_E_ARG, "submatrix too large for sytrd (%d)", max_m);
     const int npanel = max_m > 2 ? (max_m - 2 + kNB - 1) / kNB : 0;
     std::vector<int64_t> pwoff(nsm);
