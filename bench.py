@@ -37,7 +37,8 @@ WORKLOADS = {
 # bracketed names: whole stages (their GEMMs included) and single kernels
 PHASES = ["potrf", "trsm", "trsm_h", "trsm_k", "trsm_e", "trsm_u", "sytrd", "apply_q",
           "apply_q_x", "apply_q_o", "apply_q_r1", "apply_q_r2", "gemm_sytrd", "gemm_potrf",
-          "gemm_trsm", "gemm_dc", "gemm_q", "gemm_fe", "gemm_refine", "gemm_other", "refine_all"]
+          "gemm_trsm", "gemm_dc", "gemm_q", "gemm_fe", "gemm_refine", "gemm_other", "refine_all",
+          "inv_lower", "gemm_linv", "gemm_kform", "gemm_u"]
 KERNELS = ["pattern", "group_union", "group_map", "gather", "potrf_panel", "diag_inv",
            "transpose", "sytrd_panel", "zero_upper", "dc_leaf", "dc_merge", "gemm", "larft",
            "fe_rows", "fe_finish", "bisect", "occ_compact", "dmat_scatter", "to_f32", "promote",

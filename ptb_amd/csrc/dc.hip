@@ -420,6 +420,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         g->alpha = 1.0;
         g->beta = 0.0;
         g->uplo = 0;
+        g->tri = 0;
     }
     (void)srho;
 }

@@ -57,6 +57,15 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
     if (g.M <= 0 || g.N <= 0 || tile >= tilesM * tilesN) return;
     const int m0 = (tile % tilesM) * 64, n0 = (tile / tilesM) * 64;
     if (g.uplo && n0 > m0 + 63) return;  // tile strictly above the diagonal
+    // k range of this tile where triangular factors can be nonzero, aligned
+    // down to the slice depth: the other triangle of a triangular factor must
+    // hold zeros (the few of its elements inside the range add nothing)
+    int kbeg = 0, kend = g.K;
+    if (g.tri & kTriALower) kend = min(kend, m0 + 64);
+    if (g.tri & kTriAUpper) kbeg = max(kbeg, m0);
+    if (g.tri & kTriBUpper) kend = min(kend, n0 + 64);
+    if (g.tri & kTriBLower) kbeg = max(kbeg, n0);
+    kbeg &= ~(kKS - 1);
     __shared__ double As[kKS][68];
     __shared__ double Bs[kKS][68];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -88,12 +97,12 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
             a_index(q, mm, kk);
             const int gm = m0 + mm, gk = k0 + kk;
             const unsigned oa = TA ? (unsigned)gm * ulda + (unsigned)gk : (unsigned)gk * ulda + (unsigned)gm;
-            ra[q] = (gm < g.M && gk < g.K) ? (double)gA[oa] : 0.0;
+            ra[q] = (gm < g.M && gk < kend) ? (double)gA[oa] : 0.0;
             int kb, nn;
             b_index(q, kb, nn);
             const int gn = n0 + nn, gkb = k0 + kb;
             const unsigned ob = TB ? (unsigned)gkb * uldb + (unsigned)gn : (unsigned)gn * uldb + (unsigned)gkb;
-            rb[q] = (gn < g.N && gkb < g.K) ? (double)gB[ob] : 0.0;
+            rb[q] = (gn < g.N && gkb < kend) ? (double)gB[ob] : 0.0;
         }
     };
     auto store = [&]() {
@@ -106,11 +115,11 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
             Bs[kb][nn] = rb[q];
         }
     };
-    load(0);
+    load(kbeg);
     store();
     __syncthreads();
-    for (int k0 = 0; k0 < g.K; k0 += kKS) {
-        const bool more = k0 + kKS < g.K;
+    for (int k0 = kbeg; k0 < kend; k0 += kKS) {
+        const bool more = k0 + kKS < kend;
         if (more) load(k0 + kKS);
 #pragma unroll
         for (int ks = 0; ks < kKS / 4; ++ks) {
@@ -414,9 +423,90 @@ __global__ __launch_bounds__(64) void k_diag_inv(const SubDesc *__restrict__ des
     for (int r = 0; r < 64; ++r) out[r] = x[r];
 }
 
+// Li = the block-diagonal of 64x64 inverses (from k_diag_inv), zero
+// elsewhere: the first level of the batched triangular inverse.
+// grid: (submatrix, 64-column block)
+__global__ __launch_bounds__(kT) void k_linv_init(const SubDesc *__restrict__ desc,
+                                                  const int64_t *__restrict__ doff,
+                                                  const double *__restrict__ dinv,
+                                                  double *__restrict__ Li) {
+    const SubDesc d = desc[blockIdx.x];
+    const int b = blockIdx.y, c0 = 64 * b;
+    if (c0 >= d.m) return;
+    const double *blk = dinv + doff[blockIdx.x] + (int64_t)b * 4096;
+    const int nc = min(64, d.m - c0);
+    for (int l = threadIdx.x; l < nc * d.m; l += kT) {
+        const int c = l / d.m, r = l % d.m;
+        const int rr = r - c0;
+        const double v = (rr >= c && rr < 64) ? blk[c * 64 + rr] : 0.0;
+        Li[d.off + (int64_t)(c0 + c) * d.ld + r] = v;
+    }
+}
+
 }  // namespace
 
 namespace ptb {
+
+// Li = L^-1 for every lower-triangular L of the batch (the upper triangle of L
+// is not read).  The 64x64 diagonal blocks are inverted in LDS; then, with the
+// block size h doubling, every pair of diagonal blocks [L11 0; L21 L22] (the
+// second one possibly partial) gets Li21 = -Li22 (L21 Li11): two batched
+// triangular GEMMs per level, T the scratch for L21 Li11.  Li is lower
+// triangular with zeros above the diagonal.
+int inv_lower_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                    const double *L, double *Li, double *T) {
+    ProfScope prof_(ctx, "inv_lower");
+    const int nsm = (int)desc.size();
+    std::vector<int64_t> doff(nsm);
+    int64_t tot = 0;
+    int max_nb = 0, max_m = 0;
+    for (int s = 0; s < nsm; ++s) {
+        doff[s] = tot;
+        const int nb = (desc[s].m + 63) / 64;
+        tot += (int64_t)nb * 4096;
+        max_nb = std::max(max_nb, nb);
+        max_m = std::max(max_m, desc[s].m);
+    }
+    auto *dinv = static_cast<double *>(ws(ctx, "linv_dinv", sizeof(double) * std::max<int64_t>(tot, 1)));
+    auto *ddoff = static_cast<int64_t *>(upload_table(ctx, "linv_doff", doff.data(), sizeof(int64_t) * nsm));
+    if (!dinv || !ddoff) return PTBGPU_E_NOMEM;
+    {
+        ProfScope kprof_(ctx, "diag_inv");
+        hipLaunchKernelGGL(k_diag_inv, dim3(nsm, max_nb), dim3(64), 0, ctx->stream, ddesc, ddoff, L,
+                           dinv);
+        PTB_CHECK_LAUNCH(ctx);
+        hipLaunchKernelGGL(k_linv_init, dim3(nsm, max_nb), dim3(kT), 0, ctx->stream, ddesc, ddoff,
+                           dinv, Li);
+        PTB_CHECK_LAUNCH(ctx);
+    }
+    int level = 0;
+    for (int h = 64; h < max_m; h *= 2, ++level) {
+        std::vector<GemmDesc> g1, g2;
+        for (int s = 0; s < nsm; ++s) {
+            const SubDesc &x = desc[s];
+            for (int o = 0; o + h < x.m; o += 2 * h) {
+                const int o2 = o + h, n2 = std::min(h, x.m - o2);
+                const int64_t b11 = x.off + (int64_t)o * x.ld + o;
+                const int64_t b21 = x.off + (int64_t)o * x.ld + o2;
+                const int64_t b22 = x.off + (int64_t)o2 * x.ld + o2;
+                // T21 = L21 Li11 (Li11 lower)
+                g1.push_back({L + b21, Li + b11, T + b21, n2, h, h, x.ld, x.ld, x.ld, 1.0, 0.0, 0,
+                              kTriBLower});
+                // Li21 = -Li22 T21 (Li22 lower)
+                g2.push_back({Li + b22, T + b21, Li + b21, n2, h, n2, x.ld, x.ld, x.ld, -1.0, 0.0, 0,
+                              kTriALower});
+            }
+        }
+        if (g1.empty()) continue;
+        const std::string n1 = "linv_g1_" + std::to_string(level), n2 = "linv_g2_" + std::to_string(level);
+        auto *dg1 = static_cast<GemmDesc *>(upload_table(ctx, n1.c_str(), g1.data(), sizeof(GemmDesc) * g1.size()));
+        auto *dg2 = static_cast<GemmDesc *>(upload_table(ctx, n2.c_str(), g2.data(), sizeof(GemmDesc) * g2.size()));
+        if (!dg1 || !dg2) return PTBGPU_E_NOMEM;
+        PTB_TRY(gemm_vbatch(ctx, false, false, dg1, (int)g1.size(), h, h, "linv"));
+        PTB_TRY(gemm_vbatch(ctx, false, false, dg2, (int)g2.size(), h, h, "linv"));
+    }
+    return PTBGPU_OK;
+}
 
 int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int count, int max_m,
                 int max_n, const char *site, bool f32) {

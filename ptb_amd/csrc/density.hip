@@ -4,9 +4,9 @@
 // fe(k) = sum_{owned rows i} (Ssm U)(i,k) U(i,k).  With Ssm = L L^T,
 // U = L^-T Q Z (Q the Householder product, Z the tridiagonal eigenvectors) the
 // owned rows are (L Q Z)(O,:) and (L^-T Q Z)(O,:): 2c row vectors
-// [L(O,:); (L^-1 e_O)^T] are pushed through Q and Z (c = owned columns), so no
-// full back-transformation is needed for fe; owned rows accumulate in the
-// reference's order.
+// [L(O,:); (L^-1 e_O)^T] (rows of L and columns of L^-1) are pushed through Q
+// and Z (c = owned columns), so no full back-transformation is needed for fe;
+// owned rows accumulate in the reference's order.
 //
 // The bisection runs per molecule of a batch, every pass one launch over all
 // (eigenvalue, fe) pairs (per-block partials, then a fixed-order sum), so it is
@@ -30,25 +30,26 @@ constexpr int kG = 8;  // owned columns per pass
 // Right-hand sides of the owned-row projections, per submatrix with c owned
 // columns (local rows o_g = oloc[g]):
 //   XA(:, g)     = L(o_g, :)^T            (row o_g of L, zero past o_g)
-//   XB(:, g)     = e_{o_g}                (solved to L^-1 e_{o_g} into XA(:, c + g))
-// grid: (owned column, submatrix)
+//   XA(:, c + g) = Li(:, o_g) = L^-1 e_{o_g}
+// grid: (submatrix, owned column)
 __global__ __launch_bounds__(kT) void k_xall_build(const SubDesc *__restrict__ desc,
                                                    const int *__restrict__ oloc,
                                                    const double *__restrict__ La,
+                                                   const double *__restrict__ Lia,
                                                    const int64_t *__restrict__ xoff,
-                                                   const int64_t *__restrict__ xboff,
-                                                   double *__restrict__ XA,
-                                                   double *__restrict__ XB) {
+                                                   double *__restrict__ XA) {
     const SubDesc d = desc[blockIdx.x];
     const int g = d.gbeg + blockIdx.y;
     if (g >= d.gend) return;
+    const int c = d.gend - d.gbeg;
     const int row = oloc[g];
     const double *L = La + d.off;
+    const double *li = Lia + d.off + (int64_t)row * d.ld;
     double *xa = XA + xoff[blockIdx.x] + (int64_t)blockIdx.y * d.ld;
-    double *xb = XB + xboff[blockIdx.x] + (int64_t)blockIdx.y * d.ld;
+    double *xb = XA + xoff[blockIdx.x] + (int64_t)(c + blockIdx.y) * d.ld;
     for (int t = threadIdx.x; t < d.m; t += kT) {
         xa[t] = t <= row ? L[(int64_t)t * d.ld + row] : 0.0;
-        xb[t] = t == row ? 1.0 : 0.0;
+        xb[t] = li[t];
     }
 }
 
@@ -353,12 +354,11 @@ __global__ __launch_bounds__(kT) void k_dmat_compact(const SubDesc *__restrict__
 namespace ptb {
 
 int xall_build(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_own, const int *doloc,
-               const double *L, const int64_t *dxoff, const int64_t *dxboff, double *XA,
-               double *XB) {
+               const double *L, const double *Li, const int64_t *dxoff, double *XA) {
     ProfScope prof_(ctx, "fe_rows");
     if (max_own <= 0) return PTBGPU_OK;
     hipLaunchKernelGGL(k_xall_build, dim3(nsm, max_own), dim3(kT), 0, ctx->stream, ddesc, doloc, L,
-                       dxoff, dxboff, XA, XB);
+                       Li, dxoff, XA);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }

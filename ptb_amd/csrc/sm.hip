@@ -459,34 +459,23 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     double *dvec = vec, *evec = vec + rows, *tau = vec + 2 * rows, *fe = vec + 3 * rows;
     double *rot = vec + 4 * rows;   // 4 * rows
     double *wocc = vec + 8 * rows;  // rows
-    // owned-row projection buffers: XA = [L(O,:)^T, L^-1 e_O] (m x 2c), XB the
-    // unit right-hand sides (m x c), F = Z^T Q^T XA (m x 2c)
-    std::vector<int64_t> xoff(nsm), xboff(nsm);
-    std::vector<int> c2(nsm), c1(nsm);
-    RhsLayout lrhs;
-    lrhs.boff.resize(nsm);
-    lrhs.xoff.resize(nsm);
-    lrhs.cols.resize(nsm);
-    int64_t xtot = 0, xbtot = 0;
+    // owned-row projection buffers: XA = [L(O,:)^T, L^-1 e_O] (m x 2c),
+    // F = Z^T Q^T XA (m x 2c)
+    std::vector<int64_t> xoff(nsm);
+    std::vector<int> c2(nsm);
+    int64_t xtot = 0;
     for (int s = 0; s < nsm; ++s) {
         const int c = desc[s].gend - desc[s].gbeg;
         xoff[s] = xtot;
-        xboff[s] = xbtot;
-        c1[s] = c;
         c2[s] = 2 * c;
-        lrhs.boff[s] = xbtot;
-        lrhs.xoff[s] = xtot + (int64_t)c * desc[s].ld;
-        lrhs.cols[s] = c;
         xtot += (int64_t)desc[s].ld * 2 * c;
-        xbtot += (int64_t)desc[s].ld * c;
     }
     auto *XA = static_cast<double *>(ptb::ws(ctx, "sm_xa", sizeof(double) * std::max<int64_t>(xtot, 1)));
-    auto *XB = static_cast<double *>(ptb::ws(ctx, "sm_xb", sizeof(double) * std::max<int64_t>(xbtot, 1)));
     auto *F = static_cast<double *>(ptb::ws(ctx, "sm_f", sizeof(double) * std::max<int64_t>(xtot, 1)));
-    int64_t *dxoff, *dxboff;
+    auto *Li = static_cast<double *>(ptb::ws(ctx, "sm_Li", abytes));
+    int64_t *dxoff;
     PTB_TRY(upload(ctx, "sm_xoff", xoff, &dxoff));
-    PTB_TRY(upload(ctx, "sm_xboff", xboff, &dxboff));
-    if (!XA || !XB || !F) return PTBGPU_E_NOMEM;
+    if (!XA || !F || !Li) return PTBGPU_E_NOMEM;
     for (int i = 0; i < nmol; ++i)
         PTB_TRY(ptb::gather(ctx, mols[i].dH, mols[i].dS, ddesc + sbeg[i], sbeg[i + 1] - sbeg[i],
                             mol_max_m[i], dmap, A, B));
@@ -494,13 +483,28 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     // ---- eigendecomposition (every submatrix of every molecule)
     PTB_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int) * nsm, st));
     PTB_TRY(ptb::potrf_batch(ctx, desc, ddesc, B, flags));
-    PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, A, Z, false, "h"));        // Z = L^-1 H
-    PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, Z, A, false));  // A = (L^-1 H)^T
-    PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, A, Z, false, "k"));        // Z = K
-    PTB_TRY(ptb::transpose_sym_batch(ctx, ddesc, nsm, max_m, Z, A, true));   // A = sym(K)
+    // K = L^-1 H L^-T from Li = L^-1: Z = Li H, A = Z Li^T (the lower triangle,
+    // which the tridiagonalisation reads; both triangles for the mixed solve,
+    // whose refinement multiplies by K)
+    const bool mixed = opts.precision == PTBGPU_MIXED;
+    PTB_TRY(ptb::inv_lower_batch(ctx, desc, ddesc, B, Li, X));
+    {
+        std::vector<GemmDesc> g1(nsm), g2(nsm);
+        for (int s = 0; s < nsm; ++s) {
+            const SubDesc &d = desc[s];
+            g1[s] = {Li + d.off, A + d.off, Z + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0,
+                     kTriALower};
+            g2[s] = {Z + d.off, Li + d.off, A + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0, 0.0,
+                     mixed ? 0 : 1, kTriBUpper};
+        }
+        GemmDesc *dg1, *dg2;
+        PTB_TRY(upload(ctx, "sm_gk1", g1, &dg1));
+        PTB_TRY(upload(ctx, "sm_gk2", g2, &dg2));
+        PTB_TRY(ptb::gemm_vbatch(ctx, false, false, dg1, nsm, max_m, max_m, "kform"));
+        PTB_TRY(ptb::gemm_vbatch(ctx, false, true, dg2, nsm, max_m, max_m, "kform"));
+    }
     // mixed precision: the tridiagonalisation on float storage, its reflectors
     // in V, and K (FP64) kept in A for the refinement below
-    const bool mixed = opts.precision == PTBGPU_MIXED;
     float *K32 = nullptr;
     double *V = nullptr, *R = nullptr, *thn = nullptr;
     int *oidx = nullptr;
@@ -518,8 +522,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     // ---- fe from the owned rows: F = Z^T Q^T [L(O,:)^T, L^-1 e_O]
     std::vector<int> hflags(nsm);
     PTB_HIP(ctx, hipMemcpyAsync(hflags.data(), flags, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
-    PTB_TRY(ptb::xall_build(ctx, ddesc, nsm, max_own, doloc, B, dxoff, dxboff, XA, XB));
-    PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, XB, XA, false, "e", &lrhs));  // L^-1 e_O
+    PTB_TRY(ptb::xall_build(ctx, ddesc, nsm, max_own, doloc, B, Li, dxoff, XA));
     PTB_TRY(ptb::apply_q(ctx, desc, qf, XA, xoff, c2, true, "x"));           // Q^T XA
     GemmDesc *dgf;
     {
@@ -581,18 +584,23 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     PTB_HIP(ctx, hipStreamSynchronize(st));
     int max_nocc = 0;
     std::vector<int64_t> aoff(nsm);
-    RhsLayout urhs;
-    urhs.boff.resize(nsm);
-    urhs.xoff.resize(nsm);
-    urhs.cols = hnocc;
     for (int s = 0; s < nsm; ++s) {
         max_nocc = std::max(max_nocc, hnocc[s]);
         aoff[s] = desc[s].off;
-        urhs.boff[s] = desc[s].off;
-        urhs.xoff[s] = desc[s].off;
     }
     PTB_TRY(ptb::apply_q(ctx, desc, qf, W, aoff, hnocc, false, "o"));        // W = Q Z_o
-    PTB_TRY(ptb::trsm_gemm(ctx, desc, ddesc, B, W, X, true, "u", &urhs));    // X = L^-T Q Z_o
+    {
+        // X = L^-T Q Z_o = Li^T W
+        std::vector<GemmDesc> gu(nsm);
+        for (int s = 0; s < nsm; ++s) {
+            const SubDesc &d = desc[s];
+            gu[s] = {Li + d.off, W + d.off, X + d.off, d.m, hnocc[s], d.m, d.ld, d.ld, d.ld, 1.0, 0.0,
+                     0, kTriAUpper};
+        }
+        GemmDesc *dgu;
+        PTB_TRY(upload(ctx, "sm_gu", gu, &dgu));
+        PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgu, nsm, max_m, max_nocc, "u"));
+    }
     for (int i = 0; i < nmol; ++i) {
         const int64_t n = mols[i].ndim;
         PTB_HIP(ctx, hipMemsetAsync(mols[i].dP, 0, sizeof(double) * (size_t)(n * (n + 1) / 2), st));

@@ -25,6 +25,10 @@ constexpr int kBisStride = 4 + 200;
 // One product C = alpha op(A) op(B) + beta C of a variable-size batch.  With
 // uplo = 1 only the lower triangle of C (row >= column) is formed: tiles above
 // the diagonal are skipped (symmetric updates such as A -= V W^T + W V^T).
+// tri marks op(A) / op(B) as triangular (kTri* bits): each output tile runs
+// only over the k range where both factors can be nonzero (triangular products
+// at about half the flops).  The other triangle of such a factor must be
+// stored as zeros.
 struct GemmDesc {
     const double *A;
     const double *B;
@@ -33,8 +37,13 @@ struct GemmDesc {
     int lda, ldb, ldc;
     double alpha, beta;
     int uplo;
-    int reserved;
+    int tri;
 };
+
+constexpr int kTriALower = 1;  // op(A)(i, k) = 0 for k > i
+constexpr int kTriAUpper = 2;  // op(A)(i, k) = 0 for k < i
+constexpr int kTriBUpper = 4;  // op(B)(k, j) = 0 for k > j
+constexpr int kTriBLower = 8;  // op(B)(k, j) = 0 for k < j
 
 // A divide-and-conquer merge node: rows/cols [lo, hi) of submatrix s, split at sp.
 struct DcNode {
@@ -79,6 +88,10 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
                 double *A, int *flags);
 int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m,
                         const double *in, double *out, bool symmetrize);
+// Li = L^-1 of every lower-triangular L (zeros above the diagonal); T scratch
+// with the arena layout
+int inv_lower_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
+                    const double *L, double *Li, double *T);
 int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
               const double *L, double *B, double *X, bool trans, const char *tag,
               const RhsLayout *rhs = nullptr);
@@ -127,9 +140,9 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
                    double *rot, GemmDesc *dgemm);
 
 // density.hip
+// XA = [L(O,:)^T, Li(:,O)] per submatrix (Li = L^-1)
 int xall_build(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_own, const int *doloc,
-               const double *L, const int64_t *dxoff, const int64_t *dxboff, double *XA,
-               double *XB);
+               const double *L, const double *Li, const int64_t *dxoff, double *XA);
 int fe_proj(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const int64_t *dxoff,
             const double *F, double *fe);
 // mu bisection of nmol molecules at once: molecule i owns the states
