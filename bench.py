@@ -344,7 +344,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if main_prec == "fp64" else "f32+f64",
         "data": "synthetic (deterministic geometry; Gaussian/extended-Hueckel surrogate H,S)",
-        "config": {"workload": f"{args.config} (512 H2O, 1536 atoms, ndim {n}), filter {wl['filter']}, "
+        "config": {"workload": f"{args.config} ({mol.name}, {mol.n} atoms, ndim {n}), filter {wl['filter']}, "
                                f"atoms_columns, submatrix_sygv",
                    "nsm": int(r["rep"]["nsm"]), "max_m": int(r["rep"]["max_m"]),
                    "sum_m3": float(r["rep"]["sum_m3"]), "precision": main_prec},

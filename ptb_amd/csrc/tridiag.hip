@@ -267,40 +267,44 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 3 : 2) void k_sytrd_panel(cons
                 for (int w = 0; w < kWaves; ++w) cp[w * m + i] = 0.0;
             __syncthreads();
             {
-                const int base = (k + 1) & ~15;
-                const int nstrip = (m - base + 15) / 16;
+                // strip width: 16 columns for float storage, 8 for FP64
+                // storage (fewer registers per wavefront: two workgroups per
+                // CU keep more loads in flight)
+                constexpr int kSW = sizeof(T) == 8 ? 8 : 16;
+                const int base = (k + 1) & ~(kSW - 1);
+                const int nstrip = (m - base + kSW - 1) / kSW;
                 double *cpw = cp + wid * m;
                 const unsigned uld = (unsigned)ld;
-                // loads of one 64 x 16 block through 32-bit offsets from the
+                // loads of one 64 x kSW block through 32-bit offsets from the
                 // submatrix block, all in flight together (no branch per
                 // element); the last row block and the last strip clamp to
                 // in-bounds addresses (their excess is masked below)
-                auto fetch = [&](int r, int c0, T (&raw)[16]) {
+                auto fetch = [&](int r, int c0, T (&raw)[kSW]) {
                     const unsigned ic = (unsigned)min(r + lane, m - 1);
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) raw[q] = a[(unsigned)min(c0 + q, m - 1) * uld + ic];
+                    for (int q = 0; q < kSW; ++q) raw[q] = a[(unsigned)min(c0 + q, m - 1) * uld + ic];
                 };
                 // with float storage the next block's loads are issued before
                 // this block's arithmetic
                 constexpr bool kAhead = sizeof(T) == 4;
                 for (int sidx = wid; sidx < nstrip; sidx += kWaves) {
-                    const int c0 = base + sidx * 16;
+                    const int c0 = base + sidx * kSW;
                     // v of the strip's columns: in registers for FP64 storage;
                     // for float storage (three workgroups per CU, fewer
                     // registers) read from LDS per block (broadcast reads)
                     constexpr bool kVcReg = sizeof(T) == 8;
-                    double vc[16], zc[16];
+                    double vc[kSW], zc[kSW];
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) {
+                    for (int q = 0; q < kSW; ++q) {
                         const int c = c0 + q;
                         if (kVcReg) vc[q] = (c > k && c < m) ? vs[c] : 0.0;
                         zc[q] = 0.0;
                     }
-                    const bool cols_in = c0 > k && c0 + 16 <= m;  // no column of the strip masked
-                    T raw[16];
+                    const bool cols_in = c0 > k && c0 + kSW <= m;  // no column of the strip masked
+                    T raw[kSW];
                     fetch(c0, c0, raw);
                     for (int r = c0; r < m; r += 64) {
-                        T rawn[16];
+                        T rawn[kSW];
                         if (kAhead) fetch(min(r + 64, m - 1), c0, rawn);  // past the end: clamped, unused
                         const int i = r + lane;
                         double yrow = 0.0;
@@ -309,7 +313,7 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 3 : 2) void k_sytrd_panel(cons
                             // and columns inside the trailing matrix
                             const double vi = vs[i];
 #pragma unroll
-                            for (int q = 0; q < 16; ++q) {
+                            for (int q = 0; q < kSW; ++q) {
                                 const double x = (double)raw[q];
                                 yrow += x * (kVcReg ? vc[q] : vs[c0 + q]);
                                 zc[q] += x * vi;
@@ -319,7 +323,7 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 3 : 2) void k_sytrd_panel(cons
                             const bool ok = i < m && i > k;
                             const double vi = ok ? vs[i] : 0.0;
 #pragma unroll
-                            for (int q = 0; q < 16; ++q) {
+                            for (int q = 0; q < kSW; ++q) {
                                 const int c = c0 + q;
                                 const double x = (ok && c <= i && c > k) ? (double)raw[q] : 0.0;
                                 yrow += x * (kVcReg ? vc[q] : vs[min(max(c, k + 1), m - 1)]);
@@ -329,14 +333,23 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 3 : 2) void k_sytrd_panel(cons
                         }
                         if (kAhead) {
 #pragma unroll
-                            for (int q = 0; q < 16; ++q) raw[q] = rawn[q];
+                            for (int q = 0; q < kSW; ++q) raw[q] = rawn[q];
                         } else if (r + 64 < m) {
                             fetch(r + 64, c0, raw);
                         }
                     }
-                    const double colsum = butterfly16(zc, lane);
-                    const int cq = c0 + ((lane >> 2) & 15);
-                    if ((lane & 3) == 0 && cq < m && cq > k) cpw[cq] += colsum;
+                    // column partials summed over the wavefront: sum q ends in
+                    // the lanes with (lane >> 2) & 15 == q (16 wide) or
+                    // (lane >> 3) & 7 == q (8 wide)
+                    if constexpr (kSW == 16) {
+                        const double colsum = butterfly16(zc, lane);
+                        const int cq = c0 + ((lane >> 2) & 15);
+                        if ((lane & 3) == 0 && cq < m && cq > k) cpw[cq] += colsum;
+                    } else {
+                        const double colsum = butterfly8(zc, lane);
+                        const int cq = c0 + ((lane >> 3) & 7);
+                        if ((lane & 7) == 0 && cq < m && cq > k) cpw[cq] += colsum;
+                    }
                 }
             }
             __syncthreads();
