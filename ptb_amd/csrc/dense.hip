@@ -385,6 +385,95 @@ __global__ __launch_bounds__(kT) void k_larft_all(const SubDesc *__restrict__ de
     }
 }
 
+// T factor of every 64-reflector block from the factors of its two
+// 32-reflector halves (the second may be absent):
+//   (I - V1 T1 V1^T)(I - V2 T2 V2^T) = I - [V1 V2] [T1 T12; 0 T2] [V1 V2]^T,
+//   T12 = -T1 (V1^T V2) T2.
+// V1^T V2 runs over the rows where V2 is nonzero (below its first unit),
+// staged through LDS 64 rows at a time.  T64 is 64 x 64 column-major at
+// T64a + toff64[s] + p * 4096.  grid: (submatrix, pair)
+__global__ __launch_bounds__(kT) void k_larft_merge(const SubDesc *__restrict__ desc,
+                                                    const double *__restrict__ Va,
+                                                    const double *__restrict__ Ta,
+                                                    const int64_t *__restrict__ toff,
+                                                    const int64_t *__restrict__ toff64,
+                                                    double *__restrict__ T64a) {
+    const SubDesc d = desc[blockIdx.x];
+    const int nr = d.m - 2;
+    const int nblk = nr > 0 ? (nr + 31) / 32 : 0;
+    const int p = blockIdx.y, b = 2 * p;
+    if (b >= nblk) return;
+    const bool two = b + 1 < nblk;
+    __shared__ double V1s[64][33], V2s[64][33];
+    __shared__ double T1[32][33], T2[32][33], G[32][33];
+    const double *T1g = Ta + toff[blockIdx.x] + (int64_t)b * 1024;
+    for (int l = threadIdx.x; l < 1024; l += kT) {
+        const int r = l & 31, c = l >> 5;
+        T1[r][c] = T1g[c * 32 + r];
+        T2[r][c] = two ? Ta[toff[blockIdx.x] + (int64_t)(b + 1) * 1024 + c * 32 + r] : 0.0;
+    }
+    // G = V1^T V2: thread owns entries (i, j) = (l & 31, l >> 5), l = tid + 256 h
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (two) {
+        // the second half holds kb2 reflectors (its columns past them are not
+        // read: they may lie past the submatrix)
+        const int kb2 = min(32, nr - (b + 1) * 32);
+        const double *V1 = Va + d.off + (int64_t)(b * 32) * d.ld;
+        const double *V2 = Va + d.off + (int64_t)((b + 1) * 32) * d.ld;
+        for (int r0 = (b + 1) * 32 + 1; r0 < d.m; r0 += 64) {
+            __syncthreads();
+            for (int l = threadIdx.x; l < 64 * 32; l += kT) {
+                const int rr = l & 63, cc = l >> 6, r = r0 + rr;
+                V1s[rr][cc] = r < d.m ? V1[(int64_t)cc * d.ld + r] : 0.0;
+                V2s[rr][cc] = (r < d.m && cc < kb2) ? V2[(int64_t)cc * d.ld + r] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int l = threadIdx.x + h * kT, i = l & 31, j = l >> 5;
+                double sacc = 0.0;
+#pragma unroll 8
+                for (int rr = 0; rr < 64; ++rr) sacc += V1s[rr][i] * V2s[rr][j];
+                acc[h] += sacc;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const int l = threadIdx.x + h * kT;
+        G[l & 31][l >> 5] = acc[h];
+    }
+    __syncthreads();
+    // M = G T2 into V1s (32 x 32), then T12 = -T1 M
+    double *M = &V1s[0][0];  // M(i, j) at M[i * 33 + j]
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const int l = threadIdx.x + h * kT, i = l & 31, j = l >> 5;
+        double sacc = 0.0;
+        for (int k = 0; k < 32; ++k) sacc += G[i][k] * T2[k][j];
+        M[i * 33 + j] = sacc;
+    }
+    __syncthreads();
+    double *T64 = T64a + toff64[blockIdx.x] + (int64_t)p * 4096;
+    for (int l = threadIdx.x; l < 4096; l += kT) {
+        const int r = l & 63, c = l >> 6;
+        double v;
+        if (r < 32 && c < 32) {
+            v = T1[r][c];
+        } else if (r >= 32 && c >= 32) {
+            v = T2[r - 32][c - 32];
+        } else if (r < 32) {  // T12(r, c - 32) = -sum_k T1(r, k) M(k, c - 32)
+            double sacc = 0.0;
+            for (int k = 0; k < 32; ++k) sacc += T1[r][k] * M[k * 33 + (c - 32)];
+            v = -sacc;
+        } else {
+            v = 0.0;
+        }
+        T64[c * 64 + r] = v;
+    }
+}
+
 // Inverse of every 64x64 diagonal block of each lower-triangular L (identity
 // padding past m).  grid: (block, submatrix); Dinv block b of submatrix s at
 // dinv + s_doff + b*4096, column-major, leading dimension 64.
@@ -712,9 +801,24 @@ int larft_all(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
     qf.T = static_cast<double *>(ws(ctx, "q_T", sizeof(double) * std::max<int64_t>(tot, 1)));
     auto *dtoff = static_cast<int64_t *>(upload_table(ctx, "q_toff", qf.toff.data(), sizeof(int64_t) * nsm));
     if (!qf.T || !dtoff) return PTBGPU_E_NOMEM;
+    qf.toff64.assign(nsm, 0);
+    int64_t tot64 = 0;
+    for (int s = 0; s < nsm; ++s) {
+        const int nr = desc[s].m - 2;
+        const int npair = nr > 0 ? (nr + 63) / 64 : 0;
+        qf.toff64[s] = tot64;
+        tot64 += (int64_t)npair * 4096;
+    }
+    qf.maxblk64 = (maxblk + 1) / 2;
+    qf.T64 = static_cast<double *>(ws(ctx, "q_T64", sizeof(double) * std::max<int64_t>(tot64, 1)));
+    auto *dtoff64 = static_cast<int64_t *>(upload_table(ctx, "q_toff64", qf.toff64.data(), sizeof(int64_t) * nsm));
+    if (!qf.T64 || !dtoff64) return PTBGPU_E_NOMEM;
     if (maxblk > 0) {
         hipLaunchKernelGGL(k_larft_all, dim3(nsm, maxblk), dim3(kT), 0, ctx->stream, ddesc, V, tau,
                            dtoff, qf.T);
+        PTB_CHECK_LAUNCH(ctx);
+        hipLaunchKernelGGL(k_larft_merge, dim3(nsm, qf.maxblk64), dim3(kT), 0, ctx->stream, ddesc, V,
+                           qf.T, dtoff, dtoff64, qf.T64);
         PTB_CHECK_LAUNCH(ctx);
     }
     return PTBGPU_OK;
@@ -722,22 +826,24 @@ int larft_all(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
 
 // Y <- Q Y (trans = false; blocks from the last to the first) or Y <- Q^T Y
 // (trans = true; blocks from the first, Q_b^T = I - V_b T_b^T V_b^T) for
-// ncols[s] columns of submatrix s at Y + yoff[s] (leading dimension ld).  Per
-// block: W1 = V_b^T Y, W2 = T_b W1 (or T_b^T W1), Y -= V_b W2, each a batched
-// GEMM; W1 and W2 are 32 x ncols panels in engine workspace.
+// ncols[s] columns of submatrix s at Y + yoff[s] (leading dimension ld), in
+// 64-reflector blocks.  Per block: W1 = V_b^T Y, W2 = T_b W1 (or T_b^T W1),
+// Y -= V_b W2, each a batched GEMM; W1 and W2 are 64 x ncols panels in engine
+// workspace.  V_b spans the block's 64 columns from the row after its first
+// reflector's pivot; its second half is zero above its own units.
 int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf, double *Y,
             const std::vector<int64_t> &yoff, const std::vector<int> &ncols, bool trans,
             const char *tag) {
     ProfScope prof_(ctx, "apply_q");
     ProfScope tprof_(ctx, (std::string("apply_q_") + tag).c_str());
     const int nsm = (int)desc.size();
-    if (qf.maxblk == 0) return PTBGPU_OK;
+    if (qf.maxblk64 == 0) return PTBGPU_OK;
     std::vector<int64_t> woff(nsm);
     int64_t wtot = 0;
     int max_m = 0, max_c = 0;
     for (int s = 0; s < nsm; ++s) {
         woff[s] = wtot;
-        wtot += 32 * (int64_t)std::max(ncols[s], 0);
+        wtot += 64 * (int64_t)std::max(ncols[s], 0);
         max_m = std::max(max_m, desc[s].m);
         max_c = std::max(max_c, ncols[s]);
     }
@@ -746,28 +852,28 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
     auto *W2 = static_cast<double *>(ws(ctx, "q_w2", sizeof(double) * std::max<int64_t>(wtot, 1)));
     if (!W1 || !W2) return PTBGPU_E_NOMEM;
     std::vector<GemmDesc> g1, g2, g3;
-    std::vector<int> start(qf.maxblk + 1, 0);
-    for (int q = 0; q < qf.maxblk; ++q) {
+    std::vector<int> start(qf.maxblk64 + 1, 0);
+    for (int q = 0; q < qf.maxblk64; ++q) {
         start[q] = (int)g1.size();
         for (int s = 0; s < nsm; ++s) {
             const SubDesc &d = desc[s];
             const int nc = ncols[s];
             const int nr = d.m - 2;
-            const int nblk = nr > 0 ? (nr + 31) / 32 : 0;
-            if (nc <= 0 || q >= nblk) continue;
-            const int b = trans ? q : nblk - 1 - q;
-            const int b0 = b * 32, kb = std::min(32, nr - b0);
+            const int npair = nr > 0 ? (nr + 63) / 64 : 0;
+            if (nc <= 0 || q >= npair) continue;
+            const int p = trans ? q : npair - 1 - q;
+            const int b0 = p * 64, kb = std::min(64, nr - b0);
             const int rowsv = d.m - b0 - 1;
             const double *V = qf.V + d.off + (int64_t)b0 * d.ld + b0 + 1;
             double *Yb = Y + yoff[s] + b0 + 1;
-            const double *T = qf.T + qf.toff[s] + (int64_t)b * 1024;
+            const double *T = qf.T64 + qf.toff64[s] + (int64_t)p * 4096;
             double *w1 = W1 + woff[s], *w2 = W2 + woff[s];
-            g1.push_back({V, Yb, w1, kb, nc, rowsv, d.ld, d.ld, 32, 1.0, 0.0});
-            g2.push_back({T, w1, w2, kb, nc, kb, 32, 32, 32, 1.0, 0.0});
-            g3.push_back({V, w2, Yb, rowsv, nc, kb, d.ld, 32, d.ld, -1.0, 1.0});
+            g1.push_back({V, Yb, w1, kb, nc, rowsv, d.ld, d.ld, 64, 1.0, 0.0});
+            g2.push_back({T, w1, w2, kb, nc, kb, 64, 64, 64, 1.0, 0.0});
+            g3.push_back({V, w2, Yb, rowsv, nc, kb, d.ld, 64, d.ld, -1.0, 1.0});
         }
     }
-    start[qf.maxblk] = (int)g1.size();
+    start[qf.maxblk64] = (int)g1.size();
     const std::string t(tag);
     if (g1.empty()) return PTBGPU_OK;
     const size_t gb = sizeof(GemmDesc) * g1.size();
@@ -775,11 +881,11 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
     GemmDesc *dg2 = static_cast<GemmDesc *>(upload_table(ctx, ("q_g2_" + t).c_str(), g2.data(), gb));
     GemmDesc *dg3 = static_cast<GemmDesc *>(upload_table(ctx, ("q_g3_" + t).c_str(), g3.data(), gb));
     if (!dg1 || !dg2 || !dg3) return PTBGPU_E_NOMEM;
-    for (int q = 0; q < qf.maxblk; ++q) {
+    for (int q = 0; q < qf.maxblk64; ++q) {
         const int c = start[q + 1] - start[q];
         if (c == 0) continue;
-        PTB_TRY(gemm_vbatch(ctx, true, false, dg1 + start[q], c, 32, max_c, "q"));
-        PTB_TRY(gemm_vbatch(ctx, trans, false, dg2 + start[q], c, 32, max_c, "q"));
+        PTB_TRY(gemm_vbatch(ctx, true, false, dg1 + start[q], c, 64, max_c, "q"));
+        PTB_TRY(gemm_vbatch(ctx, trans, false, dg2 + start[q], c, 64, max_c, "q"));
         PTB_TRY(gemm_vbatch(ctx, false, false, dg3 + start[q], c, max_m, max_c, "q"));
     }
     return PTBGPU_OK;

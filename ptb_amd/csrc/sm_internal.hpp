@@ -55,9 +55,12 @@ struct DcNode {
 // arena of the reduced matrices), T factors of all 32-reflector blocks.
 struct QFactor {
     const double *V = nullptr;
-    double *T = nullptr;
+    double *T = nullptr;          // 32 x 32 factors of the 32-reflector blocks
     std::vector<int64_t> toff;  // per submatrix, element offset of its first T block
     int maxblk = 0;
+    double *T64 = nullptr;        // 64 x 64 factors of pairs of blocks (what apply_q uses)
+    std::vector<int64_t> toff64;
+    int maxblk64 = 0;
 };
 
 // Right-hand sides of a batched triangular solve that are not the submatrices'
