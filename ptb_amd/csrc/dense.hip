@@ -66,8 +66,11 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
     if (g.tri & kTriBUpper) kend = min(kend, n0 + 64);
     if (g.tri & kTriBLower) kbeg = max(kbeg, n0);
     kbeg &= ~(kKS - 1);
-    __shared__ double As[kKS][68];
-    __shared__ double Bs[kKS][68];
+    // A and B slices; after the k loop the same LDS holds the 64 x 64 result
+    // tile (column-major, leading dimension 65) for the epilogue
+    __shared__ double smem[2 * kKS * 68];
+    double (*As)[68] = reinterpret_cast<double (*)[68]>(smem);
+    double (*Bs)[68] = reinterpret_cast<double (*)[68]>(smem + kKS * 68);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
@@ -137,21 +140,26 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
             __syncthreads();
         }
     }
+    // epilogue: the tile goes through LDS so that C is read and written one
+    // column (64 consecutive rows) per wavefront instruction; the k loop ended
+    // with a barrier, so the slices' LDS is free
+    double *Ct = smem;  // Ct[col * 65 + row]
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int gn = n0 + wn + 16 * j + lr;
-            if (gn >= g.N) continue;
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm + 16 * i + 4 * r + lk;
-                if (gm >= g.M || (g.uplo && gm < gn)) continue;
-                T *c = gC + (int64_t)gn * g.ldc + gm;
-                const double v = acc[i][j][r];
-                *c = (T)(g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * (double)*c);
-            }
-        }
+            for (int r = 0; r < 4; ++r)
+                Ct[(wn + 16 * j + lr) * 65 + wm + 16 * i + 4 * r + lk] = acc[i][j][r];
+    __syncthreads();
+    const int gm = m0 + lane;
+    for (int cc = wave; cc < 64; cc += kT / 64) {
+        const int gn = n0 + cc;
+        if (gm >= g.M || gn >= g.N || (g.uplo && gm < gn)) continue;
+        T *c = gC + (int64_t)gn * g.ldc + gm;
+        const double v = Ct[cc * 65 + lane];
+        *c = (T)(g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * (double)*c);
+    }
 }
 
 // One 32-column panel of the lower Cholesky factor of every submatrix: factor
