@@ -261,6 +261,29 @@ __global__ __launch_bounds__(kT) void k_transpose(const SubDesc *__restrict__ de
     }
 }
 
+// A(r, c) = A(c, r) for r < c, in place (the lower triangle is read, the
+// strict upper one written).  grid: (32x32 tile, submatrix)
+__global__ __launch_bounds__(kT) void k_mirror_lower(const SubDesc *__restrict__ desc,
+                                                     double *__restrict__ Aa) {
+    const SubDesc d = desc[blockIdx.y];
+    const int nt = (d.m + 31) / 32;
+    if ((int)blockIdx.x >= nt * nt) return;
+    const int r0 = (blockIdx.x % nt) * 32, c0 = (blockIdx.x / nt) * 32;
+    if (r0 > c0) return;  // tiles on or above the diagonal receive
+    double *a = Aa + d.off;
+    __shared__ double t[32][33];
+    for (int l = threadIdx.x; l < 1024; l += kT) {  // t[r][c] = A(c0 + r, r0 + c)
+        const int r = l & 31, c = l >> 5;
+        if (c0 + r < d.m && r0 + c < d.m) t[r][c] = a[(int64_t)(r0 + c) * d.ld + c0 + r];
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < 1024; l += kT) {  // A(r0 + r, c0 + c) = A(c0 + c, r0 + r)
+        const int r = l & 31, c = l >> 5;
+        if (r0 + r < c0 + c && r0 + r < d.m && c0 + c < d.m)
+            a[(int64_t)(c0 + c) * d.ld + r0 + r] = t[c][r];
+    }
+}
+
 __global__ __launch_bounds__(kT) void k_zero_upper(const SubDesc *__restrict__ desc,
                                                    double *__restrict__ Aa) {
     const SubDesc d = desc[blockIdx.x];
@@ -761,6 +784,14 @@ int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_
     const int nt = (max_m + 31) / 32;
     hipLaunchKernelGGL(k_transpose, dim3(nt * nt, nsm), dim3(kT), 0, ctx->stream, ddesc, in, out,
                        symmetrize ? 1 : 0);
+    PTB_CHECK_LAUNCH(ctx);
+    return PTBGPU_OK;
+}
+
+int mirror_lower_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, double *A) {
+    ProfScope prof_(ctx, "mirror");
+    const int nt = (max_m + 31) / 32;
+    hipLaunchKernelGGL(k_mirror_lower, dim3(nt * nt, nsm), dim3(kT), 0, ctx->stream, ddesc, A);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }

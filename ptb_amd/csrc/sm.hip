@@ -484,8 +484,8 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     PTB_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int) * nsm, st));
     PTB_TRY(ptb::potrf_batch(ctx, desc, ddesc, B, flags));
     // K = L^-1 H L^-T from Li = L^-1: Z = Li H, A = Z Li^T (the lower triangle,
-    // which the tridiagonalisation reads; both triangles for the mixed solve,
-    // whose refinement multiplies by K)
+    // which the tridiagonalisation reads; the mixed solve, whose refinement
+    // multiplies by K, mirrors it into the upper one)
     const bool mixed = opts.precision == PTBGPU_MIXED;
     PTB_TRY(ptb::inv_lower_batch(ctx, desc, ddesc, B, Li, X));
     {
@@ -494,14 +494,15 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             const SubDesc &d = desc[s];
             g1[s] = {Li + d.off, A + d.off, Z + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0,
                      kTriALower};
-            g2[s] = {Z + d.off, Li + d.off, A + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0, 0.0,
-                     mixed ? 0 : 1, kTriBUpper};
+            g2[s] = {Z + d.off, Li + d.off, A + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 1,
+                     kTriBUpper};
         }
         GemmDesc *dg1, *dg2;
         PTB_TRY(upload(ctx, "sm_gk1", g1, &dg1));
         PTB_TRY(upload(ctx, "sm_gk2", g2, &dg2));
         PTB_TRY(ptb::gemm_vbatch(ctx, false, false, dg1, nsm, max_m, max_m, "kform"));
         PTB_TRY(ptb::gemm_vbatch(ctx, false, true, dg2, nsm, max_m, max_m, "kform"));
+        if (mixed) PTB_TRY(ptb::mirror_lower_batch(ctx, ddesc, nsm, max_m, A));
     }
     // mixed precision: the tridiagonalisation on float storage, its reflectors
     // in V, and K (FP64) kept in A for the refinement below

@@ -99,6 +99,8 @@ int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
               const double *L, double *B, double *X, bool trans, const char *tag,
               const RhsLayout *rhs = nullptr);
 int zero_upper_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, double *A);
+// A's strict upper triangle from its lower one, in place
+int mirror_lower_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, double *A);
 int larft_all(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
               const double *V, const double *tau, QFactor &qf);
 // float copy of the arena blocks (same offsets and leading dimensions)
