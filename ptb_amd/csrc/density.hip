@@ -302,24 +302,33 @@ __global__ __launch_bounds__(kT) void k_refine_scatter(const SubDesc *__restrict
     if (threadIdx.x == 0) eig_all[d.mapoff + kj] = thn[d.mapoff + j];
 }
 
+// Packed P of every molecule zeroed.  grid: (blocks, molecule)
+__global__ __launch_bounds__(kT) void k_zero_packed(const MolDev *__restrict__ mols) {
+    const MolDev mo = mols[blockIdx.y];
+    const int64_t n = mo.ndim * (mo.ndim + 1) / 2;
+    for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+        mo.P[i] = 0.0;
+}
+
 // P(:, j) = U_o diag(w) U_o(loc_j, :)^T for the owned columns j, with U_o the
 // nocc occupied generalized eigenvectors (first columns of the arena block);
-// packed P (zeroed first) receives rows map(a) <= j, the reference's packsym of
-// P(map, comb) with no symmetrisation.  grid: (submatrix, chunk of kG owned
-// columns).
-__global__ __launch_bounds__(kT) void k_dmat_compact(const SubDesc *__restrict__ desc,
+// the packed P of the submatrix's molecule (zeroed first) receives rows
+// map(a) <= j, the reference's packsym of P(map, comb) with no symmetrisation.
+// grid: (submatrix, chunk of kG owned columns).
+__global__ __launch_bounds__(kT) void k_dmat_compact(const MolDev *__restrict__ mols,
+                                                     const SubDesc *__restrict__ desc,
                                                      const int *__restrict__ gcols,
                                                      const int *__restrict__ oloc,
                                                      const int *__restrict__ map_all,
                                                      const double *__restrict__ wocc,
                                                      const int *__restrict__ nocc,
-                                                     const double *__restrict__ Ua,
-                                                     double *__restrict__ P) {
+                                                     const double *__restrict__ Ua) {
     extern __shared__ double wsh[];  // [kG][n]
     const SubDesc d = desc[blockIdx.x];
     const int g0 = d.gbeg + blockIdx.y * kG;
     const int n = nocc[blockIdx.x];
     if (g0 >= d.gend || n == 0) return;
+    double *P = mols[d.mol].P;
     const int ng = min(kG, d.gend - g0);
     const double *U = Ua + d.off;
     const double *w = wocc + d.mapoff;
@@ -453,16 +462,21 @@ int refine_scatter(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc,
     return PTBGPU_OK;
 }
 
-int dmat_compact(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, int max_own,
-                 const int *dgcols, const int *doloc, const int *dmap, const double *wocc,
-                 const int *nocc, const double *U, double *P) {
+int dmat_compact(ptbgpu_ctx *ctx, const MolDev *mols, int nmol, int64_t max_packed,
+                 const SubDesc *ddesc, int nsm, int max_nocc, int max_own, const int *dgcols,
+                 const int *doloc, const int *dmap, const double *wocc, const int *nocc,
+                 const double *U) {
     ProfScope prof_(ctx, "dmat_scatter");
+    const int zb = (int)std::max<int64_t>(1, std::min<int64_t>((max_packed + kT - 1) / kT,
+                                                              (int64_t)ctx->num_cus * 8 / nmol + 1));
+    hipLaunchKernelGGL(k_zero_packed, dim3(zb, nmol), dim3(kT), 0, ctx->stream, mols);
+    PTB_CHECK_LAUNCH(ctx);
     if (max_nocc <= 0 || max_own <= 0) return PTBGPU_OK;
     const size_t shmem = sizeof(double) * (size_t)kG * max_nocc;
     if (shmem > 150 * 1024)
         return fail(ctx, PTBGPU_E_ARG, "too many occupied states for dmat (%d)", max_nocc);
     hipLaunchKernelGGL(k_dmat_compact, dim3(nsm, (max_own + kG - 1) / kG), dim3(kT), shmem,
-                       ctx->stream, ddesc, dgcols, doloc, dmap, wocc, nocc, U, P);
+                       ctx->stream, mols, ddesc, dgcols, doloc, dmap, wocc, nocc, U);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }

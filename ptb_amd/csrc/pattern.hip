@@ -17,15 +17,16 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// one block per column j of the packed upper triangle
-__global__ __launch_bounds__(kThreads) void k_pattern_bits(const double *__restrict__ H,
-                                                           const double *__restrict__ S,
-                                                           int64_t ndim, int64_t words,
-                                                           uint32_t *__restrict__ bits) {
+// grid: (column blocks, molecule)
+__global__ __launch_bounds__(kThreads) void k_pattern_bits(const MolDev *__restrict__ mols,
+                                                           uint32_t *__restrict__ bits_all) {
+    const MolDev mo = mols[blockIdx.y];
+    const int64_t ndim = mo.ndim, words = mo.words;
+    uint32_t *bits = bits_all + mo.bitoff;
     for (int64_t j = blockIdx.x; j < ndim; j += gridDim.x) {
         const int64_t c0 = j * (j + 1) / 2;
         for (int64_t i = threadIdx.x; i <= j; i += kThreads) {
-            const bool on = (i == j) || (H[c0 + i] > 0.0) || (S[c0 + i] > 0.0);
+            const bool on = (i == j) || (mo.H[c0 + i] > 0.0) || (mo.S[c0 + i] > 0.0);
             if (on) {
                 atomicOr(&bits[j * words + (i >> 5)], 1u << (i & 31));
                 if (i != j) atomicOr(&bits[i * words + (j >> 5)], 1u << (j & 31));
@@ -34,37 +35,49 @@ __global__ __launch_bounds__(kThreads) void k_pattern_bits(const double *__restr
     }
 }
 
-// union[s][w] = OR over owned columns of bits[col][w]; count[s] via atomics on
-// per-block popcounts (integer, exact)
-__global__ __launch_bounds__(kThreads) void k_group_union(const uint32_t *__restrict__ bits,
-                                                          int64_t words,
+// grid: (group, word blocks); the group's molecule gives its bitmap
+__global__ __launch_bounds__(kThreads) void k_group_union(const uint32_t *__restrict__ bits_all,
+                                                          const MolDev *__restrict__ mols,
+                                                          const int *__restrict__ smol,
                                                           const int *__restrict__ gptr,
                                                           const int *__restrict__ gcols,
-                                                          uint32_t *__restrict__ uni,
+                                                          uint32_t *__restrict__ uni_all,
                                                           int *__restrict__ mcount) {
     __shared__ int red[16];
     const int s = blockIdx.x;
+    const MolDev mo = mols[smol[s]];
+    const int64_t words = mo.words;
+    const uint32_t *bits = bits_all + mo.bitoff;
+    uint32_t *uni = uni_all + mo.unioff + (int64_t)(s - mo.sbeg) * words;
     const int g0 = gptr[s], g1 = gptr[s + 1];
     int pc = 0;
     for (int64_t w = (int64_t)blockIdx.y * kThreads + threadIdx.x; w < words;
          w += (int64_t)gridDim.y * kThreads) {
         uint32_t acc = 0;
         for (int g = g0; g < g1; ++g) acc |= bits[(int64_t)gcols[g] * words + w];
-        uni[(int64_t)s * words + w] = acc;
+        uni[w] = acc;
         pc += __popc(acc);
     }
     int t = block_sum(pc, red);
     if (threadIdx.x == 0 && t) atomicAdd(&mcount[s], t);
 }
 
-// map rows of submatrix s: exclusive popcount prefix over its union words
-__global__ __launch_bounds__(kThreads) void k_group_map(const uint32_t *__restrict__ uni,
-                                                        int64_t words,
+__device__ __forceinline__ const uint32_t *union_of(const uint32_t *uni_all, const MolDev &mo,
+                                                    int s) {
+    return uni_all + mo.unioff + (int64_t)(s - mo.sbeg) * mo.words;
+}
+
+// grid: (submatrix)
+__global__ __launch_bounds__(kThreads) void k_group_map(const uint32_t *__restrict__ uni_all,
+                                                        const MolDev *__restrict__ mols,
+                                                        const int *__restrict__ smol,
                                                         const SubDesc *__restrict__ desc,
                                                         int *__restrict__ map) {
     __shared__ int pref[kThreads + 1];
     const int s = blockIdx.x;
-    const uint32_t *u = uni + (int64_t)s * words;
+    const MolDev mo = mols[smol[s]];
+    const int64_t words = mo.words;
+    const uint32_t *u = union_of(uni_all, mo, s);
     int *out = map + desc[s].mapoff;
     // each thread owns a contiguous chunk of words
     const int64_t per = (words + kThreads - 1) / kThreads;
@@ -89,15 +102,17 @@ __global__ __launch_bounds__(kThreads) void k_group_map(const uint32_t *__restri
     }
 }
 
-// local index (rank in the union) of every owned column
-__global__ __launch_bounds__(kThreads) void k_owned_local(const uint32_t *__restrict__ uni,
-                                                          int64_t words,
+// grid: (submatrix)
+__global__ __launch_bounds__(kThreads) void k_owned_local(const uint32_t *__restrict__ uni_all,
+                                                          const MolDev *__restrict__ mols,
+                                                          const int *__restrict__ smol,
                                                           const int *__restrict__ gptr,
                                                           const int *__restrict__ gcols,
                                                           int nsm, int *__restrict__ oloc) {
     const int s = blockIdx.x;
     if (s >= nsm) return;
-    const uint32_t *u = uni + (int64_t)s * words;
+    const MolDev mo = mols[smol[s]];
+    const uint32_t *u = union_of(uni_all, mo, s);
     for (int g = gptr[s] + threadIdx.x; g < gptr[s + 1]; g += kThreads) {
         const int c = gcols[g];
         const int64_t wc = c >> 5;
@@ -108,12 +123,8 @@ __global__ __launch_bounds__(kThreads) void k_owned_local(const uint32_t *__rest
     }
 }
 
-// Dense Hsm, Ssm (full symmetric, column-major) of every submatrix.
-// grid.x: 32x32 tiles (a-tile, b-tile) with a-tile <= b-tile; grid.y: submatrix.
-// A tile reads packed column map[b] rows map[a] (map[a] <= map[b]) and writes the
-// tile and, through LDS, its transpose.
-__global__ __launch_bounds__(kThreads) void k_gather(const double *__restrict__ H,
-                                                     const double *__restrict__ S,
+// grid: (upper 32x32 tile pairs, submatrix)
+__global__ __launch_bounds__(kThreads) void k_gather(const MolDev *__restrict__ mols,
                                                      const SubDesc *__restrict__ desc,
                                                      const int *__restrict__ map,
                                                      double *__restrict__ Hd,
@@ -121,6 +132,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(const double *__restrict__ 
     __shared__ double th[32][33], ts[32][33];
     __shared__ int mr[32], mc[32];
     const SubDesc d = desc[blockIdx.y];
+    const double *H = mols[d.mol].H, *S = mols[d.mol].S;
     const int nt = (d.m + 31) / 32;
     const int ntri = nt * (nt + 1) / 2;
     for (int t = blockIdx.x; t < ntri; t += gridDim.x) {
@@ -175,47 +187,49 @@ __global__ __launch_bounds__(kThreads) void k_gather(const double *__restrict__ 
 
 namespace ptb {
 
-int pattern_bits(ptbgpu_ctx *ctx, const double *dH, const double *dS, int64_t ndim,
-                 int64_t words, uint32_t *bits) {
+int pattern_bits(ptbgpu_ctx *ctx, const MolDev *mols, int nmol, int64_t max_ndim,
+                 int64_t total_words, uint32_t *bits) {
     ProfScope prof_(ctx, "pattern");
-    PTB_HIP(ctx, hipMemsetAsync(bits, 0, sizeof(uint32_t) * (size_t)(ndim * words), ctx->stream));
-    int blocks = (int)std::min<int64_t>(ndim, (int64_t)ctx->num_cus * 16);
-    hipLaunchKernelGGL(k_pattern_bits, dim3(std::max(blocks, 1)), dim3(kThreads), 0, ctx->stream,
-                       dH, dS, ndim, words, bits);
+    PTB_HIP(ctx, hipMemsetAsync(bits, 0, sizeof(uint32_t) * (size_t)total_words, ctx->stream));
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(max_ndim, (int64_t)ctx->num_cus * 16 / nmol + 1));
+    hipLaunchKernelGGL(k_pattern_bits, dim3(blocks, nmol), dim3(kThreads), 0, ctx->stream, mols,
+                       bits);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
 
-int group_union(ptbgpu_ctx *ctx, const uint32_t *bits, int64_t words, const int *dgptr,
-                const int *dgcols, int nsm, uint32_t *uni, int *dmcount) {
+int group_union(ptbgpu_ctx *ctx, const uint32_t *bits, const MolDev *mols, const int *smol,
+                int64_t max_words, const int *dgptr, const int *dgcols, int nsm, uint32_t *uni,
+                int *dmcount) {
     ProfScope prof_(ctx, "group_union");
     PTB_HIP(ctx, hipMemsetAsync(dmcount, 0, sizeof(int) * nsm, ctx->stream));
-    int bx = (int)std::min<int64_t>((words + kThreads - 1) / kThreads, 64);
+    int bx = (int)std::min<int64_t>((max_words + kThreads - 1) / kThreads, 64);
     hipLaunchKernelGGL(k_group_union, dim3(nsm, std::max(bx, 1)), dim3(kThreads), 0, ctx->stream,
-                       bits, words, dgptr, dgcols, uni, dmcount);
+                       bits, mols, smol, dgptr, dgcols, uni, dmcount);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
 
-int group_map(ptbgpu_ctx *ctx, const uint32_t *uni, int64_t words, const SubDesc *ddesc,
-              int nsm, int *dmap, const int *dgptr, const int *dgcols, int *doloc) {
+int group_map(ptbgpu_ctx *ctx, const uint32_t *uni, const MolDev *mols, const int *smol,
+              const SubDesc *ddesc, int nsm, int *dmap, const int *dgptr, const int *dgcols,
+              int *doloc) {
     ProfScope prof_(ctx, "group_map");
-    hipLaunchKernelGGL(k_group_map, dim3(nsm), dim3(kThreads), 0, ctx->stream, uni, words, ddesc,
-                       dmap);
+    hipLaunchKernelGGL(k_group_map, dim3(nsm), dim3(kThreads), 0, ctx->stream, uni, mols, smol,
+                       ddesc, dmap);
     PTB_CHECK_LAUNCH(ctx);
-    hipLaunchKernelGGL(k_owned_local, dim3(nsm), dim3(kThreads), 0, ctx->stream, uni, words,
+    hipLaunchKernelGGL(k_owned_local, dim3(nsm), dim3(kThreads), 0, ctx->stream, uni, mols, smol,
                        dgptr, dgcols, nsm, doloc);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
 
-int gather(ptbgpu_ctx *ctx, const double *dH, const double *dS, const SubDesc *ddesc, int nsm,
-           int max_m, const int *dmap, double *Hd, double *Sd) {
+int gather(ptbgpu_ctx *ctx, const MolDev *mols, const SubDesc *ddesc, int nsm, int max_m,
+           const int *dmap, double *Hd, double *Sd) {
     ProfScope prof_(ctx, "gather");
     const int nt = (max_m + 31) / 32;
     const int ntri = nt * (nt + 1) / 2;
-    hipLaunchKernelGGL(k_gather, dim3(std::max(ntri, 1), nsm), dim3(kThreads), 0, ctx->stream, dH,
-                       dS, ddesc, dmap, Hd, Sd);
+    hipLaunchKernelGGL(k_gather, dim3(std::max(ntri, 1), nsm), dim3(kThreads), 0, ctx->stream, mols,
+                       ddesc, dmap, Hd, Sd);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }

@@ -241,7 +241,14 @@ int kmeans_groups(ptbgpu_ctx *ctx, int64_t ndim, int64_t n, const double *xyz,
     auto *uni = static_cast<uint32_t *>(ptb::ws(ctx, "km_union", sizeof(uint32_t) * total * words));
     auto *dcount = static_cast<int *>(ptb::ws(ctx, "km_count", sizeof(int) * total));
     if (!uni || !dcount) return PTBGPU_E_NOMEM;
-    PTB_TRY(ptb::group_union(ctx, bits, words, dcptr, dccols, total, uni, dcount));
+    // every candidate group belongs to the one molecule whose bitmap is `bits`
+    std::vector<MolDev> km{{nullptr, nullptr, nullptr, ndim, words, 0, 0, 0, 0}};
+    std::vector<int> kmol(total, 0);
+    MolDev *dkm;
+    int *dkmol;
+    PTB_TRY(upload(ctx, "km_mols", km, &dkm));
+    PTB_TRY(upload(ctx, "km_smol", kmol, &dkmol));
+    PTB_TRY(ptb::group_union(ctx, bits, dkm, dkmol, words, dcptr, dccols, total, uni, dcount));
     std::vector<int> m(total);
     PTB_HIP(ctx, hipMemcpyAsync(m.data(), dcount, sizeof(int) * total, hipMemcpyDeviceToHost,
                                 ctx->stream));
@@ -368,9 +375,19 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     }
     auto *bits = static_cast<uint32_t *>(ptb::ws(ctx, "sm_bits", sizeof(uint32_t) * bitoff[nmol]));
     if (!bits) return PTBGPU_E_NOMEM;
-    for (int i = 0; i < nmol; ++i)
-        PTB_TRY(ptb::pattern_bits(ctx, mols[i].dH, mols[i].dS, mols[i].ndim, words[i],
-                                  bits + bitoff[i]));
+    // device table of the molecules (union offsets and first submatrices are
+    // filled in once the groups are known)
+    std::vector<MolDev> mtab(nmol);
+    int64_t max_ndim = 0, max_words = 0, max_packed = 0;
+    for (int i = 0; i < nmol; ++i) {
+        mtab[i] = {mols[i].dH, mols[i].dS, mols[i].dP, mols[i].ndim, words[i], bitoff[i], 0, 0, 0};
+        max_ndim = std::max(max_ndim, mols[i].ndim);
+        max_words = std::max(max_words, words[i]);
+        max_packed = std::max(max_packed, mols[i].ndim * (mols[i].ndim + 1) / 2);
+    }
+    MolDev *dmols;
+    PTB_TRY(upload(ctx, "sm_mols", mtab, &dmols));
+    PTB_TRY(ptb::pattern_bits(ctx, dmols, nmol, max_ndim, bitoff[nmol], bits));
     // ---- combination: groups of every molecule; gcols holds molecule-local
     // column indices, gptr positions in the concatenated gcols
     std::vector<int> gptr{0}, gcols, sbeg(nmol + 1, 0);
@@ -396,13 +413,20 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     tm.mark(1, st);
     // ---- stats: unions, dimensions
     std::vector<int64_t> unioff(nmol + 1, 0);
-    for (int i = 0; i < nmol; ++i) unioff[i + 1] = unioff[i] + (int64_t)(sbeg[i + 1] - sbeg[i]) * words[i];
-    auto *uni = static_cast<uint32_t *>(ptb::ws(ctx, "sm_union", sizeof(uint32_t) * unioff[nmol]));
+    std::vector<int> smol(nsm);
+    for (int i = 0; i < nmol; ++i) {
+        unioff[i + 1] = unioff[i] + (int64_t)(sbeg[i + 1] - sbeg[i]) * words[i];
+        mtab[i].unioff = unioff[i];
+        mtab[i].sbeg = sbeg[i];
+        for (int s = sbeg[i]; s < sbeg[i + 1]; ++s) smol[s] = i;
+    }
+    int *dsmol;
+    PTB_TRY(upload(ctx, "sm_mols", mtab, &dmols));
+    PTB_TRY(upload(ctx, "sm_smol", smol, &dsmol));
+    auto *uni = static_cast<uint32_t *>(ptb::ws(ctx, "sm_union", sizeof(uint32_t) * std::max<int64_t>(unioff[nmol], 1)));
     auto *dcount = static_cast<int *>(ptb::ws(ctx, "sm_count", sizeof(int) * nsm));
     if (!uni || !dcount) return PTBGPU_E_NOMEM;
-    for (int i = 0; i < nmol; ++i)
-        PTB_TRY(ptb::group_union(ctx, bits + bitoff[i], words[i], dgptr + sbeg[i], dgcols,
-                                 sbeg[i + 1] - sbeg[i], uni + unioff[i], dcount + sbeg[i]));
+    PTB_TRY(ptb::group_union(ctx, bits, dmols, dsmol, max_words, dgptr, dgcols, nsm, uni, dcount));
     std::vector<int> mcount(nsm);
     PTB_HIP(ctx, hipMemcpyAsync(mcount.data(), dcount, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
     PTB_HIP(ctx, hipStreamSynchronize(st));
@@ -440,9 +464,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     auto *dmap = static_cast<int *>(ptb::ws(ctx, "sm_map", sizeof(int) * rows));
     auto *doloc = static_cast<int *>(ptb::ws(ctx, "sm_oloc", sizeof(int) * std::max<size_t>(gcols.size(), 1)));
     if (!dmap || !doloc) return PTBGPU_E_NOMEM;
-    for (int i = 0; i < nmol; ++i)
-        PTB_TRY(ptb::group_map(ctx, uni + unioff[i], words[i], ddesc + sbeg[i], sbeg[i + 1] - sbeg[i],
-                               dmap, dgptr + sbeg[i], dgcols, doloc));
+    PTB_TRY(ptb::group_map(ctx, uni, dmols, dsmol, ddesc, nsm, dmap, dgptr, dgcols, doloc));
     tm.mark(2, st);
     // ---- construction
     const size_t abytes = sizeof(double) * (size_t)arena;
@@ -476,9 +498,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     int64_t *dxoff;
     PTB_TRY(upload(ctx, "sm_xoff", xoff, &dxoff));
     if (!XA || !F || !Li) return PTBGPU_E_NOMEM;
-    for (int i = 0; i < nmol; ++i)
-        PTB_TRY(ptb::gather(ctx, mols[i].dH, mols[i].dS, ddesc + sbeg[i], sbeg[i + 1] - sbeg[i],
-                            mol_max_m[i], dmap, A, B));
+    PTB_TRY(ptb::gather(ctx, dmols, ddesc, nsm, max_m, dmap, A, B));
     tm.mark(3, st);
     // ---- eigendecomposition (every submatrix of every molecule)
     PTB_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int) * nsm, st));
@@ -602,12 +622,8 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         PTB_TRY(upload(ctx, "sm_gu", gu, &dgu));
         PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgu, nsm, max_m, max_nocc, "u"));
     }
-    for (int i = 0; i < nmol; ++i) {
-        const int64_t n = mols[i].ndim;
-        PTB_HIP(ctx, hipMemsetAsync(mols[i].dP, 0, sizeof(double) * (size_t)(n * (n + 1) / 2), st));
-        PTB_TRY(ptb::dmat_compact(ctx, ddesc + sbeg[i], sbeg[i + 1] - sbeg[i], max_nocc, max_own,
-                                  dgcols, doloc, dmap, wocc, nocc + sbeg[i], X, mols[i].dP));
-    }
+    PTB_TRY(ptb::dmat_compact(ctx, dmols, nmol, max_packed, ddesc, nsm, max_nocc, max_own, dgcols,
+                              doloc, dmap, wocc, nocc, X));
     tm.mark(7, st);
     std::vector<double> hb((size_t)kBisStride * nmol);
     PTB_HIP(ctx, hipMemcpyAsync(hb.data(), bis, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, st));

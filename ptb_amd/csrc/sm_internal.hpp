@@ -18,6 +18,20 @@ struct SubDesc {
     int pad;
 };
 
+// One molecule of a batch on the device: its packed matrices and where its
+// pattern bitmap, its submatrices' row unions and its submatrices begin.
+struct MolDev {
+    const double *H;
+    const double *S;
+    double *P;
+    int64_t ndim;
+    int64_t words;   // ceil(ndim / 32)
+    int64_t bitoff;  // first word of its ndim x words bitmap
+    int64_t unioff;  // first word of its submatrices' unions (words each)
+    int sbeg;        // first submatrix (or k-means candidate group)
+    int pad;
+};
+
 // Per-molecule bisection record: [0] mu, [1] nel, [2] logged passes, [3]
 // converged flag, then (mu, nel) of up to 100 passes.
 constexpr int kBisStride = 4 + 200;
@@ -73,15 +87,19 @@ struct RhsLayout {
 
 namespace ptb {
 
-// pattern.hip
-int pattern_bits(ptbgpu_ctx *ctx, const double *dH, const double *dS, int64_t ndim,
-                 int64_t words, uint32_t *bits);
-int group_union(ptbgpu_ctx *ctx, const uint32_t *bits, int64_t words, const int *dgptr,
-                const int *dgcols, int nsm, uint32_t *uni, int *dmcount);
-int group_map(ptbgpu_ctx *ctx, const uint32_t *uni, int64_t words, const SubDesc *ddesc,
-              int nsm, int *dmap, const int *dgptr, const int *dgcols, int *doloc);
-int gather(ptbgpu_ctx *ctx, const double *dH, const double *dS, const SubDesc *ddesc, int nsm,
-           int max_m, const int *dmap, double *Hd, double *Sd);
+// pattern.hip: one launch per phase for every molecule of a batch.  mols
+// (device) has nmol entries; smol (device) the molecule of every submatrix or
+// k-means candidate group; bitmaps and unions at the table's offsets.
+int pattern_bits(ptbgpu_ctx *ctx, const MolDev *mols, int nmol, int64_t max_ndim,
+                 int64_t total_words, uint32_t *bits);
+int group_union(ptbgpu_ctx *ctx, const uint32_t *bits, const MolDev *mols, const int *smol,
+                int64_t max_words, const int *dgptr, const int *dgcols, int nsm, uint32_t *uni,
+                int *dmcount);
+int group_map(ptbgpu_ctx *ctx, const uint32_t *uni, const MolDev *mols, const int *smol,
+              const SubDesc *ddesc, int nsm, int *dmap, const int *dgptr, const int *dgcols,
+              int *doloc);
+int gather(ptbgpu_ctx *ctx, const MolDev *mols, const SubDesc *ddesc, int nsm, int max_m,
+           const int *dmap, double *Hd, double *Sd);
 
 // dense.hip
 // f32: A, B, C are float arrays behind the descriptor's pointers (A B^T only)
@@ -167,8 +185,11 @@ int refine_shift(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, c
                  double *G);
 int refine_scatter(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, const int *nocc,
                    const int *oidx, const double *Xo, const double *thn, double *Z, double *eig);
-int dmat_compact(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, int max_own,
-                 const int *dgcols, const int *doloc, const int *dmap, const double *wocc,
-                 const int *nocc, const double *U, double *P);
+// packed P of every molecule zeroed, then the owned columns of every
+// submatrix written into its molecule's P
+int dmat_compact(ptbgpu_ctx *ctx, const MolDev *mols, int nmol, int64_t max_packed,
+                 const SubDesc *ddesc, int nsm, int max_nocc, int max_own, const int *dgcols,
+                 const int *doloc, const int *dmap, const double *wocc, const int *nocc,
+                 const double *U);
 
 }  // namespace ptb
