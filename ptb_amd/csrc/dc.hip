@@ -3,7 +3,7 @@
 // algorithm of LAPACK dstedc, which dsygvd uses).
 //
 // Tree: each submatrix's index range is halved recursively until a piece has at
-// most 64 rows.  A split at sp couples rows sp-1 and sp through beta = e[sp-1]:
+// most kDcLeaf rows.  A split at sp couples rows sp-1 and sp through beta = e[sp-1]:
 //   T = diag(T1 - |beta| e_last e_last^T, T2 - |beta| e_1 e_1^T) + |beta| u u^T,
 //   u = [e_last ; sgn(beta) e_1].
 // Leaves: implicit QL with Wilkinson-type shifts (tql2), one wavefront per leaf,
@@ -23,7 +23,6 @@
 
 namespace {
 
-constexpr int kLeaf = 64;
 constexpr int kMT = 256;
 constexpr double kEps = DBL_EPSILON;
 
@@ -32,8 +31,8 @@ __global__ __launch_bounds__(64) void k_dc_leaf(const SubDesc *__restrict__ desc
                                                 double *__restrict__ d_all,
                                                 const double *__restrict__ e_all,
                                                 double *__restrict__ Za) {
-    __shared__ double z[kLeaf][kLeaf + 1];
-    __shared__ double dl[kLeaf], el[kLeaf];
+    __shared__ double z[kDcLeaf][kDcLeaf + 1];
+    __shared__ double dl[kDcLeaf], el[kDcLeaf];
     const DcNode nd = leaves[blockIdx.x];
     const SubDesc ds = desc[nd.s];
     const int n = nd.hi - nd.lo, lane = threadIdx.x;
@@ -46,7 +45,8 @@ __global__ __launch_bounds__(64) void k_dc_leaf(const SubDesc *__restrict__ desc
         dl[lane] = dv;
         el[lane] = (lane < n - 1) ? e[nd.lo + lane] : 0.0;
     }
-    for (int c = 0; c < kLeaf; ++c) z[lane][c] = (lane == c) ? 1.0 : 0.0;
+    if (lane < kDcLeaf)
+        for (int c = 0; c < kDcLeaf; ++c) z[lane][c] = (lane == c) ? 1.0 : 0.0;
     __syncthreads();
     // every lane runs the scalar recurrence on the shared d/e (same values);
     // lane r applies each rotation to row r of z
@@ -144,6 +144,58 @@ __device__ __forceinline__ double block_max(double v, double *red) {
     return t;
 }
 
+// 1/x: the hardware reciprocal estimate refined by two Newton-Raphson steps
+// r <- r + r (1 - x r), each of which squares the relative error, so the
+// result is within about an ulp of the rounded quotient for normal finite x
+// (the divide sequence costs twice as many instructions in these O(K^2) loops)
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// dst(0:n) = src(0:n) for one column by one wavefront, 8 loads in flight per lane
+__device__ __forceinline__ void copy_column(double *__restrict__ dst,
+                                            const double *__restrict__ src, int n, int lane) {
+    constexpr int kU = 8;
+    for (int r0 = 0; r0 < n; r0 += kU * PTB_WAVE) {
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int r = r0 + u * PTB_WAVE + lane;
+            v[u] = r < n ? src[r] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int r = r0 + u * PTB_WAVE + lane;
+            if (r < n) dst[r] = v[u];
+        }
+    }
+}
+
+// exclusive prefix sum of one int per thread over the workgroup; *total gets
+// the sum of all (wsum: kMT / PTB_WAVE ints of LDS)
+__device__ int block_scan_excl(int v, int *wsum, int *total) {
+    const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
+    int inc = v;
+    for (int off = 1; off < PTB_WAVE; off <<= 1) {
+        const int o = __shfl_up(inc, off, PTB_WAVE);
+        if (lane >= off) inc += o;
+    }
+    __syncthreads();
+    if (lane == PTB_WAVE - 1) wsum[wid] = inc;
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int w = 0; w < kMT / PTB_WAVE; ++w) {
+        if (w < wid) base += wsum[w];
+        tot += wsum[w];
+    }
+    *total = tot;
+    return base + inc - v;
+}
+
 // One merge node per workgroup.  Scratch: W node block receives the sorted,
 // rotated, compacted eigenvector columns [nondeflated | deflated]; X node block
 // receives U (K x K); rot holds the deflation rotations (4 doubles each).
@@ -164,13 +216,13 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     double *dl = z + npow;          // K non-deflated poles
     double *w = dl + npow;          // K weights
     double *tau = w + npow;         // K root offsets
-    double *zh = tau + npow;        // K Loewner vector
-    double *cinv = zh + npow;       // K inverse column norms of U
-    int *ix = reinterpret_cast<int *>(cinv + npow);  // npow sort permutation
+    double *zh = tau + npow;        // K Loewner vector (rho w_i^2 before that)
+    int *ix = reinterpret_cast<int *>(zh + npow);  // npow sort permutation
     int *slot = ix + npow;          // n output slot of sorted position
     int *org = slot + npow;         // K origin pole of each root
     __shared__ double red[kMT / PTB_WAVE];
-    __shared__ int sK, sNr;
+    __shared__ int wsum[kMT / PTB_WAVE];
+    __shared__ int sK, sNr, sClose;
     __shared__ double srho;
 
     const double beta = e_all[ds.mapoff + nd.sp - 1];
@@ -211,8 +263,55 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     for (int j = threadIdx.x; j < n; j += kMT) mx = fmax(mx, fmax(fabs(Dk[j]), fabs(z[j])));
     const double tol = 8.0 * kEps * block_max(mx, red);
 
-    // deflation scan (sequential; rotations recorded, applied to columns later)
-    if (threadIdx.x == 0) {
+    // Deflation.  Type 1: rho |z_j| <= tol.  Type 2: neighbouring surviving
+    // poles so close that a Givens rotation zeroes one weight (|t c s| <= tol);
+    // a rotation changes the weight carried to the next test, so that scan is
+    // sequential.  When no pair of neighbouring survivors passes the test on
+    // the unrotated values the scan rotates nothing and its outcome is the
+    // type-1 split, formed here in parallel from a prefix count of survivors:
+    // survivor j gets slot = survivors before j, deflated j gets
+    // slot = #survivors + deflated before j (the scan's order).
+    {
+        const int chunk = (n + kMT - 1) / kMT;
+        const int jb = min(n, (int)threadIdx.x * chunk), je = min(n, jb + chunk);
+        int cnt = 0;
+        for (int j = jb; j < je; ++j) cnt += (rho * fabs(z[j]) > tol);
+        int nc;
+        int pos = block_scan_excl(cnt, wsum, &nc);
+        if (threadIdx.x == 0) sClose = 0;
+        for (int j = jb; j < je; ++j) {
+            if (rho * fabs(z[j]) > tol) {
+                org[pos] = j;  // survivors in order (org is free until the roots)
+                slot[j] = pos++;
+            } else {
+                slot[j] = nc + (j - pos);
+            }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x + 1; q < nc; q += kMT) {
+            const int pj = org[q - 1], j = org[q];
+            double s = z[pj], c = z[j];
+            const double tv = hypot(c, s);
+            const double t = Dk[j] - Dk[pj];
+            c /= tv;
+            s = -s / tv;
+            if (fabs(t * c * s) <= tol) sClose = 1;
+        }
+        __syncthreads();
+        if (!sClose) {
+            for (int q = threadIdx.x; q < nc; q += kMT) {
+                dl[q] = Dk[org[q]];
+                w[q] = z[org[q]];
+            }
+            if (threadIdx.x == 0) {
+                sK = nc;
+                sNr = 0;
+            }
+        }
+        __syncthreads();
+    }
+    // sequential scan (rotations recorded, applied to the columns below)
+    if (sClose && threadIdx.x == 0) {
         int K = 0, nd_ = 0, nr = 0, pj = -1;
         // slot: >= 0 final slot for non-deflated (assigned after counting)
         // first pass marks: slot[j] = -1 deflated, -2 non-deflated
@@ -262,11 +361,13 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     }
     __syncthreads();
     const int K = sK, nr = sNr;
-    // sorted, compacted eigenvector columns into W: W(:, slot[j]) = Z(:, lo+ix[j])
-    for (int l = threadIdx.x; l < n * n; l += kMT) {
-        const int r = l % n, j = l / n;
-        W[(int64_t)(nd.lo + slot[j]) * ld + nd.lo + r] = Z[(int64_t)(nd.lo + ix[j]) * ld + nd.lo + r];
-    }
+    const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
+    constexpr int kWaves = kMT / PTB_WAVE;
+    // sorted, compacted eigenvector columns into W: W(:, slot[j]) = Z(:, lo+ix[j]),
+    // one column per wavefront
+    for (int j = wid; j < n; j += kWaves)
+        copy_column(W + (int64_t)(nd.lo + slot[j]) * ld + nd.lo,
+                    Z + (int64_t)(nd.lo + ix[j]) * ld + nd.lo, n, lane);
     __syncthreads();
     // deflation rotations, in order, row-parallel
     for (int r = threadIdx.x; r < n; r += kMT) {
@@ -281,12 +382,14 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     }
     __syncthreads();
     // deflated columns and eigenvalues go straight to their final place
-    for (int l = threadIdx.x; l < n * (n - K); l += kMT) {
-        const int r = l % n, c = K + l / n;
-        Z[(int64_t)(nd.lo + c) * ld + nd.lo + r] = W[(int64_t)(nd.lo + c) * ld + nd.lo + r];
-    }
+    for (int c = K + wid; c < n; c += kWaves)
+        copy_column(Z + (int64_t)(nd.lo + c) * ld + nd.lo, W + (int64_t)(nd.lo + c) * ld + nd.lo,
+                    n, lane);
     for (int j = threadIdx.x; j < n; j += kMT)
         if (slot[j] >= K) D[nd.lo + slot[j]] = Dk[j];
+    __syncthreads();
+    double *w2 = zh;
+    for (int i = threadIdx.x; i < K; i += kMT) w2[i] = rho * w[i] * w[i];
     __syncthreads();
     // secular roots, one thread per root
     for (int j = threadIdx.x; j < K; j += kMT) {
@@ -295,7 +398,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         if (j < K - 1) {
             const double mid = 0.5 * (dl[j + 1] - dl[j]);
             double f = 1.0;
-            for (int i = 0; i < K; ++i) f += rho * w[i] * w[i] / ((dl[i] - dl[j]) - mid);
+            for (int i = 0; i < K; ++i) f += w2[i] * rcp64((dl[i] - dl[j]) - mid);
             if (f >= 0.0) {
                 o = j;
                 lo = 0.0;
@@ -327,14 +430,14 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         for (int it = 0; it < 200; ++it) {
             double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
             for (int i = 0; i <= j; ++i) {
-                const double r = 1.0 / ((dl[i] - dorg) - t);
-                const double q = rho * w[i] * w[i] * r;
+                const double r = rcp64((dl[i] - dorg) - t);
+                const double q = w2[i] * r;
                 psi += q;
                 dpsi += q * r;
             }
             for (int i = j + 1; i < K; ++i) {
-                const double r = 1.0 / ((dl[i] - dorg) - t);
-                const double q = rho * w[i] * w[i] * r;
+                const double r = rcp64((dl[i] - dorg) - t);
+                const double q = w2[i] * r;
                 phi += q;
                 dphi += q * r;
             }
@@ -384,29 +487,26 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         for (int j = 0; j < K; ++j) {
             if (j == i) continue;
             const double dlt = (di - dl[org[j]]) - tau[j];  // d_i - lam_j
-            prod *= (-dlt) / (dl[j] - di);
+            prod *= (-dlt) * rcp64(dl[j] - di);
         }
         zh[i] = copysign(sqrt(fmax(prod, 0.0) / rho), w[i]);
     }
     __syncthreads();
-    // U(:, j) = zhat ./ (d - lam_j), normalised; eigenvalues lam_j.  Column
-    // norms first (one thread per column), then U written with consecutive
-    // threads on consecutive rows.
-    for (int j = threadIdx.x; j < K; j += kMT) {
+    // U(:, j) = zhat ./ (d - lam_j), normalised; eigenvalues lam_j.  One
+    // column per wavefront: its norm, then the column (lanes on rows).
+    for (int j = wid; j < K; j += kWaves) {
         const double dorg = dl[org[j]], tj = tau[j];
         double nrm = 0.0;
-        for (int i = 0; i < K; ++i) {
-            const double u = zh[i] / ((dl[i] - dorg) - tj);
+        for (int i = lane; i < K; i += PTB_WAVE) {
+            const double u = zh[i] * rcp64((dl[i] - dorg) - tj);
             nrm += u * u;
         }
-        cinv[j] = 1.0 / sqrt(nrm);
-        D[nd.lo + j] = dorg + tj;
+        const double ci = 1.0 / sqrt(wave_sum(nrm));
+        double *xc = X + (int64_t)(nd.lo + j) * ld + nd.lo;
+        for (int i = lane; i < K; i += PTB_WAVE) xc[i] = ci * zh[i] * rcp64((dl[i] - dorg) - tj);
+        if (lane == 0) D[nd.lo + j] = dorg + tj;
     }
     __syncthreads();
-    for (int l = threadIdx.x; l < K * K; l += kMT) {
-        const int i = l % K, j = l / K;
-        X[(int64_t)(nd.lo + j) * ld + nd.lo + i] = cinv[j] * zh[i] / ((dl[i] - dl[org[j]]) - tau[j]);
-    }
     if (threadIdx.x == 0) {
         g->A = W + (int64_t)nd.lo * ld + nd.lo;
         g->B = X + (int64_t)nd.lo * ld + nd.lo;
@@ -444,7 +544,7 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
     if (count <= 0) return PTBGPU_OK;
     int npow = 1;
     while (npow < max_n) npow <<= 1;
-    const size_t shmem = (size_t)npow * (7 * sizeof(double) + 3 * sizeof(int));
+    const size_t shmem = (size_t)npow * (6 * sizeof(double) + 3 * sizeof(int));
     if (shmem > 150 * 1024)
         return fail(ctx, PTBGPU_E_ARG, "merge of %d rows exceeds the LDS budget", max_n);
     {

@@ -25,8 +25,6 @@
 
 namespace {
 
-constexpr int kLeafMax = 16;
-
 struct Timer {
     hipEvent_t ev[PTBGPU_T_NSLOT + 1];
     bool ok = false;
@@ -189,7 +187,7 @@ void cluster_columns(const std::vector<int> &z, int nsm, int64_t ndim, const int
 
 void build_tree(int s, int lo, int hi, std::vector<DcNode> &leaves,
                 std::vector<std::vector<DcNode>> &levels, int &height) {
-    if (hi - lo <= kLeafMax) {
+    if (hi - lo <= kDcLeaf) {
         leaves.push_back({s, lo, hi, hi});
         height = 0;
         return;

@@ -65,6 +65,10 @@ struct DcNode {
     int lo, sp, hi;
 };
 
+// largest leaf of the divide-and-conquer tree (rows of the tridiagonal piece
+// that the leaf kernel diagonalises directly)
+constexpr int kDcLeaf = 16;
+
 // Blocked Householder factor of a batch: reflectors in the columns of V (the
 // arena of the reduced matrices), T factors of all 32-reflector blocks.
 struct QFactor {
