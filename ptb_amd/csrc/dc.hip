@@ -156,16 +156,18 @@ __device__ __forceinline__ double rcp64(double x) {
     return fma(r, e, r);
 }
 
-// dst(0:n) = src(0:n) for one column by one wavefront, 8 loads in flight per lane
+// dst(0:n) = src(0:n) for one column by one wavefront, 8 loads in flight per
+// lane; rows outside [rb, re) are known to be zero and are written, not read
 __device__ __forceinline__ void copy_column(double *__restrict__ dst,
-                                            const double *__restrict__ src, int n, int lane) {
+                                            const double *__restrict__ src, int n, int lane,
+                                            int rb = 0, int re = 1 << 30) {
     constexpr int kU = 8;
     for (int r0 = 0; r0 < n; r0 += kU * PTB_WAVE) {
         double v[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int r = r0 + u * PTB_WAVE + lane;
-            v[u] = r < n ? src[r] : 0.0;
+            v[u] = (r < n && r >= rb && r < re) ? src[r] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -232,12 +234,20 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     double *D = D_all + ds.mapoff;
     double *rot = rot_all + 4 * (int64_t)(ds.mapoff + nd.lo);
     GemmDesc *g = gd + blockIdx.x;
+    const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
+    constexpr int kWaves = kMT / PTB_WAVE;
 
     if (rho == 0.0) {  // decoupled: children are already eigenpairs of T
         if (threadIdx.x == 0) {
             g->M = 0;
             g->N = 0;
             g->K = 0;
+        }
+        // the node block is diag(Z1, Z2): zero its off-diagonal blocks
+        for (int c = wid; c < n; c += kWaves) {
+            double *zc = Z + (int64_t)(nd.lo + c) * ld + nd.lo;
+            const int rb = c < n1 ? n1 : 0, re = c < n1 ? n : n1;
+            for (int r = rb + lane; r < re; r += PTB_WAVE) zc[r] = 0.0;
         }
         return;
     }
@@ -361,13 +371,16 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     }
     __syncthreads();
     const int K = sK, nr = sNr;
-    const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
-    constexpr int kWaves = kMT / PTB_WAVE;
     // sorted, compacted eigenvector columns into W: W(:, slot[j]) = Z(:, lo+ix[j]),
-    // one column per wavefront
-    for (int j = wid; j < n; j += kWaves)
+    // one column per wavefront.  A child's column is nonzero only on the
+    // child's rows; the rest of the node block is written as zeros, never
+    // read, so the arena needs no clearing beforehand.
+    for (int j = wid; j < n; j += kWaves) {
+        const int c = ix[j];
         copy_column(W + (int64_t)(nd.lo + slot[j]) * ld + nd.lo,
-                    Z + (int64_t)(nd.lo + ix[j]) * ld + nd.lo, n, lane);
+                    Z + (int64_t)(nd.lo + c) * ld + nd.lo, n, lane, c < n1 ? 0 : n1,
+                    c < n1 ? n1 : n);
+    }
     __syncthreads();
     // deflation rotations, in order, row-parallel
     for (int r = threadIdx.x; r < n; r += kMT) {
@@ -540,7 +553,7 @@ int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int 
 
 int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, int count,
                    int max_n, const double *e, double *D, double *Z, double *W, double *X,
-                   double *rot, GemmDesc *dgemm) {
+                   double *rot, GemmDesc *dgemm, const TileList &tiles) {
     if (count <= 0) return PTBGPU_OK;
     int npow = 1;
     while (npow < max_n) npow <<= 1;
@@ -553,7 +566,7 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
                            npow, e, D, Z, W, X, rot, dgemm);
         PTB_CHECK_LAUNCH(ctx);
     }
-    return gemm_vbatch(ctx, false, false, dgemm, count, max_n, max_n, "dc");
+    return tiles.launch(ctx, 0, false, false, dgemm, "dc");
 }
 
 }  // namespace ptb

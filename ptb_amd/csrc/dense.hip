@@ -41,22 +41,37 @@ constexpr int kKS = 32;
 // T*, leading dimensions in elements of T); products and sums are FP64 either
 // way, so float storage halves the traffic of the FP32 tridiagonalisation's
 // updates without changing the arithmetic.
+//
+// With a tile list (work != nullptr) workgroup b computes the tile work[b],
+// already dealt to XCDs by the host; the grid is the list's length.
 template <bool TA, bool TB, typename T>
 __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ descs, int count,
-                                               int tiles) {
-    const int64_t lin = blockIdx.x;
-    const int64_t slot = lin >> 3;
-    const int gi = (int)((slot / tiles) * 8 + (lin & 7));
-    const int tile = (int)(slot % tiles);
-    if (gi >= count) return;
+                                               int tiles, const GemmTile *__restrict__ work) {
+    int gi, m0, n0;
+    if (work) {
+        const GemmTile w = work[blockIdx.x];
+        if (w.g < 0) return;
+        gi = w.g;
+        m0 = (w.t & 0xffff) * 64;
+        n0 = (w.t >> 16) * 64;
+    } else {
+        const int64_t lin = blockIdx.x;
+        const int64_t slot = lin >> 3;
+        gi = (int)((slot / tiles) * 8 + (lin & 7));
+        if (gi >= count) return;
+        const int tile = (int)(slot % tiles);
+        const int tM = (descs[gi].M + 63) / 64, tN = (descs[gi].N + 63) / 64;
+        if (tile >= tM * tN) return;
+        m0 = (tile % max(tM, 1)) * 64;
+        n0 = (tile / max(tM, 1)) * 64;
+        if (descs[gi].uplo && n0 > m0 + 63) return;  // tile strictly above the diagonal
+    }
     const GemmDesc g = descs[gi];
+    // (a list may hold tiles of a bound on a shape the device fills in)
+    if (m0 >= g.M || n0 >= g.N) return;
     const T *__restrict__ gA = reinterpret_cast<const T *>(g.A);
     const T *__restrict__ gB = reinterpret_cast<const T *>(g.B);
     T *gC = reinterpret_cast<T *>(g.C);
-    const int tilesM = (g.M + 63) / 64, tilesN = (g.N + 63) / 64;
-    if (g.M <= 0 || g.N <= 0 || tile >= tilesM * tilesN) return;
-    const int m0 = (tile % tilesM) * 64, n0 = (tile / tilesM) * 64;
-    if (g.uplo && n0 > m0 + 63) return;  // tile strictly above the diagonal
     // k range of this tile where triangular factors can be nonzero, aligned
     // down to the slice depth: the other triangle of a triangular factor must
     // hold zeros (the few of its elements inside the range add nothing)
@@ -622,9 +637,27 @@ int inv_lower_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
         auto *dg1 = static_cast<GemmDesc *>(upload_table(ctx, n1.c_str(), g1.data(), sizeof(GemmDesc) * g1.size()));
         auto *dg2 = static_cast<GemmDesc *>(upload_table(ctx, n2.c_str(), g2.data(), sizeof(GemmDesc) * g2.size()));
         if (!dg1 || !dg2) return PTBGPU_E_NOMEM;
-        PTB_TRY(gemm_vbatch(ctx, false, false, dg1, (int)g1.size(), h, h, "linv"));
-        PTB_TRY(gemm_vbatch(ctx, false, false, dg2, (int)g2.size(), h, h, "linv"));
+        TileList t1, t2;
+        t1.add(g1.data(), (int)g1.size());
+        t2.add(g2.data(), (int)g2.size());
+        PTB_TRY(t1.upload(ctx, ("linv_w1_" + std::to_string(level)).c_str()));
+        PTB_TRY(t2.upload(ctx, ("linv_w2_" + std::to_string(level)).c_str()));
+        PTB_TRY(t1.launch(ctx, 0, false, false, dg1, "linv"));
+        PTB_TRY(t2.launch(ctx, 0, false, false, dg2, "linv"));
     }
+    return PTBGPU_OK;
+}
+
+static int launch_gemm(ptbgpu_ctx *ctx, bool ta, bool tb, bool f32, dim3 grid, const GemmDesc *ddesc,
+                       int count, int tiles, const GemmTile *work) {
+    if (f32) {  // float storage: only the tridiagonal update's form is used
+        if (ta || !tb) return fail(ctx, PTBGPU_E_ARG, "float-storage GEMM supports A B^T only");
+        hipLaunchKernelGGL((k_gemm<false, true, float>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
+    } else if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
+    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
+    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
+    else hipLaunchKernelGGL((k_gemm<true, true, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
+    PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
 
@@ -638,16 +671,52 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
     const int64_t total = groups8 * 8 * tiles;
     if (total > 0x7fffffff) return fail(ctx, PTBGPU_E_ARG, "GEMM batch too large (%lld workgroups)",
                                         (long long)total);
-    dim3 grid((unsigned)total);
-    if (f32) {  // float storage: only the tridiagonal update's form is used
-        if (ta || !tb) return fail(ctx, PTBGPU_E_ARG, "float-storage GEMM supports A B^T only");
-        hipLaunchKernelGGL((k_gemm<false, true, float>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
-    } else if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
-    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
-    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
-    else hipLaunchKernelGGL((k_gemm<true, true, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles);
-    PTB_CHECK_LAUNCH(ctx);
-    return PTBGPU_OK;
+    return launch_gemm(ctx, ta, tb, f32, dim3((unsigned)total), ddesc, count, tiles, nullptr);
+}
+
+int gemm_tiles(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, const GemmTile *dwork,
+               int64_t nwork, const char *site, bool f32) {
+    ProfScope prof_(ctx, "gemm");
+    ProfScope sprof_(ctx, site ? (std::string("gemm_") + site).c_str() : "gemm_other");
+    if (nwork <= 0) return PTBGPU_OK;
+    if (nwork > 0x7fffffff) return fail(ctx, PTBGPU_E_ARG, "GEMM tile list too long (%lld)", (long long)nwork);
+    return launch_gemm(ctx, ta, tb, f32, dim3((unsigned)nwork), ddesc, 0, 1, dwork);
+}
+
+int TileList::add(const GemmDesc *h, int count) {
+    std::vector<GemmTile> q[8];
+    int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < count; ++i) {
+        const GemmDesc &g = h[i];
+        if (g.M <= 0 || g.N <= 0) continue;
+        const int tM = (g.M + 63) / 64, tN = (g.N + 63) / 64;
+        int x = 0;
+        for (int y = 1; y < 8; ++y)
+            if (load[y] < load[x]) x = y;
+        for (int tn = 0; tn < tN; ++tn)
+            for (int tm = g.uplo ? tn : 0; tm < tM; ++tm) q[x].push_back({i, tm | (tn << 16)});
+        load[x] = (int64_t)q[x].size();
+    }
+    size_t len = 0;
+    for (int x = 0; x < 8; ++x) len = std::max(len, q[x].size());
+    for (size_t k = 0; k < len; ++k)
+        for (int x = 0; x < 8; ++x) work_.push_back(k < q[x].size() ? q[x][k] : GemmTile{-1, 0});
+    start_.push_back((int64_t)work_.size());
+    return (int)start_.size() - 2;
+}
+
+int TileList::upload(ptbgpu_ctx *ctx, const char *name) {
+    if (work_.empty()) return PTBGPU_OK;
+    dwork_ = static_cast<const GemmTile *>(upload_table(ctx, name, work_.data(), sizeof(GemmTile) * work_.size()));
+    return dwork_ ? PTBGPU_OK : PTBGPU_E_NOMEM;
+}
+
+int TileList::launch(ptbgpu_ctx *ctx, int i, bool ta, bool tb, const GemmDesc *ddesc,
+                     const char *site, bool f32) const {
+    const int64_t n = start_[i + 1] - start_[i];
+    if (n == 0) return PTBGPU_OK;
+    if (!dwork_) return fail(ctx, PTBGPU_E_ARG, "GEMM tile list launched before upload");
+    return gemm_tiles(ctx, ta, tb, ddesc, dwork_ + start_[i], n, site, f32);
 }
 
 // X = L^-1 B (trans=false) or L^-T B (trans=true), all m columns, blocked by
@@ -761,6 +830,13 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
     auto *dgs = static_cast<GemmDesc *>(upload_table(ctx, "potrf_gs", gs.data(), sizeof(GemmDesc) * gs.size()));
     auto *dgb = static_cast<GemmDesc *>(upload_table(ctx, "potrf_gb", gb.data(), sizeof(GemmDesc) * gb.size()));
     if (!dgs || !dgb) return PTBGPU_E_NOMEM;
+    TileList ts, tb;
+    for (int b = 0; b < nblk; ++b) {
+        ts.add(gs.data() + ss[b], ss[b + 1] - ss[b]);
+        tb.add(gb.data() + sb[b], sb[b + 1] - sb[b]);
+    }
+    PTB_TRY(ts.upload(ctx, "potrf_ws"));
+    PTB_TRY(tb.upload(ctx, "potrf_wb"));
     auto panel = [&](int p) -> int {
         ProfScope kprof_(ctx, "potrf_panel");
         hipLaunchKernelGGL(k_potrf_panel, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, A, flags, p);
@@ -769,11 +845,9 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
     };
     for (int b = 0; b < nblk; ++b) {
         PTB_TRY(panel(2 * b));
-        int c = ss[b + 1] - ss[b];
-        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dgs + ss[b], c, max_m, 32, "potrf"));
+        PTB_TRY(ts.launch(ctx, b, false, true, dgs + ss[b], "potrf"));
         if (2 * b + 1 < npanel) PTB_TRY(panel(2 * b + 1));
-        c = sb[b + 1] - sb[b];
-        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dgb + sb[b], c, max_m, max_m, "potrf"));
+        PTB_TRY(tb.launch(ctx, b, false, true, dgb + sb[b], "potrf"));
     }
     return PTBGPU_OK;
 }
@@ -920,12 +994,21 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
     GemmDesc *dg2 = static_cast<GemmDesc *>(upload_table(ctx, ("q_g2_" + t).c_str(), g2.data(), gb));
     GemmDesc *dg3 = static_cast<GemmDesc *>(upload_table(ctx, ("q_g3_" + t).c_str(), g3.data(), gb));
     if (!dg1 || !dg2 || !dg3) return PTBGPU_E_NOMEM;
+    TileList t1, t2, t3;
     for (int q = 0; q < qf.maxblk64; ++q) {
         const int c = start[q + 1] - start[q];
-        if (c == 0) continue;
-        PTB_TRY(gemm_vbatch(ctx, true, false, dg1 + start[q], c, 64, max_c, "q"));
-        PTB_TRY(gemm_vbatch(ctx, trans, false, dg2 + start[q], c, 64, max_c, "q"));
-        PTB_TRY(gemm_vbatch(ctx, false, false, dg3 + start[q], c, max_m, max_c, "q"));
+        t1.add(g1.data() + start[q], c);
+        t2.add(g2.data() + start[q], c);
+        t3.add(g3.data() + start[q], c);
+    }
+    PTB_TRY(t1.upload(ctx, ("q_w1_" + t).c_str()));
+    PTB_TRY(t2.upload(ctx, ("q_w2_" + t).c_str()));
+    PTB_TRY(t3.upload(ctx, ("q_w3_" + t).c_str()));
+    for (int q = 0; q < qf.maxblk64; ++q) {
+        if (start[q + 1] == start[q]) continue;
+        PTB_TRY(t1.launch(ctx, q, true, false, dg1 + start[q], "q"));
+        PTB_TRY(t2.launch(ctx, q, trans, false, dg2 + start[q], "q"));
+        PTB_TRY(t3.launch(ctx, q, false, false, dg3 + start[q], "q"));
     }
     return PTBGPU_OK;
 }

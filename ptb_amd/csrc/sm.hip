@@ -303,33 +303,36 @@ int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
     ptb::ProfScope prof_(ctx, "refine_all");
     PTB_TRY(ptb::apply_q(ctx, desc, qf, X, off, hn, false, "r1"));  // X = Q Zo
     std::vector<GemmDesc> g(nsm);
-    auto run = [&](const char *name, bool ta, int mm, int nn, auto make) -> int {
+    auto run = [&](const char *name, bool ta, auto make) -> int {
         for (int s = 0; s < nsm; ++s) g[s] = make(desc[s], hn[s]);
         GemmDesc *dg;
         PTB_TRY(upload(ctx, name, g, &dg));
-        return ptb::gemm_vbatch(ctx, ta, false, dg, nsm, mm, nn, "refine");
+        ptb::TileList tl;
+        tl.add(g.data(), nsm);
+        PTB_TRY(tl.upload(ctx, (std::string(name) + "_w").c_str()));
+        return tl.launch(ctx, 0, ta, false, dg, "refine");
     };
     // R = K (Q Zo)
-    PTB_TRY(run("rf_g1", false, max_m, max_n, [&](const SubDesc &d, int n) {
+    PTB_TRY(run("rf_g1", false, [&](const SubDesc &d, int n) {
         return GemmDesc{K + d.off, X + d.off, R + d.off, d.m, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
     PTB_TRY(ptb::apply_q(ctx, desc, qf, R, off, hn, true, "r2"));  // R = Q^T K Q Zo
     PTB_TRY(ptb::refine_resid(ctx, ddesc, nsm, max_n, nocc, oidx, eig, W, R));
     // X = Z^T R, then E in place and the Rayleigh quotients
-    PTB_TRY(run("rf_g2", true, max_m, max_n, [&](const SubDesc &d, int n) {
+    PTB_TRY(run("rf_g2", true, [&](const SubDesc &d, int n) {
         return GemmDesc{Z + d.off, R + d.off, X + d.off, d.m, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
     PTB_TRY(ptb::refine_corr(ctx, ddesc, nsm, max_n, nocc, oidx, eig, bis, X, thn));
     // W = Zo + Z E
-    PTB_TRY(run("rf_g3", false, max_m, max_n, [&](const SubDesc &d, int n) {
+    PTB_TRY(run("rf_g3", false, [&](const SubDesc &d, int n) {
         return GemmDesc{Z + d.off, X + d.off, W + d.off, d.m, n, d.m, d.ld, d.ld, d.ld, 1.0, 1.0, 0, 0};
     }));
     // R = (3 I - W^T W) / 2, X = W R
-    PTB_TRY(run("rf_g4", true, max_n, max_n, [&](const SubDesc &d, int n) {
+    PTB_TRY(run("rf_g4", true, [&](const SubDesc &d, int n) {
         return GemmDesc{W + d.off, W + d.off, R + d.off, n, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
     PTB_TRY(ptb::refine_shift(ctx, ddesc, nsm, max_n, nocc, R));
-    PTB_TRY(run("rf_g5", false, max_m, max_n, [&](const SubDesc &d, int n) {
+    PTB_TRY(run("rf_g5", false, [&](const SubDesc &d, int n) {
         return GemmDesc{W + d.off, R + d.off, X + d.off, d.m, n, n, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
     return ptb::refine_scatter(ctx, ddesc, nsm, max_n, nocc, oidx, X, thn, Z, eig);
@@ -518,8 +521,13 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         GemmDesc *dg1, *dg2;
         PTB_TRY(upload(ctx, "sm_gk1", g1, &dg1));
         PTB_TRY(upload(ctx, "sm_gk2", g2, &dg2));
-        PTB_TRY(ptb::gemm_vbatch(ctx, false, false, dg1, nsm, max_m, max_m, "kform"));
-        PTB_TRY(ptb::gemm_vbatch(ctx, false, true, dg2, nsm, max_m, max_m, "kform"));
+        ptb::TileList t1, t2;
+        t1.add(g1.data(), nsm);
+        t2.add(g2.data(), nsm);
+        PTB_TRY(t1.upload(ctx, "sm_wk1"));
+        PTB_TRY(t2.upload(ctx, "sm_wk2"));
+        PTB_TRY(t1.launch(ctx, 0, false, false, dg1, "kform"));
+        PTB_TRY(t2.launch(ctx, 0, false, true, dg2, "kform"));
         if (mixed) PTB_TRY(ptb::mirror_lower_batch(ctx, ddesc, nsm, max_m, A));
     }
     // mixed precision: the tridiagonalisation on float storage, its reflectors
@@ -544,6 +552,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     PTB_TRY(ptb::xall_build(ctx, ddesc, nsm, max_own, doloc, B, Li, dxoff, XA));
     PTB_TRY(ptb::apply_q(ctx, desc, qf, XA, xoff, c2, true, "x"));           // Q^T XA
     GemmDesc *dgf;
+    ptb::TileList tf;
     {
         std::vector<GemmDesc> gf(nsm);
         for (int s = 0; s < nsm; ++s) {
@@ -551,7 +560,9 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             gf[s] = {Z + d.off, XA + xoff[s], F + xoff[s], d.m, c2[s], d.m, d.ld, d.ld, d.ld, 1.0, 0.0};
         }
         PTB_TRY(upload(ctx, "sm_gf", gf, &dgf));
-        PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgf, nsm, max_m, 2 * max_own, "fe"));
+        tf.add(gf.data(), nsm);
+        PTB_TRY(tf.upload(ctx, "sm_wf"));
+        PTB_TRY(tf.launch(ctx, 0, true, false, dgf, "fe"));
     }
     PTB_HIP(ctx, hipStreamSynchronize(st));
     std::vector<int64_t> not_spd(nmol, 0);
@@ -591,7 +602,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         for (int sweep = 0; sweep < sweeps; ++sweep)
             PTB_TRY(refine_occupied(ctx, desc, ddesc, qf, A, Z, W, X, R, dvec, bis, wocc, nocc,
                                     oidx, thn));
-        PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgf, nsm, max_m, 2 * max_own, "fe"));
+        PTB_TRY(tf.launch(ctx, 0, true, false, dgf, "fe"));
         PTB_TRY(ptb::fe_proj(ctx, ddesc, nsm, max_m, dxoff, F, fe));
         PTB_TRY(bisect_all());
     }
@@ -618,7 +629,10 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         }
         GemmDesc *dgu;
         PTB_TRY(upload(ctx, "sm_gu", gu, &dgu));
-        PTB_TRY(ptb::gemm_vbatch(ctx, true, false, dgu, nsm, max_m, max_nocc, "u"));
+        ptb::TileList tu;
+        tu.add(gu.data(), nsm);
+        PTB_TRY(tu.upload(ctx, "sm_wu"));
+        PTB_TRY(tu.launch(ctx, 0, true, false, dgu, "u"));
     }
     PTB_TRY(ptb::dmat_compact(ctx, dmols, nmol, max_packed, ddesc, nsm, max_nocc, max_own, dgcols,
                               doloc, dmap, wocc, nocc, X));
@@ -709,7 +723,6 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
         int h;
         build_tree(s, 0, desc[s].m, leaves, levels, h);
     }
-    PTB_HIP(ctx, hipMemsetAsync(Z, 0, abytes, st));
     DcNode *dleaves;
     PTB_TRY(upload(ctx, "sm_leaves", leaves, &dleaves));
     PTB_TRY(dc_leaves(ctx, ddesc, dleaves, (int)leaves.size(), dvec, evec, Z));
@@ -723,8 +736,13 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
         std::string gm = "sm_lgemm" + std::to_string(lv);
         auto *dg = static_cast<GemmDesc *>(ws(ctx, gm.c_str(), sizeof(GemmDesc) * nodes.size()));
         if (!dg) return PTBGPU_E_NOMEM;
+        std::vector<GemmDesc> shape(nodes.size(), GemmDesc{});
+        for (size_t i = 0; i < nodes.size(); ++i) shape[i].M = shape[i].N = nodes[i].hi - nodes[i].lo;
+        TileList tl;
+        tl.add(shape.data(), (int)shape.size());
+        PTB_TRY(tl.upload(ctx, ("sm_lwork" + std::to_string(lv)).c_str()));
         PTB_TRY(dc_merge_level(ctx, ddesc, dnodes, (int)nodes.size(), max_n, evec, dvec, Z,
-                                    W, X, rot, dg));
+                                    W, X, rot, dg, tl));
     }
     return PTBGPU_OK;
 }

@@ -59,6 +59,13 @@ constexpr int kTriAUpper = 2;  // op(A)(i, k) = 0 for k < i
 constexpr int kTriBUpper = 4;  // op(B)(k, j) = 0 for k > j
 constexpr int kTriBLower = 8;  // op(B)(k, j) = 0 for k < j
 
+// One output tile of a batched GEMM: product g of the launch's descriptors
+// (-1: nothing to do), tile row t & 0xffff and tile column t >> 16 (64 x 64).
+struct GemmTile {
+    int g;
+    int t;
+};
+
 // A divide-and-conquer merge node: rows/cols [lo, hi) of submatrix s, split at sp.
 struct DcNode {
     int s;
@@ -106,9 +113,32 @@ int gather(ptbgpu_ctx *ctx, const MolDev *mols, const SubDesc *ddesc, int nsm, i
            const int *dmap, double *Hd, double *Sd);
 
 // dense.hip
-// f32: A, B, C are float arrays behind the descriptor's pointers (A B^T only)
+// f32: A, B, C are float arrays behind the descriptor's pointers (A B^T only).
+// gemm_vbatch launches count x tiles(max_m, max_n) workgroups (shapes may be
+// written on the device); gemm_tiles launches one workgroup per entry of a
+// tile list (TileList).
 int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int count, int max_m,
                 int max_n, const char *site = nullptr, bool f32 = false);
+int gemm_tiles(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, const GemmTile *dwork,
+               int64_t nwork, const char *site = nullptr, bool f32 = false);
+
+// Tile lists of batched GEMMs whose shapes the host knows.  Each launch gets
+// exactly the tiles that hold output (lower-triangle tiles for uplo), dealt to
+// the 8 XCDs so that a product's tiles share one XCD's L2 and the XCDs get
+// equal tile counts.  All launches' lists go to the device as one table.
+class TileList {
+  public:
+    // appends the list of one launch over products h[0 .. count); returns its index
+    int add(const GemmDesc *h, int count);
+    int upload(ptbgpu_ctx *ctx, const char *name);
+    int launch(ptbgpu_ctx *ctx, int i, bool ta, bool tb, const GemmDesc *ddesc,
+               const char *site = nullptr, bool f32 = false) const;
+
+  private:
+    std::vector<GemmTile> work_;
+    std::vector<int64_t> start_{0};
+    const GemmTile *dwork_ = nullptr;
+};
 int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                 double *A, int *flags);
 int transpose_sym_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m,
@@ -162,9 +192,11 @@ int dense_eig(ptbgpu_ctx *ctx, const double *M, int64_t n, double *V, std::vecto
 // dc.hip
 int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int nleaves,
               double *d, const double *e, double *Z);
+// tiles: the level's GEMM tile list (an n x n product per node, n = hi - lo;
+// the merge kernel writes N = the node's non-deflated count <= n)
 int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, int count,
                    int max_n, const double *e, double *D, double *Z, double *W, double *X,
-                   double *rot, GemmDesc *dgemm);
+                   double *rot, GemmDesc *dgemm, const TileList &tiles);
 
 // density.hip
 // XA = [L(O,:)^T, Li(:,O)] per submatrix (Li = L^-1)

@@ -462,6 +462,9 @@ int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc 
     auto *dg1 = static_cast<GemmDesc *>(upload_table(ctx, f32 ? "sytrd_g1f" : "sytrd_g1", g1.data(),
                                                      sizeof(GemmDesc) * g1.size()));
     if (!dg1) return PTBGPU_E_NOMEM;
+    TileList tl;
+    for (int p = 0; p < npanel; ++p) tl.add(g1.data() + start[p], start[p + 1] - start[p]);
+    PTB_TRY(tl.upload(ctx, f32 ? "sytrd_w1f" : "sytrd_w1"));
     for (int p = 0; p < npanel; ++p) {
         {
             ProfScope kprof_(ctx, "sytrd_panel");
@@ -469,8 +472,7 @@ int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc 
                                d, e, tau, dpwoff, p);
             PTB_CHECK_LAUNCH(ctx);
         }
-        const int c = start[p + 1] - start[p];
-        if (c > 0) PTB_TRY(gemm_vbatch(ctx, false, true, dg1 + start[p], c, max_m, max_m, "sytrd", f32));
+        PTB_TRY(tl.launch(ctx, p, false, true, dg1 + start[p], "sytrd", f32));
     }
     hipLaunchKernelGGL(k_sytrd_finish<T>, dim3((nsm + 255) / 256), dim3(256), 0, ctx->stream, ddesc, A,
                        d, e, tau, nsm);
