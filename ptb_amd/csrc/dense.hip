@@ -177,6 +177,90 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
     }
 }
 
+// Products of one k slice in the A B^T form (K <= 32, no triangular factor),
+// such as the rank-32 symmetric updates C -= [W V] [V W]^T: the C tile is
+// read into registers together with the A and B slices, so that a tile waits
+// on memory once before its MFMAs instead of a second time in the epilogue.
+// Tiles come from a tile list; layouts as in k_gemm.
+template <typename T>
+__global__ __launch_bounds__(kT, 4) void k_gemm_nt1(const GemmDesc *__restrict__ descs,
+                                                    const GemmTile *__restrict__ work) {
+    const GemmTile wt = work[blockIdx.x];
+    if (wt.g < 0) return;
+    const GemmDesc g = descs[wt.g];
+    const int m0 = (wt.t & 0xffff) * 64, n0 = (wt.t >> 16) * 64;
+    if (m0 >= g.M || n0 >= g.N) return;
+    const T *__restrict__ gA = reinterpret_cast<const T *>(g.A);
+    const T *__restrict__ gB = reinterpret_cast<const T *>(g.B);
+    T *gC = reinterpret_cast<T *>(g.C);
+    __shared__ double smem[2 * kKS * 68];
+    double (*As)[68] = reinterpret_cast<double (*)[68]>(smem);
+    double (*Bs)[68] = reinterpret_cast<double (*)[68]>(smem + kKS * 68);
+    // wave-uniform wavefront index: the C column offsets stay scalar
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int gm = m0 + lane;
+    const unsigned uldc = (unsigned)g.ldc;
+    // C: wavefront w holds columns n0 + w + 4 t, lane = row
+    double cold[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const int gn = n0 + wave + 4 * t;
+        const bool in = gm < g.M && gn < g.N && !(g.uplo && gm < gn) && g.beta != 0.0;
+        cold[t] = in ? (double)gC[(unsigned)gn * uldc + gm] : 0.0;
+    }
+    // A (64 x K) and B^T (K x 64) slices: element q of this thread is row
+    // mm = e & 63, depth kk = e >> 6 (e = thread + 256 q) of both
+    double ra[8], rb[8];
+    const unsigned ulda = (unsigned)g.lda, uldb = (unsigned)g.ldb;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = threadIdx.x + q * kT, mm = e & 63, kk = e >> 6;
+        ra[q] = (m0 + mm < g.M && kk < g.K) ? (double)gA[(unsigned)kk * ulda + (unsigned)(m0 + mm)] : 0.0;
+        rb[q] = (n0 + mm < g.N && kk < g.K) ? (double)gB[(unsigned)kk * uldb + (unsigned)(n0 + mm)] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int e = threadIdx.x + q * kT, mm = e & 63, kk = e >> 6;
+        As[kk][mm] = ra[q];
+        Bs[kk][mm] = rb[q];
+    }
+    __syncthreads();
+    d4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < kKS / 4; ++ks) {
+        const int kk = ks * 4 + lk;
+        const double a0 = As[kk][wm + lr], a1 = As[kk][wm + 16 + lr];
+        const double b0 = Bs[kk][wn + lr], b1 = Bs[kk][wn + 16 + lr];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+    double *Ct = smem;  // Ct[col * 65 + row]
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                Ct[(wn + 16 * j + lr) * 65 + wm + 16 * i + 4 * r + lk] = acc[i][j][r];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const int cc = wave + 4 * t, gn = n0 + cc;
+        if (gm >= g.M || gn >= g.N || (g.uplo && gm < gn)) continue;
+        const double v = g.alpha * Ct[cc * 65 + lane];
+        gC[(unsigned)gn * uldc + gm] = (T)(g.beta == 0.0 ? v : v + g.beta * cold[t]);
+    }
+}
+
 // One 32-column panel of the lower Cholesky factor of every submatrix: factor
 // the diagonal block in LDS, invert it, and form the panel below as
 // X = P L_kk^-T (rows independent).  The trailing update that follows is a
@@ -674,21 +758,32 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
     return launch_gemm(ctx, ta, tb, f32, dim3((unsigned)total), ddesc, count, tiles, nullptr);
 }
 
+// one: every product of the launch is a single k slice without triangular
+// factors (the A B^T form then runs k_gemm_nt1)
 int gemm_tiles(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, const GemmTile *dwork,
-               int64_t nwork, const char *site, bool f32) {
+               int64_t nwork, const char *site, bool f32, bool one) {
     ProfScope prof_(ctx, "gemm");
     ProfScope sprof_(ctx, site ? (std::string("gemm_") + site).c_str() : "gemm_other");
     if (nwork <= 0) return PTBGPU_OK;
     if (nwork > 0x7fffffff) return fail(ctx, PTBGPU_E_ARG, "GEMM tile list too long (%lld)", (long long)nwork);
-    return launch_gemm(ctx, ta, tb, f32, dim3((unsigned)nwork), ddesc, 0, 1, dwork);
+    const dim3 grid((unsigned)nwork);
+    if (one && !ta && tb) {
+        if (f32) hipLaunchKernelGGL(k_gemm_nt1<float>, grid, dim3(kT), 0, ctx->stream, ddesc, dwork);
+        else hipLaunchKernelGGL(k_gemm_nt1<double>, grid, dim3(kT), 0, ctx->stream, ddesc, dwork);
+        PTB_CHECK_LAUNCH(ctx);
+        return PTBGPU_OK;
+    }
+    return launch_gemm(ctx, ta, tb, f32, grid, ddesc, 0, 1, dwork);
 }
 
 int TileList::add(const GemmDesc *h, int count) {
     std::vector<GemmTile> q[8];
     int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool one = true;
     for (int i = 0; i < count; ++i) {
         const GemmDesc &g = h[i];
         if (g.M <= 0 || g.N <= 0) continue;
+        one = one && g.K <= kKS && g.tri == 0;
         const int tM = (g.M + 63) / 64, tN = (g.N + 63) / 64;
         int x = 0;
         for (int y = 1; y < 8; ++y)
@@ -702,6 +797,7 @@ int TileList::add(const GemmDesc *h, int count) {
     for (size_t k = 0; k < len; ++k)
         for (int x = 0; x < 8; ++x) work_.push_back(k < q[x].size() ? q[x][k] : GemmTile{-1, 0});
     start_.push_back((int64_t)work_.size());
+    one_.push_back(one);
     return (int)start_.size() - 2;
 }
 
@@ -716,7 +812,7 @@ int TileList::launch(ptbgpu_ctx *ctx, int i, bool ta, bool tb, const GemmDesc *d
     const int64_t n = start_[i + 1] - start_[i];
     if (n == 0) return PTBGPU_OK;
     if (!dwork_) return fail(ctx, PTBGPU_E_ARG, "GEMM tile list launched before upload");
-    return gemm_tiles(ctx, ta, tb, ddesc, dwork_ + start_[i], n, site, f32);
+    return gemm_tiles(ctx, ta, tb, ddesc, dwork_ + start_[i], n, site, f32, one_[i] != 0);
 }
 
 // X = L^-1 B (trans=false) or L^-T B (trans=true), all m columns, blocked by

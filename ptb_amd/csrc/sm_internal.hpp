@@ -120,7 +120,7 @@ int gather(ptbgpu_ctx *ctx, const MolDev *mols, const SubDesc *ddesc, int nsm, i
 int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int count, int max_m,
                 int max_n, const char *site = nullptr, bool f32 = false);
 int gemm_tiles(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, const GemmTile *dwork,
-               int64_t nwork, const char *site = nullptr, bool f32 = false);
+               int64_t nwork, const char *site = nullptr, bool f32 = false, bool one = false);
 
 // Tile lists of batched GEMMs whose shapes the host knows.  Each launch gets
 // exactly the tiles that hold output (lower-triangle tiles for uplo), dealt to
@@ -137,6 +137,7 @@ class TileList {
   private:
     std::vector<GemmTile> work_;
     std::vector<int64_t> start_{0};
+    std::vector<char> one_;  // per launch: every product is one k slice (K <= 32, no tri)
     const GemmTile *dwork_ = nullptr;
 };
 int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,

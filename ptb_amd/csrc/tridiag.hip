@@ -25,7 +25,7 @@
 namespace {
 
 constexpr int kT = 256;
-constexpr int kNB = 32;
+constexpr int kNB = 16;
 constexpr int kWaves = kT / PTB_WAVE;
 
 __device__ __forceinline__ double block_reduce(double v, double *red) {
