@@ -14,6 +14,8 @@
 //               inverted block per 64 rows).
 // * k_transpose, k_zero_upper, k_larft_all: helpers for K = L^-1 H L^-T and for
 //               the blocked (WY) application of the Householder reflectors.
+#include <algorithm>
+
 #include "sm_internal.hpp"
 
 namespace {
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-    double ra[8], rb[8];
+    T xa[8], xb[8];  // the next slices' raw elements (clamped addresses)
     // element q of this thread's share: A_op(mm, kk) and B_op(kk, nn)
     auto a_index = [&](int q, int &mm, int &kk) {
         const int e = threadIdx.x + q * kT;
@@ -107,35 +109,43 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
     // element offsets are 32-bit (every operand block of the batch is below
     // 2^32 elements from its descriptor pointer), which keeps the 16 addresses
     // of a slice in half the registers of 64-bit pointers
+    // Every load is issued, from an address clamped into the operand block;
+    // elements outside it are zeroed when the slice is stored to LDS.  The
+    // empty asm statement there consumes the raw values unconditionally, so
+    // that no load can be moved under the range test: a guarded load waits
+    // for memory inside its branch, one element at a time.  Callers ensure
+    // kend > kbeg >= 0.
     const unsigned ulda = (unsigned)g.lda, uldb = (unsigned)g.ldb;
     auto load = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             int mm, kk;
             a_index(q, mm, kk);
-            const int gm = m0 + mm, gk = k0 + kk;
-            const unsigned oa = TA ? (unsigned)gm * ulda + (unsigned)gk : (unsigned)gk * ulda + (unsigned)gm;
-            ra[q] = (gm < g.M && gk < kend) ? (double)gA[oa] : 0.0;
+            const unsigned cm = (unsigned)min(m0 + mm, g.M - 1), ck = (unsigned)min(k0 + kk, kend - 1);
+            xa[q] = gA[TA ? cm * ulda + ck : ck * ulda + cm];
             int kb, nn;
             b_index(q, kb, nn);
-            const int gn = n0 + nn, gkb = k0 + kb;
-            const unsigned ob = TB ? (unsigned)gkb * uldb + (unsigned)gn : (unsigned)gn * uldb + (unsigned)gkb;
-            rb[q] = (gn < g.N && gkb < kend) ? (double)gB[ob] : 0.0;
+            const unsigned cn = (unsigned)min(n0 + nn, g.N - 1), ckb = (unsigned)min(k0 + kb, kend - 1);
+            xb[q] = gB[TB ? ckb * uldb + cn : cn * uldb + ckb];
         }
     };
-    auto store = [&]() {
+    auto store = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) asm volatile("" : "+v"(xa[q]), "+v"(xb[q]));
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             int mm, kk, kb, nn;
             a_index(q, mm, kk);
             b_index(q, kb, nn);
-            As[kk][mm] = ra[q];
-            Bs[kb][nn] = rb[q];
+            As[kk][mm] = (m0 + mm < g.M && k0 + kk < kend) ? (double)xa[q] : 0.0;
+            Bs[kb][nn] = (n0 + nn < g.N && k0 + kb < kend) ? (double)xb[q] : 0.0;
         }
     };
-    load(kbeg);
-    store();
-    __syncthreads();
+    if (kbeg < kend) {
+        load(kbeg);
+        store(kbeg);
+        __syncthreads();
+    }
     for (int k0 = kbeg; k0 < kend; k0 += kKS) {
         const bool more = k0 + kKS < kend;
         if (more) load(k0 + kKS);
@@ -151,7 +161,7 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
         }
         __syncthreads();
         if (more) {
-            store();
+            store(k0 + kKS);
             __syncthreads();
         }
     }
@@ -202,29 +212,42 @@ __global__ __launch_bounds__(kT, 4) void k_gemm_nt1(const GemmDesc *__restrict__
     const int lr = lane & 15, lk = lane >> 4;
     const int gm = m0 + lane;
     const unsigned uldc = (unsigned)g.ldc;
-    // C: wavefront w holds columns n0 + w + 4 t, lane = row
-    double cold[16];
+    // All loads are issued from addresses clamped into their blocks, with
+    // out-of-range elements zeroed afterwards (no load waits under a branch).
+    // C: wavefront w holds columns n0 + w + 4 t, lane = row; read only when
+    // beta != 0 (a uniform branch).
+    const unsigned cmr = (unsigned)min(gm, g.M - 1);
+    T craw[16];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        const int gn = n0 + wave + 4 * t;
-        const bool in = gm < g.M && gn < g.N && !(g.uplo && gm < gn) && g.beta != 0.0;
-        cold[t] = in ? (double)gC[(unsigned)gn * uldc + gm] : 0.0;
+    for (int t = 0; t < 16; ++t) craw[t] = T(0);
+    if (g.beta != 0.0) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            craw[t] = gC[(unsigned)min(n0 + wave + 4 * t, g.N - 1) * uldc + cmr];
     }
     // A (64 x K) and B^T (K x 64) slices: element q of this thread is row
     // mm = e & 63, depth kk = e >> 6 (e = thread + 256 q) of both
-    double ra[8], rb[8];
+    T xa[8], xb[8];
     const unsigned ulda = (unsigned)g.lda, uldb = (unsigned)g.ldb;
+    const int kmax = max(g.K - 1, 0);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const int e = threadIdx.x + q * kT, mm = e & 63, kk = e >> 6;
-        ra[q] = (m0 + mm < g.M && kk < g.K) ? (double)gA[(unsigned)kk * ulda + (unsigned)(m0 + mm)] : 0.0;
-        rb[q] = (n0 + mm < g.N && kk < g.K) ? (double)gB[(unsigned)kk * uldb + (unsigned)(n0 + mm)] : 0.0;
+        const int e = threadIdx.x + q * kT, mm = e & 63;
+        const unsigned kk = (unsigned)min(e >> 6, kmax);
+        xa[q] = gA[kk * ulda + (unsigned)min(m0 + mm, g.M - 1)];
+        xb[q] = gB[kk * uldb + (unsigned)min(n0 + mm, g.N - 1)];
     }
+    // consume the raw values unconditionally (see k_gemm), then mask
+#pragma unroll
+    for (int q = 0; q < 8; ++q) asm volatile("" : "+v"(xa[q]), "+v"(xb[q]));
+#pragma unroll
+    for (int t = 0; t < 16; ++t) asm volatile("" : "+v"(craw[t]));
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int e = threadIdx.x + q * kT, mm = e & 63, kk = e >> 6;
-        As[kk][mm] = ra[q];
-        Bs[kk][mm] = rb[q];
+        const bool kin = kk < g.K;
+        As[kk][mm] = (kin && m0 + mm < g.M) ? (double)xa[q] : 0.0;
+        Bs[kk][mm] = (kin && n0 + mm < g.N) ? (double)xb[q] : 0.0;
     }
     __syncthreads();
     d4_t acc[2][2];
@@ -257,7 +280,7 @@ __global__ __launch_bounds__(kT, 4) void k_gemm_nt1(const GemmDesc *__restrict__
         const int cc = wave + 4 * t, gn = n0 + cc;
         if (gm >= g.M || gn >= g.N || (g.uplo && gm < gn)) continue;
         const double v = g.alpha * Ct[cc * 65 + lane];
-        gC[(unsigned)gn * uldc + gm] = (T)(g.beta == 0.0 ? v : v + g.beta * cold[t]);
+        gC[(unsigned)gn * uldc + gm] = (T)(g.beta == 0.0 ? v : v + g.beta * (double)craw[t]);
     }
 }
 
@@ -780,7 +803,16 @@ int TileList::add(const GemmDesc *h, int count) {
     std::vector<GemmTile> q[8];
     int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool one = true;
+    // largest products first (longest-processing-time order): each XCD's
+    // queue ends with its small products
+    std::vector<int> ord(count);
+    std::vector<int64_t> area(count);
     for (int i = 0; i < count; ++i) {
+        ord[i] = i;
+        area[i] = (int64_t)std::max(h[i].M, 0) * std::max(h[i].N, 0) * std::max(h[i].K, 1);
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return area[a] > area[b]; });
+    for (int i : ord) {
         const GemmDesc &g = h[i];
         if (g.M <= 0 || g.N <= 0) continue;
         one = one && g.K <= kKS && g.tri == 0;

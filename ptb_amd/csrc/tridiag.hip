@@ -20,6 +20,8 @@
 // solve, float for the mixed-precision one (half the bytes of the GEMV).
 // Sums, the reflectors and the LDS vectors are FP64 in both cases; only the
 // stored values are rounded to T.
+#include <algorithm>
+
 #include "sm_internal.hpp"
 
 namespace {
@@ -106,15 +108,18 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 3 : 2) void k_sytrd_panel(cons
                                                     double *__restrict__ d_all,
                                                     double *__restrict__ e_all,
                                                     double *__restrict__ tau_all,
-                                                    const int64_t *__restrict__ pwoff, int panel) {
+                                                    const int64_t *__restrict__ pwoff,
+                                                    const int *__restrict__ order, int panel) {
     extern __shared__ double sm[];
-    const SubDesc ds = desc[blockIdx.x];
+    // submatrices largest first, so that the last workgroups to start are small
+    const int si = order[blockIdx.x];
+    const SubDesc ds = desc[si];
     const int m = ds.m, ld = ds.ld;
     T *a = Aa + ds.off;
     // panel buffer [W | V | W], ld x 96: W columns 0..31, a copy of the panel's
     // reflectors in 32..63 and of W in 64..95, so that the trailing update
     // A -= W V^T + V W^T is one product [W V] [V W]^T
-    T *W = Wa + pwoff[blockIdx.x];
+    T *W = Wa + pwoff[si];
     double *dd = d_all + ds.mapoff, *ee = e_all + ds.mapoff, *tt = tau_all + ds.mapoff;
     double *vs = sm;              // m   current Householder vector (rows > k)
     double *cp = vs + m;          // kWaves x m  per-wavefront partials of A22 v
@@ -437,6 +442,11 @@ int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc 
     auto *PW = static_cast<T *>(ws(ctx, f32 ? "sytrd_pw32" : "sytrd_pw", sizeof(T) * std::max<int64_t>(pwtot, 1)));
     auto *dpwoff = static_cast<int64_t *>(upload_table(ctx, "sytrd_pwoff", pwoff.data(), sizeof(int64_t) * nsm));
     if (!PW || !dpwoff) return PTBGPU_E_NOMEM;
+    std::vector<int> order(nsm);
+    for (int si = 0; si < nsm; ++si) order[si] = si;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return desc[a].m > desc[b].m; });
+    auto *dorder = static_cast<int *>(upload_table(ctx, "sytrd_order", order.data(), sizeof(int) * nsm));
+    if (!dorder) return PTBGPU_E_NOMEM;
     std::vector<GemmDesc> g1;
     std::vector<int> start(npanel + 1, 0);
     for (int p = 0; p < npanel; ++p) {
@@ -469,7 +479,7 @@ int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc 
         {
             ProfScope kprof_(ctx, "sytrd_panel");
             hipLaunchKernelGGL(k_sytrd_panel<T>, dim3(nsm), dim3(kT), shmem, ctx->stream, ddesc, A, PW,
-                               d, e, tau, dpwoff, p);
+                               d, e, tau, dpwoff, dorder, p);
             PTB_CHECK_LAUNCH(ctx);
         }
         PTB_TRY(tl.launch(ctx, p, false, true, dg1 + start[p], "sytrd", f32));
