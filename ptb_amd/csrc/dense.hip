@@ -15,6 +15,7 @@
 // * k_transpose, k_zero_upper, k_larft_all: helpers for K = L^-1 H L^-T and for
 //               the blocked (WY) application of the Householder reflectors.
 #include <algorithm>
+#include <cstring>
 
 #include "sm_internal.hpp"
 
@@ -800,43 +801,115 @@ int gemm_tiles(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, const G
 }
 
 int TileList::add(const GemmDesc *h, int count) {
-    std::vector<GemmTile> q[8];
-    int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    bool one = true;
-    // largest products first (longest-processing-time order): each XCD's
-    // queue ends with its small products
-    std::vector<int> ord(count);
-    std::vector<int64_t> area(count);
+    sig_.push_back(count);
     for (int i = 0; i < count; ++i) {
-        ord[i] = i;
-        area[i] = (int64_t)std::max(h[i].M, 0) * std::max(h[i].N, 0) * std::max(h[i].K, 1);
-    }
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return area[a] > area[b]; });
-    for (int i : ord) {
         const GemmDesc &g = h[i];
-        if (g.M <= 0 || g.N <= 0) continue;
-        one = one && g.K <= kKS && g.tri == 0;
-        const int tM = (g.M + 63) / 64, tN = (g.N + 63) / 64;
-        int x = 0;
-        for (int y = 1; y < 8; ++y)
-            if (load[y] < load[x]) x = y;
-        for (int tn = 0; tn < tN; ++tn)
-            for (int tm = g.uplo ? tn : 0; tm < tM; ++tm) q[x].push_back({i, tm | (tn << 16)});
-        load[x] = (int64_t)q[x].size();
+        sig_.insert(sig_.end(), {g.M, g.N, g.K, g.uplo, g.tri});
     }
-    size_t len = 0;
-    for (int x = 0; x < 8; ++x) len = std::max(len, q[x].size());
-    for (size_t k = 0; k < len; ++k)
-        for (int x = 0; x < 8; ++x) work_.push_back(k < q[x].size() ? q[x][k] : GemmTile{-1, 0});
-    start_.push_back((int64_t)work_.size());
-    one_.push_back(one);
-    return (int)start_.size() - 2;
+    return nlaunch_++;
+}
+
+// Every launch: products largest first (longest-processing-time order), each
+// dealt whole to the XCD with the fewest tiles so far; entry 8 k + x of the
+// launch's block is XCD x's k-th tile (-1 where its queue has ended).  Tiles
+// of a product in column-major tile order, lower-triangle tiles only for uplo.
+void TileList::build(std::vector<GemmTile> &work) {
+    start_.assign(1, 0);
+    one_.clear();
+    size_t p = 0;
+    for (int l = 0; l < nlaunch_; ++l) {
+        const int count = sig_[p++];
+        const int *sh = sig_.data() + p;  // 5 ints per product
+        p += 5 * (size_t)count;
+        std::vector<int> ord(count), xcd(count);
+        std::vector<int64_t> area(count), pos(count);
+        for (int i = 0; i < count; ++i) {
+            ord[i] = i;
+            area[i] = (int64_t)std::max(sh[5 * i], 0) * std::max(sh[5 * i + 1], 0) * std::max(sh[5 * i + 2], 1);
+        }
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return area[a] > area[b]; });
+        int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        bool one = true;
+        auto tiles_of = [&](int i) -> int64_t {
+            const int M = sh[5 * i], N = sh[5 * i + 1];
+            if (M <= 0 || N <= 0) return 0;
+            const int64_t tM = (M + 63) / 64, tN = (N + 63) / 64;
+            if (!sh[5 * i + 3]) return tM * tN;
+            const int64_t c = std::min(tM, tN);  // columns tn < c hold tM - tn tiles
+            return c * tM - c * (c - 1) / 2;
+        };
+        for (int i : ord) {
+            const int64_t nt = tiles_of(i);
+            if (nt == 0) continue;
+            one = one && sh[5 * i + 2] <= kKS && sh[5 * i + 4] == 0;
+            int x = 0;
+            for (int y = 1; y < 8; ++y)
+                if (load[y] < load[x]) x = y;
+            xcd[i] = x;
+            pos[i] = load[x];
+            load[x] += nt;
+        }
+        int64_t len = 0;
+        for (int x = 0; x < 8; ++x) len = std::max(len, load[x]);
+        const size_t base = work.size();
+        work.resize(base + 8 * (size_t)len, GemmTile{-1, 0});
+        for (int i : ord) {
+            const int M = sh[5 * i], N = sh[5 * i + 1];
+            if (tiles_of(i) == 0) continue;
+            const int tM = (M + 63) / 64, tN = (N + 63) / 64;
+            int64_t k = pos[i];
+            for (int tn = 0; tn < tN; ++tn)
+                for (int tm = sh[5 * i + 3] ? tn : 0; tm < tM; ++tm, ++k)
+                    work[base + 8 * (size_t)k + xcd[i]] = GemmTile{i, tm | (tn << 16)};
+        }
+        start_.push_back((int64_t)work.size());
+        one_.push_back(one);
+    }
 }
 
 int TileList::upload(ptbgpu_ctx *ctx, const char *name) {
-    if (work_.empty()) return PTBGPU_OK;
-    dwork_ = static_cast<const GemmTile *>(upload_table(ctx, name, work_.data(), sizeof(GemmTile) * work_.size()));
-    return dwork_ ? PTBGPU_OK : PTBGPU_E_NOMEM;
+    // the shadow entry of `name` holds the shapes of the table in its buffer
+    // (ws drops it when the buffer is reallocated); "<name>#meta" holds the
+    // launch starts and flags built from them
+    const std::string mkey = std::string(name) + "#meta";
+    const char *sb = reinterpret_cast<const char *>(sig_.data());
+    const size_t sn = sizeof(int) * sig_.size();
+    auto sh = ctx->shadow.find(name);
+    auto mh = ctx->shadow.find(mkey);
+    const size_t nl = (size_t)nlaunch_;
+    if (sh != ctx->shadow.end() && mh != ctx->shadow.end() && sh->second.size() == sn &&
+        memcmp(sh->second.data(), sb, sn) == 0 &&
+        mh->second.size() == sizeof(int64_t) * (nl + 1) + nl) {
+        start_.resize(nl + 1);
+        one_.resize(nl);
+        memcpy(start_.data(), mh->second.data(), sizeof(int64_t) * (nl + 1));
+        memcpy(one_.data(), mh->second.data() + sizeof(int64_t) * (nl + 1), nl);
+        if (start_.back() == 0) return PTBGPU_OK;
+        dwork_ = static_cast<const GemmTile *>(ws(ctx, name, sizeof(GemmTile) * start_.back()));
+        return dwork_ ? PTBGPU_OK : PTBGPU_E_NOMEM;
+    }
+    std::vector<GemmTile> work;
+    build(work);
+    std::vector<char> meta(sizeof(int64_t) * (nl + 1) + nl);
+    memcpy(meta.data(), start_.data(), sizeof(int64_t) * (nl + 1));
+    memcpy(meta.data() + sizeof(int64_t) * (nl + 1), one_.data(), nl);
+    if (work.empty()) {
+        ctx->shadow[name].assign(sb, sb + sn);
+        ctx->shadow[mkey] = meta;
+        return PTBGPU_OK;
+    }
+    const size_t bytes = sizeof(GemmTile) * work.size();
+    void *d = ws(ctx, name, bytes);
+    if (!d) return PTBGPU_E_NOMEM;
+    if (hipMemcpyAsync(d, work.data(), bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        ctx->shadow.erase(name);
+        return fail(ctx, PTBGPU_E_HIP, "upload of tile list '%s' failed: %s", name,
+                    hipGetErrorString(hipGetLastError()));
+    }
+    ctx->shadow[name].assign(sb, sb + sn);
+    ctx->shadow[mkey] = meta;
+    dwork_ = static_cast<const GemmTile *>(d);
+    return PTBGPU_OK;
 }
 
 int TileList::launch(ptbgpu_ctx *ctx, int i, bool ta, bool tb, const GemmDesc *ddesc,

@@ -727,7 +727,11 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
     PTB_TRY(upload(ctx, "sm_leaves", leaves, &dleaves));
     PTB_TRY(dc_leaves(ctx, ddesc, dleaves, (int)leaves.size(), dvec, evec, Z));
     for (size_t lv = 0; lv < levels.size(); ++lv) {
-        const auto &nodes = levels[lv];
+        // largest nodes first, so that the level's last workgroups are small
+        auto &nodes = levels[lv];
+        std::stable_sort(nodes.begin(), nodes.end(), [](const DcNode &a, const DcNode &b) {
+            return a.hi - a.lo > b.hi - b.lo;
+        });
         int max_n = 0;
         for (const auto &nd : nodes) max_n = std::max(max_n, nd.hi - nd.lo);
         std::string nm = "sm_level" + std::to_string(lv);

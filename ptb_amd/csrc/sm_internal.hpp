@@ -126,18 +126,24 @@ int gemm_tiles(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, const G
 // exactly the tiles that hold output (lower-triangle tiles for uplo), dealt to
 // the 8 XCDs so that a product's tiles share one XCD's L2 and the XCDs get
 // equal tile counts.  All launches' lists go to the device as one table.
+// The lists are built on the host only when the shapes differ from those of
+// the table already in the device buffer of that name (repeated builds of the
+// same system reuse it).
 class TileList {
   public:
-    // appends the list of one launch over products h[0 .. count); returns its index
+    // records the shapes of one launch over products h[0 .. count); returns its index
     int add(const GemmDesc *h, int count);
+    // builds (if the shapes changed) and uploads the lists of every launch
     int upload(ptbgpu_ctx *ctx, const char *name);
     int launch(ptbgpu_ctx *ctx, int i, bool ta, bool tb, const GemmDesc *ddesc,
                const char *site = nullptr, bool f32 = false) const;
 
   private:
-    std::vector<GemmTile> work_;
-    std::vector<int64_t> start_{0};
-    std::vector<char> one_;  // per launch: every product is one k slice (K <= 32, no tri)
+    void build(std::vector<GemmTile> &work);
+    std::vector<int> sig_;        // per launch: count, then M, N, K, uplo, tri per product
+    int nlaunch_ = 0;
+    std::vector<int64_t> start_;  // per launch: first entry in the table (+ the total)
+    std::vector<char> one_;       // per launch: every product is one k slice (K <= 32, no tri)
     const GemmTile *dwork_ = nullptr;
 };
 int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
