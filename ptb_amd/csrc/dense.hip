@@ -25,6 +25,13 @@ constexpr int kT = 256;
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
+// Operand pointers read from a descriptor in memory are generic (flat) to the
+// compiler.  Flat loads also count against the LDS wait counter, so the
+// k loop's wait for its LDS operands would wait for the next slice's global
+// prefetch as well.  Pointers into device memory are global: say so.
+template <typename T>
+using gmem = __attribute__((address_space(1))) T;
+
 // 64x64 output tile per 256-thread workgroup; each wavefront owns a 32x32
 // quarter as 2x2 blocks of v_mfma_f64_16x16x4.  MFMA operand layout: lane l
 // supplies A(l%16, l/16) of a 16x4 slice and B(l/16, l%16) of a 4x16 slice;
@@ -72,9 +79,9 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
     const GemmDesc g = descs[gi];
     // (a list may hold tiles of a bound on a shape the device fills in)
     if (m0 >= g.M || n0 >= g.N) return;
-    const T *__restrict__ gA = reinterpret_cast<const T *>(g.A);
-    const T *__restrict__ gB = reinterpret_cast<const T *>(g.B);
-    T *gC = reinterpret_cast<T *>(g.C);
+    const gmem<T> *__restrict__ gA = (const gmem<T> *)(const void *)g.A;
+    const gmem<T> *__restrict__ gB = (const gmem<T> *)(const void *)g.B;
+    gmem<T> *gC = (gmem<T> *)(void *)g.C;
     // k range of this tile where triangular factors can be nonzero, aligned
     // down to the slice depth: the other triangle of a triangular factor must
     // hold zeros (the few of its elements inside the range add nothing)
@@ -182,7 +189,7 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
     for (int cc = wave; cc < 64; cc += kT / 64) {
         const int gn = n0 + cc;
         if (gm >= g.M || gn >= g.N || (g.uplo && gm < gn)) continue;
-        T *c = gC + (int64_t)gn * g.ldc + gm;
+        auto *c = gC + (int64_t)gn * g.ldc + gm;
         const double v = Ct[cc * 65 + lane];
         *c = (T)(g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * (double)*c);
     }
@@ -201,9 +208,9 @@ __global__ __launch_bounds__(kT, 4) void k_gemm_nt1(const GemmDesc *__restrict__
     const GemmDesc g = descs[wt.g];
     const int m0 = (wt.t & 0xffff) * 64, n0 = (wt.t >> 16) * 64;
     if (m0 >= g.M || n0 >= g.N) return;
-    const T *__restrict__ gA = reinterpret_cast<const T *>(g.A);
-    const T *__restrict__ gB = reinterpret_cast<const T *>(g.B);
-    T *gC = reinterpret_cast<T *>(g.C);
+    const gmem<T> *__restrict__ gA = (const gmem<T> *)(const void *)g.A;
+    const gmem<T> *__restrict__ gB = (const gmem<T> *)(const void *)g.B;
+    gmem<T> *gC = (gmem<T> *)(void *)g.C;
     __shared__ double smem[2 * kKS * 68];
     double (*As)[68] = reinterpret_cast<double (*)[68]>(smem);
     double (*Bs)[68] = reinterpret_cast<double (*)[68]>(smem + kKS * 68);
