@@ -505,9 +505,20 @@ __global__ __launch_bounds__(kT) void k_larft_all(const SubDesc *__restrict__ de
     // column c is zero above its unit at row b0 + c + 1, so every pair can run
     // over rows b0 + 1 .. m - 1
     for (int r0 = b0 + 1; r0 < d.m; r0 += 64) {
-        for (int l = threadIdx.x; l < 64 * 32; l += kT) {
-            const int rr = l & 63, cc = l >> 6, r = r0 + rr;
-            Vs[rr][cc] = (r < d.m && cc < kb) ? V[(int64_t)cc * d.ld + r] : 0.0;
+        // the chunk's 8 elements per thread: every load issued from a clamped
+        // address, then masked (the empty asm keeps loads out of branches)
+        double x[64 * 32 / kT];
+#pragma unroll
+        for (int u = 0; u < 64 * 32 / kT; ++u) {
+            const int l = threadIdx.x + u * kT, rr = l & 63, cc = l >> 6;
+            x[u] = V[(int64_t)min(cc, kb - 1) * d.ld + min(r0 + rr, d.m - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < 64 * 32 / kT; ++u) asm volatile("" : "+v"(x[u]));
+#pragma unroll
+        for (int u = 0; u < 64 * 32 / kT; ++u) {
+            const int l = threadIdx.x + u * kT, rr = l & 63, cc = l >> 6;
+            Vs[rr][cc] = (r0 + rr < d.m && cc < kb) ? x[u] : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -582,11 +593,24 @@ __global__ __launch_bounds__(kT) void k_larft_merge(const SubDesc *__restrict__ 
         const double *V1 = Va + d.off + (int64_t)(b * 32) * d.ld;
         const double *V2 = Va + d.off + (int64_t)((b + 1) * 32) * d.ld;
         for (int r0 = (b + 1) * 32 + 1; r0 < d.m; r0 += 64) {
+            // 2 x 8 elements per thread, loads issued together (see k_larft_all)
+            double x1[64 * 32 / kT], x2[64 * 32 / kT];
+#pragma unroll
+            for (int u = 0; u < 64 * 32 / kT; ++u) {
+                const int l = threadIdx.x + u * kT, rr = l & 63, cc = l >> 6;
+                const int64_t r = min(r0 + rr, d.m - 1);
+                x1[u] = V1[(int64_t)cc * d.ld + r];
+                x2[u] = V2[(int64_t)min(cc, kb2 - 1) * d.ld + r];
+            }
+#pragma unroll
+            for (int u = 0; u < 64 * 32 / kT; ++u) asm volatile("" : "+v"(x1[u]), "+v"(x2[u]));
             __syncthreads();
-            for (int l = threadIdx.x; l < 64 * 32; l += kT) {
-                const int rr = l & 63, cc = l >> 6, r = r0 + rr;
-                V1s[rr][cc] = r < d.m ? V1[(int64_t)cc * d.ld + r] : 0.0;
-                V2s[rr][cc] = (r < d.m && cc < kb2) ? V2[(int64_t)cc * d.ld + r] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 64 * 32 / kT; ++u) {
+                const int l = threadIdx.x + u * kT, rr = l & 63, cc = l >> 6;
+                const bool in = r0 + rr < d.m;
+                V1s[rr][cc] = in ? x1[u] : 0.0;
+                V2s[rr][cc] = (in && cc < kb2) ? x2[u] : 0.0;
             }
             __syncthreads();
 #pragma unroll
