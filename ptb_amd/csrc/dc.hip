@@ -201,6 +201,16 @@ __device__ int block_scan_excl(int v, int *wsum, int *total) {
 // One merge node per workgroup.  Scratch: W node block receives the sorted,
 // rotated, compacted eigenvector columns [nondeflated | deflated]; X node block
 // receives U (K x K); rot holds the deflation rotations (4 doubles each).
+//
+// A left child's column is zero below the split and a right child's above it;
+// a deflation rotation between a left and a right column makes both dense.
+// The non-deflated columns are placed [left-only | dense | right-only] (pole
+// order within each group), with U's rows in the same order, so that the
+// merged vectors are two products of half the height:
+//   rows above the split:  W(0:n1, left+dense) U(left+dense, :),
+//   rows below it:         W(n1:n, dense+right) U(dense+right, :)
+// (about half the flops of W(:, 0:K) U; the structure dlaed3 exploits).  The
+// kernel writes both descriptors, gd[2 node] and gd[2 node + 1].
 __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ desc,
                                                   const DcNode *__restrict__ nodes, int npow,
                                                   const double *__restrict__ e_all,
@@ -221,10 +231,11 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     double *zh = tau + npow;        // K Loewner vector (rho w_i^2 before that)
     int *ix = reinterpret_cast<int *>(zh + npow);  // npow sort permutation
     int *slot = ix + npow;          // n output slot of sorted position
-    int *org = slot + npow;         // K origin pole of each root
+    int *org = slot + npow;         // K sorted position of each pole, then origin pole of each root
+    int *prow = org + npow;         // K W slot (and U row) of each pole
     __shared__ double red[kMT / PTB_WAVE];
     __shared__ int wsum[kMT / PTB_WAVE];
-    __shared__ int sK, sNr, sClose;
+    __shared__ int sK, sNr, sClose, sKL, sKD;
     __shared__ double srho;
 
     const double beta = e_all[ds.mapoff + nd.sp - 1];
@@ -233,15 +244,15 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     double *Z = Za + ds.off, *W = Wa + ds.off, *X = Xa + ds.off;
     double *D = D_all + ds.mapoff;
     double *rot = rot_all + 4 * (int64_t)(ds.mapoff + nd.lo);
-    GemmDesc *g = gd + blockIdx.x;
+    GemmDesc *g = gd + 2 * blockIdx.x;
     const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
     constexpr int kWaves = kMT / PTB_WAVE;
 
     if (rho == 0.0) {  // decoupled: children are already eigenpairs of T
-        if (threadIdx.x == 0) {
-            g->M = 0;
-            g->N = 0;
-            g->K = 0;
+        if (threadIdx.x < 2) {
+            g[threadIdx.x].M = 0;
+            g[threadIdx.x].N = 0;
+            g[threadIdx.x].K = 0;
         }
         // the node block is diag(Z1, Z2): zero its off-diagonal blocks
         for (int c = wid; c < n; c += kWaves) {
@@ -360,6 +371,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
             if (slot[j] == -2) {
                 dl[K] = Dk[j];
                 w[K] = z[j];
+                org[K] = j;
                 slot[j] = K++;
             }
         nd_ = K;
@@ -371,6 +383,46 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     }
     __syncthreads();
     const int K = sK, nr = sNr;
+    // Column types: bit 0 = rows above the split, bit 1 = rows below, from the
+    // child of origin and the rotations in order (zh is free until the roots).
+    int *cmask = reinterpret_cast<int *>(zh);
+    for (int j = threadIdx.x; j < n; j += kMT) cmask[j] = ix[j] < n1 ? 1 : 2;
+    __syncthreads();
+    if (nr > 0 && threadIdx.x == 0)
+        for (int q = 0; q < nr; ++q) {
+            const int a = (int)rot[4 * q], b = (int)rot[4 * q + 1];
+            const int mm = cmask[a] | cmask[b];
+            cmask[a] = mm;
+            cmask[b] = mm;
+        }
+    __syncthreads();
+    // slots [left-only | dense | right-only] by prefix counts over the poles
+    {
+        const int chunk = (K + kMT - 1) / kMT;
+        const int ib = min(K, (int)threadIdx.x * chunk), ie = min(K, ib + chunk);
+        int cl = 0, cd = 0;
+        for (int i = ib; i < ie; ++i) {
+            const int t = cmask[org[i]];
+            cl += t == 1;
+            cd += t == 3;
+        }
+        int KL, KD;
+        int pl = block_scan_excl(cl, wsum, &KL);
+        int pd = block_scan_excl(cd, wsum, &KD);
+        int pr = ib - pl - pd;  // right-only poles before ib
+        for (int i = ib; i < ie; ++i) {
+            const int t = cmask[org[i]];
+            prow[i] = t == 1 ? pl++ : (t == 3 ? KL + pd++ : KL + KD + pr++);
+        }
+        if (threadIdx.x == 0) {
+            sKL = KL;
+            sKD = KD;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < K; i += kMT) slot[org[i]] = prow[i];
+        __syncthreads();
+    }
+    const int KL = sKL, KD = sKD;
     // sorted, compacted eigenvector columns into W: W(:, slot[j]) = Z(:, lo+ix[j]),
     // one column per wavefront.  A child's column is nonzero only on the
     // child's rows; the rest of the node block is written as zeros, never
@@ -516,24 +568,29 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         }
         const double ci = 1.0 / sqrt(wave_sum(nrm));
         double *xc = X + (int64_t)(nd.lo + j) * ld + nd.lo;
-        for (int i = lane; i < K; i += PTB_WAVE) xc[i] = ci * zh[i] * rcp64((dl[i] - dorg) - tj);
+        for (int i = lane; i < K; i += PTB_WAVE) xc[prow[i]] = ci * zh[i] * rcp64((dl[i] - dorg) - tj);
         if (lane == 0) D[nd.lo + j] = dorg + tj;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        g->A = W + (int64_t)nd.lo * ld + nd.lo;
-        g->B = X + (int64_t)nd.lo * ld + nd.lo;
-        g->C = Z + (int64_t)nd.lo * ld + nd.lo;
-        g->M = n;
-        g->N = K;
-        g->K = K;
-        g->lda = ld;
-        g->ldb = ld;
-        g->ldc = ld;
-        g->alpha = 1.0;
-        g->beta = 0.0;
-        g->uplo = 0;
-        g->tri = 0;
+    if (threadIdx.x < 2) {
+        // 0: rows above the split from slots [0, KL + KD); 1: rows below it
+        // from slots [KL, K)
+        const bool top = threadIdx.x == 0;
+        const int r0 = top ? 0 : n1, s0 = top ? 0 : KL;
+        GemmDesc &gg = g[threadIdx.x];
+        gg.A = W + (int64_t)(nd.lo + s0) * ld + nd.lo + r0;
+        gg.B = X + (int64_t)nd.lo * ld + nd.lo + s0;
+        gg.C = Z + (int64_t)nd.lo * ld + nd.lo + r0;
+        gg.M = top ? n1 : n - n1;
+        gg.N = K;
+        gg.K = top ? KL + KD : K - KL;
+        gg.lda = ld;
+        gg.ldb = ld;
+        gg.ldc = ld;
+        gg.alpha = 1.0;
+        gg.beta = 0.0;
+        gg.uplo = 0;
+        gg.tri = 0;
     }
     (void)srho;
 }
@@ -557,7 +614,7 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
     if (count <= 0) return PTBGPU_OK;
     int npow = 1;
     while (npow < max_n) npow <<= 1;
-    const size_t shmem = (size_t)npow * (6 * sizeof(double) + 3 * sizeof(int));
+    const size_t shmem = (size_t)npow * (6 * sizeof(double) + 4 * sizeof(int));
     if (shmem > 150 * 1024)
         return fail(ctx, PTBGPU_E_ARG, "merge of %d rows exceeds the LDS budget", max_n);
     {

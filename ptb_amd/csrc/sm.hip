@@ -738,10 +738,16 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
         DcNode *dnodes;
         PTB_TRY(upload(ctx, nm.c_str(), nodes, &dnodes));
         std::string gm = "sm_lgemm" + std::to_string(lv);
-        auto *dg = static_cast<GemmDesc *>(ws(ctx, gm.c_str(), sizeof(GemmDesc) * nodes.size()));
+        // two products per node: the rows above and below its split
+        auto *dg = static_cast<GemmDesc *>(ws(ctx, gm.c_str(), sizeof(GemmDesc) * 2 * nodes.size()));
         if (!dg) return PTBGPU_E_NOMEM;
-        std::vector<GemmDesc> shape(nodes.size(), GemmDesc{});
-        for (size_t i = 0; i < nodes.size(); ++i) shape[i].M = shape[i].N = nodes[i].hi - nodes[i].lo;
+        std::vector<GemmDesc> shape(2 * nodes.size(), GemmDesc{});
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            const int n = nodes[i].hi - nodes[i].lo, n1 = nodes[i].sp - nodes[i].lo;
+            shape[2 * i].M = n1;
+            shape[2 * i + 1].M = n - n1;
+            shape[2 * i].N = shape[2 * i + 1].N = n;
+        }
         TileList tl;
         tl.add(shape.data(), (int)shape.size());
         PTB_TRY(tl.upload(ctx, ("sm_lwork" + std::to_string(lv)).c_str()));

@@ -199,8 +199,10 @@ int dense_eig(ptbgpu_ctx *ctx, const double *M, int64_t n, double *V, std::vecto
 // dc.hip
 int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int nleaves,
               double *d, const double *e, double *Z);
-// tiles: the level's GEMM tile list (an n x n product per node, n = hi - lo;
-// the merge kernel writes N = the node's non-deflated count <= n)
+// tiles: the level's GEMM tile list, two products per node (rows above and
+// below the split: n1 x n and (n - n1) x n bounds, n = hi - lo; the merge
+// kernel writes N = the node's non-deflated count <= n); dgemm holds
+// 2 x count descriptors
 int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, int count,
                    int max_n, const double *e, double *D, double *Z, double *W, double *X,
                    double *rot, GemmDesc *dgemm, const TileList &tiles);
