@@ -156,23 +156,23 @@ __device__ __forceinline__ double rcp64(double x) {
     return fma(r, e, r);
 }
 
-// dst(0:n) = src(0:n) for one column by one wavefront, 8 loads in flight per
-// lane; rows outside [rb, re) are known to be zero and are written, not read
+// dst(wb:we) = src(wb:we) for one column by one wavefront, 8 loads in flight
+// per lane; rows outside [rb, re) are known to be zero and are written, not read
 __device__ __forceinline__ void copy_column(double *__restrict__ dst,
-                                            const double *__restrict__ src, int n, int lane,
-                                            int rb = 0, int re = 1 << 30) {
+                                            const double *__restrict__ src, int wb, int we,
+                                            int lane, int rb = 0, int re = 1 << 30) {
     constexpr int kU = 8;
-    for (int r0 = 0; r0 < n; r0 += kU * PTB_WAVE) {
+    for (int r0 = wb; r0 < we; r0 += kU * PTB_WAVE) {
         double v[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int r = r0 + u * PTB_WAVE + lane;
-            v[u] = (r < n && r >= rb && r < re) ? src[r] : 0.0;
+            v[u] = (r < we && r >= rb && r < re) ? src[r] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int r = r0 + u * PTB_WAVE + lane;
-            if (r < n) dst[r] = v[u];
+            if (r < we) dst[r] = v[u];
         }
     }
 }
@@ -384,14 +384,15 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     __syncthreads();
     const int K = sK, nr = sNr;
     // Column types: bit 0 = rows above the split, bit 1 = rows below, from the
-    // child of origin and the rotations in order (zh is free until the roots).
+    // child of origin and the rotations in order; bit 2 = touched by a
+    // rotation (zh is free until the roots).
     int *cmask = reinterpret_cast<int *>(zh);
     for (int j = threadIdx.x; j < n; j += kMT) cmask[j] = ix[j] < n1 ? 1 : 2;
     __syncthreads();
     if (nr > 0 && threadIdx.x == 0)
         for (int q = 0; q < nr; ++q) {
             const int a = (int)rot[4 * q], b = (int)rot[4 * q + 1];
-            const int mm = cmask[a] | cmask[b];
+            const int mm = cmask[a] | cmask[b] | 4;
             cmask[a] = mm;
             cmask[b] = mm;
         }
@@ -402,7 +403,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         const int ib = min(K, (int)threadIdx.x * chunk), ie = min(K, ib + chunk);
         int cl = 0, cd = 0;
         for (int i = ib; i < ie; ++i) {
-            const int t = cmask[org[i]];
+            const int t = cmask[org[i]] & 3;
             cl += t == 1;
             cd += t == 3;
         }
@@ -411,7 +412,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         int pd = block_scan_excl(cd, wsum, &KD);
         int pr = ib - pl - pd;  // right-only poles before ib
         for (int i = ib; i < ie; ++i) {
-            const int t = cmask[org[i]];
+            const int t = cmask[org[i]] & 3;
             prow[i] = t == 1 ? pl++ : (t == 3 ? KL + pd++ : KL + KD + pr++);
         }
         if (threadIdx.x == 0) {
@@ -425,13 +426,17 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     const int KL = sKL, KD = sKD;
     // sorted, compacted eigenvector columns into W: W(:, slot[j]) = Z(:, lo+ix[j]),
     // one column per wavefront.  A child's column is nonzero only on the
-    // child's rows; the rest of the node block is written as zeros, never
-    // read, so the arena needs no clearing beforehand.
+    // child's rows, which are the only rows read.  Left-only and right-only
+    // survivors that no rotation touches feed one of the two products, so only
+    // their child's rows are written; every other column (rotated row by row
+    // below, or copied back whole) gets the other rows as zeros, so the arena
+    // needs no clearing beforehand.
     for (int j = wid; j < n; j += kWaves) {
-        const int c = ix[j];
-        copy_column(W + (int64_t)(nd.lo + slot[j]) * ld + nd.lo,
-                    Z + (int64_t)(nd.lo + c) * ld + nd.lo, n, lane, c < n1 ? 0 : n1,
-                    c < n1 ? n1 : n);
+        const int c = ix[j], sl = slot[j];
+        const int cb = c < n1 ? 0 : n1, ce = c < n1 ? n1 : n;
+        const bool half = !(cmask[j] & 4) && (sl < KL || (sl >= KL + KD && sl < K));
+        copy_column(W + (int64_t)(nd.lo + sl) * ld + nd.lo, Z + (int64_t)(nd.lo + c) * ld + nd.lo,
+                    half ? cb : 0, half ? ce : n, lane, cb, ce);
     }
     __syncthreads();
     // deflation rotations, in order, row-parallel
@@ -449,7 +454,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     // deflated columns and eigenvalues go straight to their final place
     for (int c = K + wid; c < n; c += kWaves)
         copy_column(Z + (int64_t)(nd.lo + c) * ld + nd.lo, W + (int64_t)(nd.lo + c) * ld + nd.lo,
-                    n, lane);
+                    0, n, lane);
     for (int j = threadIdx.x; j < n; j += kMT)
         if (slot[j] >= K) D[nd.lo + slot[j]] = Dk[j];
     __syncthreads();
