@@ -423,8 +423,9 @@ __global__ __launch_bounds__(kT) void k_zero_upper(const SubDesc *__restrict__ d
     for (int r = threadIdx.x; r <= c; r += kT) col[r] = 0.0;
 }
 
-// Float copy of every arena block (column c of submatrix s per workgroup row):
-// the input of the FP32 tridiagonalisation.  grid: (submatrix, column)
+// Float copy of the lower triangle of every arena block (column c of
+// submatrix s per workgroup row): the input of the FP32 tridiagonalisation.
+// grid: (submatrix, column)
 __global__ __launch_bounds__(kT) void k_to_f32(const SubDesc *__restrict__ desc,
                                                const double *__restrict__ Aa,
                                                float *__restrict__ Fa) {
@@ -433,7 +434,8 @@ __global__ __launch_bounds__(kT) void k_to_f32(const SubDesc *__restrict__ desc,
     if (c >= d.m) return;
     const double *src = Aa + d.off + (int64_t)c * d.ld;
     float *dst = Fa + d.off + (int64_t)c * d.ld;
-    for (int r = threadIdx.x; r < d.m; r += kT) dst[r] = (float)src[r];
+    // the lower triangle only: the float tridiagonalisation never reads above it
+    for (int r = c + threadIdx.x; r < d.m; r += kT) dst[r] = (float)src[r];
 }
 
 // Reflectors of the FP32 tridiagonalisation in FP64: V(:, c) = the stored
