@@ -661,6 +661,35 @@ __global__ __launch_bounds__(kT) void k_larft_merge(const SubDesc *__restrict__ 
     }
 }
 
+// Diagonal blocks of every 128-reflector factor T = [T1 T12; 0 T2] from the
+// 64-reflector factors of its halves; T12 = -T1 (V1^T V2) T2 is written by
+// three batched products afterwards (zero when the second half is absent).
+// T128 is 128 x 128 column-major at T128a + toff128[s] + p * 16384.
+// grid: (submatrix, block)
+__global__ __launch_bounds__(kT) void k_t128_assemble(const SubDesc *__restrict__ desc,
+                                                      const double *__restrict__ T64a,
+                                                      const int64_t *__restrict__ toff64,
+                                                      const int64_t *__restrict__ toff128,
+                                                      double *__restrict__ T128a) {
+    const SubDesc d = desc[blockIdx.x];
+    const int nr = d.m - 2;
+    const int n64 = nr > 0 ? (nr + 63) / 64 : 0;
+    const int p = blockIdx.y;
+    if (2 * p >= n64) return;
+    const bool two = 2 * p + 1 < n64;
+    const double *T1 = T64a + toff64[blockIdx.x] + (int64_t)(2 * p) * 4096;
+    const double *T2 = T1 + 4096;
+    double *T = T128a + toff128[blockIdx.x] + (int64_t)p * 16384;
+    for (int l = threadIdx.x; l < 16384; l += kT) {  // l = c * 128 + r
+        const int r = l & 127, c = l >> 7;
+        if (r < 64 && c >= 64 && two) continue;  // T12
+        double v = 0.0;
+        if (r < 64 && c < 64) v = T1[c * 64 + r];
+        else if (r >= 64 && c >= 64 && two) v = T2[(c - 64) * 64 + (r - 64)];
+        T[l] = v;
+    }
+}
+
 // Inverse of every 64x64 diagonal block of each lower-triangular L (identity
 // padding past m).  grid: (block, submatrix); Dinv block b of submatrix s at
 // dinv + s_doff + b*4096, column-major, leading dimension 64.
@@ -1168,29 +1197,86 @@ int larft_all(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
                            qf.T, dtoff, dtoff64, qf.T64);
         PTB_CHECK_LAUNCH(ctx);
     }
-    return PTBGPU_OK;
+    // 128-reflector factors from pairs of 64-reflector ones:
+    //   T = [T1 T12; 0 T2],  T12 = -T1 (V1^T V2) T2,
+    // V1^T V2 over the rows where V2 is nonzero (from its first unit on)
+    qf.toff128.assign(nsm, 0);
+    int64_t tot128 = 0, npairs = 0;
+    for (int s = 0; s < nsm; ++s) {
+        const int nr = desc[s].m - 2;
+        const int n64 = nr > 0 ? (nr + 63) / 64 : 0;
+        qf.toff128[s] = tot128;
+        tot128 += (int64_t)((n64 + 1) / 2) * 16384;
+        npairs += n64 / 2;
+    }
+    qf.maxblk128 = (qf.maxblk64 + 1) / 2;
+    qf.T128 = static_cast<double *>(ws(ctx, "q_T128", sizeof(double) * std::max<int64_t>(tot128, 1)));
+    auto *dtoff128 = static_cast<int64_t *>(upload_table(ctx, "q_toff128", qf.toff128.data(), sizeof(int64_t) * nsm));
+    auto *GM = static_cast<double *>(ws(ctx, "q_t128_gm", sizeof(double) * std::max<int64_t>(8192 * npairs, 1)));
+    if (!qf.T128 || !dtoff128 || !GM) return PTBGPU_E_NOMEM;
+    if (qf.maxblk128 == 0) return PTBGPU_OK;
+    hipLaunchKernelGGL(k_t128_assemble, dim3(nsm, qf.maxblk128), dim3(kT), 0, ctx->stream, ddesc,
+                       qf.T64, dtoff64, dtoff128, qf.T128);
+    PTB_CHECK_LAUNCH(ctx);
+    std::vector<GemmDesc> ga, gm, gt;
+    int64_t sc = 0;
+    for (int s = 0; s < nsm; ++s) {
+        const SubDesc &d = desc[s];
+        const int nr = d.m - 2;
+        const int n64 = nr > 0 ? (nr + 63) / 64 : 0;
+        for (int p = 0; 2 * p + 1 < n64; ++p) {
+            const int b0 = 128 * p, kb2 = std::min(64, nr - (b0 + 64)), r0 = b0 + 65;
+            const double *V1 = V + d.off + (int64_t)b0 * d.ld + r0;
+            const double *V2 = V + d.off + (int64_t)(b0 + 64) * d.ld + r0;
+            const double *T1 = qf.T64 + qf.toff64[s] + (int64_t)(2 * p) * 4096, *T2 = T1 + 4096;
+            double *G = GM + sc, *Mm = G + 4096;
+            sc += 8192;
+            double *T12 = qf.T128 + qf.toff128[s] + (int64_t)p * 16384 + 64 * 128;
+            ga.push_back({V1, V2, G, 64, kb2, d.m - r0, d.ld, d.ld, 64, 1.0, 0.0, 0, 0});  // V1^T V2
+            gm.push_back({G, T2, Mm, 64, kb2, kb2, 64, 64, 64, 1.0, 0.0, 0, kTriBUpper});  // G T2
+            gt.push_back({T1, Mm, T12, 64, kb2, 64, 64, 64, 128, -1.0, 0.0, 0, kTriAUpper});  // -T1 M
+        }
+    }
+    if (ga.empty()) return PTBGPU_OK;
+    const size_t gbytes = sizeof(GemmDesc) * ga.size();
+    auto *dga = static_cast<GemmDesc *>(upload_table(ctx, "q_t128_ga", ga.data(), gbytes));
+    auto *dgm = static_cast<GemmDesc *>(upload_table(ctx, "q_t128_gm_d", gm.data(), gbytes));
+    auto *dgt = static_cast<GemmDesc *>(upload_table(ctx, "q_t128_gt", gt.data(), gbytes));
+    if (!dga || !dgm || !dgt) return PTBGPU_E_NOMEM;
+    TileList ta, tm, tt;
+    ta.add(ga.data(), (int)ga.size());
+    tm.add(gm.data(), (int)gm.size());
+    tt.add(gt.data(), (int)gt.size());
+    PTB_TRY(ta.upload(ctx, "q_t128_wa"));
+    PTB_TRY(tm.upload(ctx, "q_t128_wm"));
+    PTB_TRY(tt.upload(ctx, "q_t128_wt"));
+    PTB_TRY(ta.launch(ctx, 0, true, false, dga, "larft"));
+    PTB_TRY(tm.launch(ctx, 0, false, false, dgm, "larft"));
+    return tt.launch(ctx, 0, false, false, dgt, "larft");
 }
 
 // Y <- Q Y (trans = false; blocks from the last to the first) or Y <- Q^T Y
 // (trans = true; blocks from the first, Q_b^T = I - V_b T_b^T V_b^T) for
 // ncols[s] columns of submatrix s at Y + yoff[s] (leading dimension ld), in
-// 64-reflector blocks.  Per block: W1 = V_b^T Y, W2 = T_b W1 (or T_b^T W1),
-// Y -= V_b W2, each a batched GEMM; W1 and W2 are 64 x ncols panels in engine
-// workspace.  V_b spans the block's 64 columns from the row after its first
-// reflector's pivot; its second half is zero above its own units.
+// 128-reflector blocks (each block reads and rewrites Y once).  Per block:
+// W1 = V_b^T Y, W2 = T_b W1 (or T_b^T W1), Y -= V_b W2, each a batched GEMM;
+// W1 and W2 are 128 x ncols panels in engine workspace.  V_b spans the
+// block's columns from the row after its first reflector's pivot; every
+// column is zero above its own unit.
 int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf, double *Y,
             const std::vector<int64_t> &yoff, const std::vector<int> &ncols, bool trans,
             const char *tag) {
     ProfScope prof_(ctx, "apply_q");
     ProfScope tprof_(ctx, (std::string("apply_q_") + tag).c_str());
     const int nsm = (int)desc.size();
-    if (qf.maxblk64 == 0) return PTBGPU_OK;
+    if (qf.maxblk128 == 0) return PTBGPU_OK;
+    constexpr int kQB = 128;  // reflectors per block
     std::vector<int64_t> woff(nsm);
     int64_t wtot = 0;
     int max_m = 0, max_c = 0;
     for (int s = 0; s < nsm; ++s) {
         woff[s] = wtot;
-        wtot += 64 * (int64_t)std::max(ncols[s], 0);
+        wtot += kQB * (int64_t)std::max(ncols[s], 0);
         max_m = std::max(max_m, desc[s].m);
         max_c = std::max(max_c, ncols[s]);
     }
@@ -1199,28 +1285,28 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
     auto *W2 = static_cast<double *>(ws(ctx, "q_w2", sizeof(double) * std::max<int64_t>(wtot, 1)));
     if (!W1 || !W2) return PTBGPU_E_NOMEM;
     std::vector<GemmDesc> g1, g2, g3;
-    std::vector<int> start(qf.maxblk64 + 1, 0);
-    for (int q = 0; q < qf.maxblk64; ++q) {
+    std::vector<int> start(qf.maxblk128 + 1, 0);
+    for (int q = 0; q < qf.maxblk128; ++q) {
         start[q] = (int)g1.size();
         for (int s = 0; s < nsm; ++s) {
             const SubDesc &d = desc[s];
             const int nc = ncols[s];
             const int nr = d.m - 2;
-            const int npair = nr > 0 ? (nr + 63) / 64 : 0;
-            if (nc <= 0 || q >= npair) continue;
-            const int p = trans ? q : npair - 1 - q;
-            const int b0 = p * 64, kb = std::min(64, nr - b0);
+            const int nblk = nr > 0 ? (nr + kQB - 1) / kQB : 0;
+            if (nc <= 0 || q >= nblk) continue;
+            const int p = trans ? q : nblk - 1 - q;
+            const int b0 = p * kQB, kb = std::min(kQB, nr - b0);
             const int rowsv = d.m - b0 - 1;
             const double *V = qf.V + d.off + (int64_t)b0 * d.ld + b0 + 1;
             double *Yb = Y + yoff[s] + b0 + 1;
-            const double *T = qf.T64 + qf.toff64[s] + (int64_t)p * 4096;
+            const double *T = qf.T128 + qf.toff128[s] + (int64_t)p * kQB * kQB;
             double *w1 = W1 + woff[s], *w2 = W2 + woff[s];
-            g1.push_back({V, Yb, w1, kb, nc, rowsv, d.ld, d.ld, 64, 1.0, 0.0});
-            g2.push_back({T, w1, w2, kb, nc, kb, 64, 64, 64, 1.0, 0.0});
-            g3.push_back({V, w2, Yb, rowsv, nc, kb, d.ld, 64, d.ld, -1.0, 1.0});
+            g1.push_back({V, Yb, w1, kb, nc, rowsv, d.ld, d.ld, kQB, 1.0, 0.0});
+            g2.push_back({T, w1, w2, kb, nc, kb, kQB, kQB, kQB, 1.0, 0.0});
+            g3.push_back({V, w2, Yb, rowsv, nc, kb, d.ld, kQB, d.ld, -1.0, 1.0});
         }
     }
-    start[qf.maxblk64] = (int)g1.size();
+    start[qf.maxblk128] = (int)g1.size();
     const std::string t(tag);
     if (g1.empty()) return PTBGPU_OK;
     const size_t gb = sizeof(GemmDesc) * g1.size();
@@ -1229,7 +1315,7 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
     GemmDesc *dg3 = static_cast<GemmDesc *>(upload_table(ctx, ("q_g3_" + t).c_str(), g3.data(), gb));
     if (!dg1 || !dg2 || !dg3) return PTBGPU_E_NOMEM;
     TileList t1, t2, t3;
-    for (int q = 0; q < qf.maxblk64; ++q) {
+    for (int q = 0; q < qf.maxblk128; ++q) {
         const int c = start[q + 1] - start[q];
         t1.add(g1.data() + start[q], c);
         t2.add(g2.data() + start[q], c);
@@ -1238,7 +1324,7 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
     PTB_TRY(t1.upload(ctx, ("q_w1_" + t).c_str()));
     PTB_TRY(t2.upload(ctx, ("q_w2_" + t).c_str()));
     PTB_TRY(t3.upload(ctx, ("q_w3_" + t).c_str()));
-    for (int q = 0; q < qf.maxblk64; ++q) {
+    for (int q = 0; q < qf.maxblk128; ++q) {
         if (start[q + 1] == start[q]) continue;
         PTB_TRY(t1.launch(ctx, q, true, false, dg1 + start[q], "q"));
         PTB_TRY(t2.launch(ctx, q, trans, false, dg2 + start[q], "q"));

@@ -83,9 +83,12 @@ struct QFactor {
     double *T = nullptr;          // 32 x 32 factors of the 32-reflector blocks
     std::vector<int64_t> toff;  // per submatrix, element offset of its first T block
     int maxblk = 0;
-    double *T64 = nullptr;        // 64 x 64 factors of pairs of blocks (what apply_q uses)
+    double *T64 = nullptr;        // 64 x 64 factors of pairs of blocks
     std::vector<int64_t> toff64;
     int maxblk64 = 0;
+    double *T128 = nullptr;       // 128 x 128 factors of pairs of those (what apply_q uses)
+    std::vector<int64_t> toff128;
+    int maxblk128 = 0;
 };
 
 // Right-hand sides of a batched triangular solve that are not the submatrices'
