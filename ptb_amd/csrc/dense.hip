@@ -22,6 +22,11 @@
 namespace {
 
 constexpr int kT = 256;
+// reflectors per block of the WY application of Q.  Every reader of the
+// reflector arena (T factors, Q application) reads column c only from row
+// (c rounded down to a block) + 1 on, so only those rows above the unit need
+// to hold zeros.
+constexpr int kQBlk = 128;
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
@@ -420,7 +425,8 @@ __global__ __launch_bounds__(kT) void k_zero_upper(const SubDesc *__restrict__ d
     const int c = blockIdx.y;
     if (c >= d.m) return;
     double *col = Aa + d.off + (int64_t)c * d.ld;
-    for (int r = threadIdx.x; r <= c; r += kT) col[r] = 0.0;
+    // rows above the unit that the block readers see (see kQBlk)
+    for (int r = (c & ~(kQBlk - 1)) + 1 + threadIdx.x; r <= c; r += kT) col[r] = 0.0;
 }
 
 // Float copy of the lower triangle of every arena block (column c of
@@ -454,7 +460,8 @@ __global__ __launch_bounds__(kT) void k_promote_reflectors(const SubDesc *__rest
     const float *src = Fa + d.off + (int64_t)c * d.ld;
     double *dst = Va + d.off + (int64_t)c * d.ld;
     double ss = 0.0;
-    for (int r = lane; r < d.m; r += PTB_WAVE) {
+    // zeros only in the rows above the unit that the block readers see (see kQBlk)
+    for (int r = (c & ~(kQBlk - 1)) + 1 + lane; r < d.m; r += PTB_WAVE) {
         const double v = r > c ? (double)src[r] : 0.0;
         dst[r] = v;
         ss += v * v;
@@ -1270,7 +1277,7 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
     ProfScope tprof_(ctx, (std::string("apply_q_") + tag).c_str());
     const int nsm = (int)desc.size();
     if (qf.maxblk128 == 0) return PTBGPU_OK;
-    constexpr int kQB = 128;  // reflectors per block
+    constexpr int kQB = kQBlk;
     std::vector<int64_t> woff(nsm);
     int64_t wtot = 0;
     int max_m = 0, max_c = 0;
