@@ -25,6 +25,25 @@ def test_matrix_sign(engine, n):
     assert np.abs(engine.matrix_sign(Ag) - S).max() < 1e-10
 
 
+@pytest.mark.parametrize("n", [97, 301])
+def test_matrix_sign_degenerate_spectrum(engine, n):
+    """Four eigenvalues, each repeated n/4 times: the divide-and-conquer merges
+    deflate most of their poles, rotating pairs across the two children (the
+    rotation path and the dense-column products).  Closed forms:
+    sign(A) = Q sign(L) Q^T and A^{-1/2} = Q L^{-1/2} Q^T (A = Q L Q^T)."""
+    rng = np.random.default_rng(7 + n)
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    lam = np.array([-2.0, -1.0, 1.0, 3.0])[np.arange(n) % 4]
+    A = (Q * lam) @ Q.T
+    A = 0.5 * (A + A.T)
+    S = engine.matrix_sign(A)
+    assert np.abs(S - (Q * np.sign(lam)) @ Q.T).max() < 1e-10
+    assert np.abs(S @ S - np.eye(n)).max() < 1e-10
+    P = A + 3.0 * np.eye(n)  # eigenvalues 1, 2, 4, 6
+    R = engine.matrix_root(P)
+    assert np.abs(R - (Q * (lam + 3.0) ** -0.5) @ Q.T).max() < 1e-10
+
+
 @pytest.mark.parametrize("n", [5, 64, 250])
 def test_matrix_root(engine, n):
     from oracle import ptb_oracle as O
