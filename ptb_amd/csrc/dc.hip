@@ -156,6 +156,36 @@ __device__ __forceinline__ double rcp64(double x) {
     return fma(r, e, r);
 }
 
+// s = sum_{i0 <= i < i1} q_i, ds = sum q_i r_i with r_i = 1 / ((d_i - dorg) - t),
+// q_i = w2_i r_i: the secular function's pole terms and their derivatives.
+// Four interleaved partial sums, so that the terms' dependent chains
+// (reciprocal, Newton steps, accumulation) overlap.
+__device__ __forceinline__ void secular_sums(const double *dl, const double *w2, int i0, int i1,
+                                             double dorg, double t, double &s, double &ds) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+    int i = i0;
+    for (; i + 4 <= i1; i += 4) {
+        const double r0 = rcp64((dl[i] - dorg) - t), r1 = rcp64((dl[i + 1] - dorg) - t);
+        const double r2 = rcp64((dl[i + 2] - dorg) - t), r3 = rcp64((dl[i + 3] - dorg) - t);
+        const double q0 = w2[i] * r0, q1 = w2[i + 1] * r1, q2 = w2[i + 2] * r2, q3 = w2[i + 3] * r3;
+        a0 += q0;
+        a1 += q1;
+        a2 += q2;
+        a3 += q3;
+        b0 += q0 * r0;
+        b1 += q1 * r1;
+        b2 += q2 * r2;
+        b3 += q3 * r3;
+    }
+    for (; i < i1; ++i) {
+        const double r = rcp64((dl[i] - dorg) - t), q = w2[i] * r;
+        a0 += q;
+        b0 += q * r;
+    }
+    s = (a0 + a1) + (a2 + a3);
+    ds = (b0 + b1) + (b2 + b3);
+}
+
 // dst(wb:we) = src(wb:we) for one column by one wavefront, 8 loads in flight
 // per lane; rows outside [rb, re) are known to be zero and are written, not read
 __device__ __forceinline__ void copy_column(double *__restrict__ dst,
@@ -467,8 +497,9 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         double lo, hi;
         if (j < K - 1) {
             const double mid = 0.5 * (dl[j + 1] - dl[j]);
-            double f = 1.0;
-            for (int i = 0; i < K; ++i) f += w2[i] * rcp64((dl[i] - dl[j]) - mid);
+            double fs, fds;
+            secular_sums(dl, w2, 0, K, dl[j], mid, fs, fds);
+            const double f = 1.0 + fs;
             if (f >= 0.0) {
                 o = j;
                 lo = 0.0;
@@ -498,19 +529,9 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         const double da = dl[j] - dorg, db = last ? 0.0 : dl[j + 1] - dorg;
         double t = 0.5 * (lo + hi);
         for (int it = 0; it < 200; ++it) {
-            double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
-            for (int i = 0; i <= j; ++i) {
-                const double r = rcp64((dl[i] - dorg) - t);
-                const double q = w2[i] * r;
-                psi += q;
-                dpsi += q * r;
-            }
-            for (int i = j + 1; i < K; ++i) {
-                const double r = rcp64((dl[i] - dorg) - t);
-                const double q = w2[i] * r;
-                phi += q;
-                dphi += q * r;
-            }
+            double psi, dpsi, phi, dphi;
+            secular_sums(dl, w2, 0, j + 1, dorg, t, psi, dpsi);
+            secular_sums(dl, w2, j + 1, K, dorg, t, phi, dphi);
             const double f = 1.0 + psi + phi;
             if (f == 0.0 || fabs(f) <= 8.0 * kEps * (double)K * (1.0 + fabs(psi) + fabs(phi))) {
                 lo = hi = t;
