@@ -1060,51 +1060,58 @@ int trsm_gemm(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
     return PTBGPU_OK;
 }
 
-// 64-column blocks: factor panel 2b, apply its rank-32 update to the 32
-// columns of panel 2b+1 only, factor panel 2b+1, then one rank-64 update of the
-// trailing matrix (lower triangle), so the trailing matrix is read and
-// written once per 64 columns.
+// Blocks of kPB = 4 panels of 32 columns (right-looking within a block):
+// after panel j of a block, one rank-32 update of the block's remaining
+// columns; after the block, one rank-128 update of the trailing matrix (lower
+// triangle), so the trailing matrix is read and written once per 128 columns.
 int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                 double *A, int *flags) {
     ProfScope prof_(ctx, "potrf");
+    constexpr int kPB = 4, kBW = 32 * kPB;
     const int nsm = (int)desc.size();
     int max_m = 0;
     for (const auto &x : desc) max_m = std::max(max_m, x.m);
     const int npanel = (max_m + 31) / 32;
-    const int nblk = (max_m + 63) / 64;
+    const int nblk = (max_m + kBW - 1) / kBW;
     std::vector<GemmDesc> gs, gb;
-    std::vector<int> ss(nblk + 1, 0), sb(nblk + 1, 0);
+    // gs launches (block b, panel j < kPB - 1) in order; gb launches per block
+    std::vector<int> ss(nblk * (kPB - 1) + 1, 0), sb(nblk + 1, 0);
     for (int b = 0; b < nblk; ++b) {
-        ss[b] = (int)gs.size();
+        const int k0 = kBW * b;
+        for (int j = 0; j < kPB - 1; ++j) {
+            ss[b * (kPB - 1) + j] = (int)gs.size();
+            for (int si = 0; si < nsm; ++si) {
+                const SubDesc &x = desc[si];
+                double *a = A + x.off;
+                const int pc = k0 + 32 * j, s1 = pc + 32, se = std::min(k0 + kBW, x.m);
+                if (s1 < se) {  // block columns s1..se-1 -= P P(s1:se, :)^T
+                    double *P = a + (int64_t)pc * x.ld + s1;
+                    gs.push_back({P, P, a + (int64_t)s1 * x.ld + s1, x.m - s1, se - s1, 32, x.ld, x.ld,
+                                  x.ld, -1.0, 1.0, 1, 0});
+                }
+            }
+        }
         sb[b] = (int)gb.size();
         for (int si = 0; si < nsm; ++si) {
             const SubDesc &x = desc[si];
             double *a = A + x.off;
-            const int k0 = 64 * b, s1 = k0 + 32, s2 = k0 + 64;
-            if (s1 < x.m) {  // panel 2b+1 -= P P(s1:s1+w, :)^T
-                const int n = x.m - s1, w = std::min(32, n);
-                double *P = a + (int64_t)k0 * x.ld + s1;
-                gs.push_back({P, P, a + (int64_t)s1 * x.ld + s1, n, w, 32, x.ld, x.ld, x.ld, -1.0, 1.0,
-                              1, 0});
-            }
-            if (s2 < x.m) {  // trailing matrix -= [P P'] [P P']^T
+            const int s2 = k0 + kBW;
+            if (s2 < x.m) {  // trailing matrix -= P P^T, P = the block's columns
                 const int n = x.m - s2;
                 double *P = a + (int64_t)k0 * x.ld + s2;
-                gb.push_back({P, P, a + (int64_t)s2 * x.ld + s2, n, n, 64, x.ld, x.ld, x.ld, -1.0, 1.0,
+                gb.push_back({P, P, a + (int64_t)s2 * x.ld + s2, n, n, kBW, x.ld, x.ld, x.ld, -1.0, 1.0,
                               1, 0});
             }
         }
     }
-    ss[nblk] = (int)gs.size();
+    ss[nblk * (kPB - 1)] = (int)gs.size();
     sb[nblk] = (int)gb.size();
     auto *dgs = static_cast<GemmDesc *>(upload_table(ctx, "potrf_gs", gs.data(), sizeof(GemmDesc) * gs.size()));
     auto *dgb = static_cast<GemmDesc *>(upload_table(ctx, "potrf_gb", gb.data(), sizeof(GemmDesc) * gb.size()));
     if (!dgs || !dgb) return PTBGPU_E_NOMEM;
     TileList ts, tb;
-    for (int b = 0; b < nblk; ++b) {
-        ts.add(gs.data() + ss[b], ss[b + 1] - ss[b]);
-        tb.add(gb.data() + sb[b], sb[b + 1] - sb[b]);
-    }
+    for (int l = 0; l < nblk * (kPB - 1); ++l) ts.add(gs.data() + ss[l], ss[l + 1] - ss[l]);
+    for (int b = 0; b < nblk; ++b) tb.add(gb.data() + sb[b], sb[b + 1] - sb[b]);
     PTB_TRY(ts.upload(ctx, "potrf_ws"));
     PTB_TRY(tb.upload(ctx, "potrf_wb"));
     auto panel = [&](int p) -> int {
@@ -1114,9 +1121,14 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
         return PTBGPU_OK;
     };
     for (int b = 0; b < nblk; ++b) {
-        PTB_TRY(panel(2 * b));
-        PTB_TRY(ts.launch(ctx, b, false, true, dgs + ss[b], "potrf"));
-        if (2 * b + 1 < npanel) PTB_TRY(panel(2 * b + 1));
+        for (int j = 0; j < kPB; ++j) {
+            const int p = kPB * b + j;
+            if (p < npanel) PTB_TRY(panel(p));
+            if (j < kPB - 1) {
+                const int l = b * (kPB - 1) + j;
+                PTB_TRY(ts.launch(ctx, l, false, true, dgs + ss[l], "potrf"));
+            }
+        }
         PTB_TRY(tb.launch(ctx, b, false, true, dgb + sb[b], "potrf"));
     }
     return PTBGPU_OK;
