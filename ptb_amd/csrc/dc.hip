@@ -266,7 +266,6 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     __shared__ double red[kMT / PTB_WAVE];
     __shared__ int wsum[kMT / PTB_WAVE];
     __shared__ int sK, sNr, sClose, sKL, sKD;
-    __shared__ double srho;
 
     const double beta = e_all[ds.mapoff + nd.sp - 1];
     const double rho = 2.0 * fabs(beta);
@@ -409,7 +408,6 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
             if (slot[j] == -1) slot[j] = nd_++;
         sK = K;
         sNr = nr;
-        srho = rho;
     }
     __syncthreads();
     const int K = sK, nr = sNr;
@@ -618,7 +616,6 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         gg.uplo = 0;
         gg.tri = 0;
     }
-    (void)srho;
 }
 
 }  // namespace
