@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
     const int gm = m0 + lane;
     for (int cc = wave; cc < 64; cc += kT / 64) {
         const int gn = n0 + cc;
-        if (gm >= g.M || gn >= g.N || (g.uplo && gm < gn)) continue;
+        if (gm >= g.M || gn >= g.N || (g.uplo == 1 && gm < gn)) continue;
         auto *c = gC + (int64_t)gn * g.ldc + gm;
         const double v = Ct[cc * 65 + lane];
         *c = (T)(g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * (double)*c);
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(kT, 4) void k_gemm_nt1(const GemmDesc *__restrict__
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
         const int cc = wave + 4 * t, gn = n0 + cc;
-        if (gm >= g.M || gn >= g.N || (g.uplo && gm < gn)) continue;
+        if (gm >= g.M || gn >= g.N || (g.uplo == 1 && gm < gn)) continue;
         const double v = g.alpha * Ct[cc * 65 + lane];
         gC[(unsigned)gn * uldc + gm] = (T)(g.beta == 0.0 ? v : v + g.beta * (double)craw[t]);
     }

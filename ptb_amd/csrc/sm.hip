@@ -506,14 +506,16 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     PTB_TRY(ptb::potrf_batch(ctx, desc, ddesc, B, flags));
     // K = L^-1 H L^-T from Li = L^-1: Z = Li H, A = Z Li^T (the lower triangle,
     // which the tridiagonalisation reads; the mixed solve, whose refinement
-    // multiplies by K, mirrors it into the upper one)
+    // multiplies by K, mirrors it into the upper one).  A(i, j), i >= j, needs
+    // Z(i, k) for k <= j only, so Z is formed on the tiles on and below the
+    // diagonal (diagonal tiles whole: the second product reads them whole)
     const bool mixed = opts.precision == PTBGPU_MIXED;
     PTB_TRY(ptb::inv_lower_batch(ctx, desc, ddesc, B, Li, X));
     {
         std::vector<GemmDesc> g1(nsm), g2(nsm);
         for (int s = 0; s < nsm; ++s) {
             const SubDesc &d = desc[s];
-            g1[s] = {Li + d.off, A + d.off, Z + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0,
+            g1[s] = {Li + d.off, A + d.off, Z + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 2,
                      kTriALower};
             g2[s] = {Z + d.off, Li + d.off, A + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 1,
                      kTriBUpper};

@@ -39,6 +39,8 @@ constexpr int kBisStride = 4 + 200;
 // One product C = alpha op(A) op(B) + beta C of a variable-size batch.  With
 // uplo = 1 only the lower triangle of C (row >= column) is formed: tiles above
 // the diagonal are skipped (symmetric updates such as A -= V W^T + W V^T).
+// uplo = 2 skips the same tiles but writes the diagonal tiles whole (for a
+// product whose reader needs the lower tiles only).
 // tri marks op(A) / op(B) as triangular (kTri* bits): each output tile runs
 // only over the k range where both factors can be nonzero (triangular products
 // at about half the flops).  The other triangle of such a factor must be
