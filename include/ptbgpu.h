@@ -10,7 +10,15 @@
  * Entry points without the _dev suffix take host arrays (the true drop-in:
  * the engine copies in and out).  The _dev variants take device pointers on
  * the context's device (inputs already resident in HBM); xyz, aoat and all
- * scalar outputs stay on the host.
+ * scalar outputs stay on the host.  Device arrays must hold the element counts
+ * stated for each call (packed matrices: ndim*(ndim+1)/2 doubles).
+ *
+ * Ordering of _dev calls: the context's stream is a non-blocking stream and
+ * does not wait on any other stream.  The caller must have completed the work
+ * that produces the device inputs (synchronized its stream, or made the
+ * context's stream wait on an event recorded after that work, see
+ * ptbgpu_stream) before the call.  Every call returns with its own work on
+ * the context's stream complete, so outputs are ready for any stream.
  *
  * Threading: one context per host thread; calls on one context are not
  * re-entrant.  Each context owns one HIP stream (default priority) and a
@@ -45,10 +53,16 @@ enum ptbgpu_status {
 
 /* arithmetic of the per-submatrix solve */
 enum ptbgpu_precision {
-    PTBGPU_FP64 = 0,   /* plain FP64 drop-in */
-    PTBGPU_MIXED = 1   /* tridiagonalisation on float storage + FP64 refinement of
-                          the occupied eigenvectors (Cholesky, K, divide and
-                          conquer, fe, mu and the density stay FP64) */
+    PTBGPU_FP64 = 0,       /* plain FP64 drop-in */
+    PTBGPU_MIXED = 1,      /* the tridiagonalisation on float storage, the divide-
+                              and-conquer merge products and the refinement's
+                              correction products on the BF16 matrix cores as
+                              split-BF16x3 with FP32 accumulation, then one FP64
+                              refinement of the occupied eigenvectors (Cholesky,
+                              K, the secular equations, fe, mu and the density
+                              stay FP64) */
+    PTBGPU_MIXED_BF16 = 2  /* the same with plain BF16 products (8-bit operands);
+                              for accuracy studies of the precision ladder */
 };
 
 typedef struct ptbgpu_ctx ptbgpu_ctx;
@@ -121,6 +135,20 @@ int ptbgpu_stream(ptbgpu_ctx *ctx, void **stream);
 int ptbgpu_profile(ptbgpu_ctx *ctx, int64_t enable);
 int ptbgpu_profile_query(ptbgpu_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 
+/* Batched products C_i = alpha op(A_i) op(B_i) + beta C_i on the engine's
+ * matrix-core GEMM kernels (no reference counterpart: the product family the
+ * eigensolver runs on, exposed for tests and tile tuning).  Device pointers,
+ * column-major FP64 storage; op(A_i) is M_i x K_i, op(B_i) K_i x N_i;
+ * transa/transb 1 = transposed.  precision: 0 FP64 MFMA; 1 split-BF16x3 and
+ * 2 BF16 on the BF16 matrix cores with FP32 accumulation (each operand
+ * element x enters as bf16(x) + bf16(x - bf16(x)), or bf16(x) alone).
+ * Dimensions <= 32768; each operand below 2^32 elements. */
+int ptbgpu_gemm_batch_dev(ptbgpu_ctx *ctx, int64_t precision, int64_t transa, int64_t transb,
+                          int64_t count, const int64_t *M, const int64_t *N, const int64_t *K,
+                          const double *const *A, const int64_t *lda, const double *const *B,
+                          const int64_t *ldb, double *const *C, const int64_t *ldc,
+                          double alpha, double beta);
+
 /* ---- filter / analyse ----------------------------------------------------
  * Replaces filter_matrix_sym (source/pgtb.f90:1532-1548): A(k) = 0 where
  * abs(A(k)) < filter (strict), in place over the packed matrix.  counts, if
@@ -187,6 +215,23 @@ int ptbgpu_sm_density_batch_dev(ptbgpu_ctx *ctx, int64_t nmol, const int64_t *nd
 int ptbgpu_sm_spectra(ptbgpu_ctx *ctx, int64_t nsm, int64_t sum_m, int64_t *m, double *eig,
                       double *fe);
 
+/* ---- populations -------------------------------------------------------------
+ * Replaces mlpop2(n, nao, p, s1, s2, q, qsh) (source/mlpop.f:198-238): atomic
+ * populations q(n) and shell populations qsh(10, n) (column-major) from the
+ * packed P and s1 = S^3/4, s2 = S^1/4 (REAL(4), nao x nao column-major, as
+ * mlpop14 makes them, source/mlpop.f:142-190): diag(S^1/4 x S^3/4) with x = P
+ * rounded to float, summed over the AOs of each atom (aoat, 1..n) and each
+ * shell (shell2ao, 1..10) in AO order.  The products run in FP64 on the
+ * float-rounded inputs; the result differs from the reference's REAL(4)
+ * products by their rounding.  nao <= 46340.  The _dev variant takes P, s1,
+ * s2 on the device; aoat, shell2ao, q and qsh stay on the host. */
+int ptbgpu_mlpop2(ptbgpu_ctx *ctx, int64_t n, int64_t nao, const double *P, const float *s1,
+                  const float *s2, const int64_t *aoat, const int64_t *shell2ao, double *q,
+                  double *qsh);
+int ptbgpu_mlpop2_dev(ptbgpu_ctx *ctx, int64_t n, int64_t nao, const double *dP,
+                      const float *ds1, const float *ds2, const int64_t *aoat,
+                      const int64_t *shell2ao, double *q, double *qsh);
+
 /* ---- density checks (source/metrics.f90) ---------------------------------
  * get_nel_packed (:202-230) = tr(P S); get_band_structure_E_packed (:253-277)
  * = tr(P H); idempotent_packed (:316-360) deviation = |sum(0.5 P S P - P)|
@@ -197,6 +242,7 @@ int ptbgpu_band_energy(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const dou
                        double *energy);
 int ptbgpu_idempotency(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const double *S,
                        double *deviation);
+
 
 /* ---- purification (source/purification.f90) -------------------------------
  * The fields of tPurificationSet (source/purification_settings.f90:29-86) that
@@ -216,6 +262,34 @@ typedef struct ptbgpu_pur_settings {
     double chempot_increment;
     int64_t nel;
 } ptbgpu_pur_settings;
+
+/* Everything a .PUR file sets (tPurificationSet, source/purification_settings.f90:
+ * 29-86), with the reference's defaults: mode 1 full / 2 submatrix (default
+ * 1), submatrix_columns 1..4 (default 2 atoms_columns), submatrix_mode 1 sign /
+ * 2 sygv (default 2), pr (default 1), the dev (default 1) and check (default 0)
+ * flags, and the purification fields. */
+typedef struct ptbgpu_pur_file {
+    int64_t mode;
+    int64_t submatrix_columns;
+    int64_t submatrix_mode;
+    int64_t prlvl;
+    int64_t dev;
+    int64_t check;
+    ptbgpu_pur_settings pur;
+} ptbgpu_pur_file;
+
+/* The defaults of ptbgpu_pur_file (host only, no context). */
+int ptbgpu_pur_defaults(ptbgpu_pur_file *f);
+/* Replaces initialize_purification (source/purification_settings.f90:91-218):
+ * parses the text of a .PUR file (len bytes, lines separated by '\n') into f,
+ * starting from the defaults, with the reference's tokenizer
+ * (source/readline.f90, source/readl2.f90:52-92) and its quirks (numeric
+ * values truncated by int(), so chempot_increment 0.1 reads as 0; unknown
+ * keys ignored; '!' comment lines).  An invalid value of a string key, where
+ * the reference ends in error stop, returns PTBGPU_E_ARG with the reference's
+ * message in errmsg (errlen bytes, may be NULL).  Host only, no context. */
+int ptbgpu_pur_parse(const char *text, int64_t len, ptbgpu_pur_file *f, char *errmsg,
+                     int64_t errlen);
 
 /* Outcome of the chemical-potential search (source/purification.f90:232-353). */
 typedef struct ptbgpu_pur_report {

@@ -7,14 +7,16 @@ Every function cites the reference lines it restates (paths under the
 reference checkout).  Dense LAPACK calls go to scipy (OpenBLAS ILP64 in this
 image): dsygvd via ``eigh(..., driver="gvd")``, dsyevd via ``driver="evd"``.
 
-Parity pins (what checks each function; see tests/test_oracle_*.py):
+Parity pins (what checks each function; tests/test_oracle.py and
+tests/test_oracle_configs.py):
   filter_matrix_sym, analyse_matrix_sym   elementwise definition + compiled
                                           reference (oracle/_ref) on fixtures
   lin / blowsym / packsym                 closed-form index map, round trip
   column_groups modes 1-3                 closed forms + compiled reference
-  column_groups mode 4 (k-means)          parity unpinned: the reference draws
-                                          from the compiler's RANDOM_NUMBER
-                                          stream, which no check here reproduces
+  column_groups mode 4 (k-means)          compiled reference: fed the compiler's
+                                          RANDOM_NUMBER stream (extracted through
+                                          the reference's own initseed), KMPP
+                                          reproduces its groups and densities
   index sets / gather / sm_stupid_simple  compiled reference (oracle/_ref) on
                                           fixtures; brute force on tiny inputs;
                                           full_matrix mode == dense projector
@@ -26,6 +28,9 @@ Parity pins (what checks each function; see tests/test_oracle_*.py):
                                           converged P == dense projector
   fermismear / occu / solve2              compiled reference FERMISMEAR/occu;
                                           zero-temperature limit
+  mlpop2                                  compiled reference mlpop14 + mlpop2 on
+                                          two pencils (float rounding bound);
+                                          sum q = tr(P S) for S^1/4 S^3/4 = S
 """
 from __future__ import annotations
 
@@ -301,18 +306,22 @@ def sm_stupid_simple(ndim: int, nelref: int, Hvec: np.ndarray, Svec: np.ndarray,
     stats["ratio"] = stats["effort_sms"] / stats["effort_full"]
 
     maps, eig, fes, Us, infos = [], [], [], [], []
-    for g in groups:
-        rows = np.nonzero(pat[:, g].any(axis=1))[0]          # :210-228
-        imap = np.full(ndim, -1, dtype=np.int64)
-        imap[rows] = np.arange(len(rows))
-        Hsm = H[np.ix_(rows, rows)]                           # :240-246
-        Ssm = S[np.ix_(rows, rows)]
-        e, U, info = sm_eigendecomp(Hsm, Ssm)
-        maps.append(rows)
-        eig.append(e)
-        Us.append(U)
-        infos.append(info)
-        fes.append(sm_fe(Ssm, U, imap[g]))
+    # small dense problems run faster on one BLAS thread than on many (the
+    # arithmetic is the same LAPACK routine either way)
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1 if max(dims, default=0) < 400 else None):
+        for g in groups:
+            rows = np.nonzero(pat[:, g].any(axis=1))[0]          # :210-228
+            imap = np.full(ndim, -1, dtype=np.int64)
+            imap[rows] = np.arange(len(rows))
+            Hsm = H[np.ix_(rows, rows)]                           # :240-246
+            Ssm = S[np.ix_(rows, rows)]
+            e, U, info = sm_eigendecomp(Hsm, Ssm)
+            maps.append(rows)
+            eig.append(e)
+            Us.append(U)
+            infos.append(info)
+            fes.append(sm_fe(Ssm, U, imap[g]))
 
     mua, mub = -10.0, 10.0                                    # :191-192
     finaliter = False
@@ -529,6 +538,25 @@ def parse_pur(text: str) -> PurSettings:
             elif arg1 == "iterative_inversion":
                 s.metric_iterative = True
     return s
+
+
+# --------------------------------------------------------------------------
+# populations (source/mlpop.f)
+# --------------------------------------------------------------------------
+def mlpop2(n: int, nao: int, P: np.ndarray, s1: np.ndarray, s2: np.ndarray, aoat, shell2ao):
+    """mlpop2 (source/mlpop.f:198-238), in REAL(4) like the reference:
+    x = blowsym84(P) (P rounded to float), cc = x s1 (s1 = S^3/4), x = s2 cc
+    (s2 = S^1/4), then q(aoat(i)) += x(i,i) and qsh(shell2ao(i), aoat(i)) +=
+    x(i,i) in AO order, in double."""
+    x = blowsym(np.asarray(P, dtype=np.float64), nao).astype(np.float32)
+    cc = x @ np.asarray(s1, dtype=np.float32)
+    d = np.einsum("ik,ki->i", np.asarray(s2, dtype=np.float32), cc).astype(np.float64)
+    q = np.zeros(n)
+    qsh = np.zeros((10, n))
+    for i in range(nao):
+        q[aoat[i] - 1] += d[i]
+        qsh[shell2ao[i] - 1, aoat[i] - 1] += d[i]
+    return q, qsh
 
 
 # --------------------------------------------------------------------------

@@ -158,3 +158,24 @@ def sygvd(A, B):
     info = I64()
     lib().ref_sygvd(I64(n), _p(a), _p(b), _p(w), ctypes.byref(info))
     return w, a, info.value
+
+
+def mlpop14(nao, S):
+    """(s34, s14) float32 (nao x nao, column-major as Fortran arrays) from
+    packed S (source/mlpop.f:142-190)."""
+    s34 = np.zeros((nao, nao), dtype=np.float32, order="F")
+    s14 = np.zeros((nao, nao), dtype=np.float32, order="F")
+    lib().ref_mlpop14(I64(nao), _p(np.ascontiguousarray(S, dtype=np.float64)), _p(s34), _p(s14))
+    return s34, s14
+
+
+def mlpop2(n, nao, P, s1, s2, aoat, shell2ao):
+    """(q(n), qsh(10, n)) of source/mlpop.f:198-238."""
+    q = np.zeros(n)
+    qsh = np.zeros((10, n), order="F")
+    lib().ref_mlpop2(I64(n), I64(nao), _p(np.ascontiguousarray(P, dtype=np.float64)),
+                     _p(np.asfortranarray(s1, dtype=np.float32)),
+                     _p(np.asfortranarray(s2, dtype=np.float32)),
+                     _p(np.ascontiguousarray(aoat, dtype=np.int64)),
+                     _p(np.ascontiguousarray(shell2ao, dtype=np.int64)), _p(q), _p(qsh))
+    return q, qsh

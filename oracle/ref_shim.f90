@@ -147,4 +147,31 @@ contains
       call la_sygvd(A, B, w, info)
    end subroutine
 
+   ! mlpop14 (source/mlpop.f:142-190): S^3/4 and S^1/4 in REAL(4) from packed S
+   subroutine ref_mlpop14(nao, S, s34, s14) bind(C, name="ref_mlpop14")
+      integer(c_int64_t), value :: nao
+      real(c_double), intent(in) :: S(nao*(nao+1)/2)
+      real(c_float), intent(out) :: s34(nao, nao), s14(nao, nao)
+      call mlpop14(nao, S, s34, s14)
+   end subroutine
+
+   ! mlpop2 (source/mlpop.f:198-238): atomic and shell populations from the
+   ! packed P and the REAL(4) S^3/4 (s1), S^1/4 (s2); aoat and shell2ao are the
+   ! module arrays of bascom the routine reads
+   subroutine ref_mlpop2(n, nao, P, s1, s2, aoat_in, shell2ao_in, q, qsh) &
+         bind(C, name="ref_mlpop2")
+      use bascom, only: aoat, shell2ao
+      integer(c_int64_t), value :: n, nao
+      real(c_double), intent(in) :: P(nao*(nao+1)/2)
+      real(c_float), intent(in) :: s1(nao, nao), s2(nao, nao)
+      integer(c_int64_t), intent(in) :: aoat_in(nao), shell2ao_in(nao)
+      real(c_double), intent(out) :: q(n), qsh(10, n)
+      if (allocated(aoat)) deallocate (aoat)
+      if (allocated(shell2ao)) deallocate (shell2ao)
+      allocate (aoat(nao), shell2ao(nao))
+      aoat = aoat_in
+      shell2ao = shell2ao_in
+      call mlpop2(n, nao, P, s1, s2, q, qsh)
+   end subroutine
+
 end module ptbref_shim

@@ -33,6 +33,10 @@ struct ptbgpu_ctx {
     std::map<std::string, DevBuf> host_bufs; // pinned host staging
     std::map<std::string, std::vector<char>> shadow;  // last upload_table contents per buffer
     int num_cus = 0;
+    // the last completed density build (what ptbgpu_sm_spectra may read back):
+    // submatrices and their summed dimensions; 0 while none or during a build
+    int64_t last_nsm = 0;
+    int64_t last_sum_m = 0;
 };
 
 namespace ptb {

@@ -633,7 +633,7 @@ int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int 
 
 int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, int count,
                    int max_n, const double *e, double *D, double *Z, double *W, double *X,
-                   double *rot, GemmDesc *dgemm, const TileList &tiles) {
+                   double *rot, GemmDesc *dgemm, const TileList &tiles, int lowp) {
     if (count <= 0) return PTBGPU_OK;
     int npow = 1;
     while (npow < max_n) npow <<= 1;
@@ -646,7 +646,7 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
                            npow, e, D, Z, W, X, rot, dgemm);
         PTB_CHECK_LAUNCH(ctx);
     }
-    return tiles.launch(ctx, 0, false, false, dgemm, "dc");
+    return tiles.launch(ctx, 0, false, false, dgemm, "dc", false, lowp);
 }
 
 }  // namespace ptb

@@ -854,10 +854,14 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
 // one: every product of the launch is a single k slice without triangular
 // factors (the A B^T form then runs k_gemm_nt1)
 int gemm_tiles(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, const GemmTile *dwork,
-               int64_t nwork, const char *site, bool f32, bool one) {
+               int64_t nwork, const char *site, bool f32, bool one, int lowp) {
     ProfScope prof_(ctx, "gemm");
     ProfScope sprof_(ctx, site ? (std::string("gemm_") + site).c_str() : "gemm_other");
     if (nwork <= 0) return PTBGPU_OK;
+    if (lowp != kLowpNone) {
+        ProfScope lprof_(ctx, lowp == kLowpBF16 ? "gemm_bf16" : "gemm_bf16x3");
+        return gemm_tiles_bf(ctx, ta, tb, ddesc, dwork, nwork, lowp, f32);
+    }
     if (nwork > 0x7fffffff) return fail(ctx, PTBGPU_E_ARG, "GEMM tile list too long (%lld)", (long long)nwork);
     const dim3 grid((unsigned)nwork);
     if (one && !ta && tb) {
@@ -982,11 +986,11 @@ int TileList::upload(ptbgpu_ctx *ctx, const char *name) {
 }
 
 int TileList::launch(ptbgpu_ctx *ctx, int i, bool ta, bool tb, const GemmDesc *ddesc,
-                     const char *site, bool f32) const {
+                     const char *site, bool f32, int lowp) const {
     const int64_t n = start_[i + 1] - start_[i];
     if (n == 0) return PTBGPU_OK;
     if (!dwork_) return fail(ctx, PTBGPU_E_ARG, "GEMM tile list launched before upload");
-    return gemm_tiles(ctx, ta, tb, ddesc, dwork_ + start_[i], n, site, f32, one_[i] != 0);
+    return gemm_tiles(ctx, ta, tb, ddesc, dwork_ + start_[i], n, site, f32, one_[i] != 0, lowp);
 }
 
 // X = L^-1 B (trans=false) or L^-T B (trans=true), all m columns, blocked by

@@ -19,6 +19,7 @@ namespace {
 
 constexpr int kThreads = 256;
 
+
 // packed index k -> column j (largest j with j(j+1)/2 <= k), exact for k < 2^52
 __device__ __forceinline__ int64_t col_of(int64_t k) {
     double d = sqrt(8.0 * (double)k + 1.0);
@@ -123,6 +124,11 @@ __global__ void k_ordered_sum(const double *__restrict__ partial, int n, double 
     }
 }
 
+// deviation = |0.5 u^T S u - 1^T u| from the two ordered sums
+__global__ void k_idem_finish(const double *__restrict__ fin, double *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = fabs(0.5 * fin[0] - fin[1]);
+}
+
 }  // namespace
 
 namespace ptb {
@@ -145,14 +151,24 @@ int weighted_dot(ptbgpu_ctx *ctx, const double *dA, const double *dB, int64_t nd
     return PTBGPU_OK;
 }
 
-int idempotency(ptbgpu_ctx *ctx, const double *dP, const double *dS, int64_t ndim,
-                double *result) {
-    const int64_t chunk = 512;
+// The idempotency check's scratch, in doubles: row-sum partials of 512-column
+// chunks, u = P 1, the quadratic form's block partials, two ordered sums.
+static constexpr int64_t kIdemChunk = 512;
+static int idem_qblocks(const ptbgpu_ctx *ctx, int64_t ndim) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ndim, (int64_t)ctx->num_cus * 4));
+}
+static size_t idem_scratch_doubles(const ptbgpu_ctx *ctx, int64_t ndim) {
+    const int64_t nchunk = (ndim + kIdemChunk - 1) / kIdemChunk;
+    return (size_t)(nchunk * ndim + ndim + 2 * idem_qblocks(ctx, ndim) + 2);
+}
+
+// Enqueues the check on the context's stream: buf holds
+// idem_scratch_doubles(ndim) doubles, the deviation goes to *dout (device).
+static int idempotency_enqueue(ptbgpu_ctx *ctx, const double *dP, const double *dS, int64_t ndim,
+                               double *buf, double *dout) {
+    const int64_t chunk = kIdemChunk;
     const int64_t nchunk = (ndim + chunk - 1) / chunk;
-    const int qblocks = (int)std::max<int64_t>(1, std::min<int64_t>(ndim, (int64_t)ctx->num_cus * 4));
-    size_t need = sizeof(double) * (nchunk * ndim + ndim + 2 * qblocks + 2);
-    auto *buf = static_cast<double *>(ws(ctx, "metric_idem", need));
-    if (!buf) return PTBGPU_E_NOMEM;
+    const int qblocks = idem_qblocks(ctx, ndim);
     double *urow = buf, *u = urow + nchunk * ndim, *part = u + ndim, *part2 = part + qblocks;
     double *fin = part2 + qblocks;
     hipLaunchKernelGGL(k_rowsum_upper, dim3((ndim + kThreads - 1) / kThreads, nchunk),
@@ -167,12 +183,23 @@ int idempotency(ptbgpu_ctx *ctx, const double *dP, const double *dS, int64_t ndi
     hipLaunchKernelGGL(k_ordered_sum, dim3(1), dim3(1), 0, ctx->stream, part, qblocks, fin);
     hipLaunchKernelGGL(k_ordered_sum, dim3(1), dim3(1), 0, ctx->stream, part2, qblocks, fin + 1);
     PTB_CHECK_LAUNCH(ctx);
-    double h[2];
-    PTB_HIP(ctx, hipMemcpyAsync(h, fin, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    *result = fabs(0.5 * h[0] - h[1]);
+    hipLaunchKernelGGL(k_idem_finish, dim3(1), dim3(1), 0, ctx->stream, fin, dout);
+    PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
+
+int idempotency(ptbgpu_ctx *ctx, const double *dP, const double *dS, int64_t ndim,
+                double *result) {
+    const size_t need = idem_scratch_doubles(ctx, ndim) + 1;
+    auto *buf = static_cast<double *>(ws(ctx, "metric_idem", sizeof(double) * need));
+    if (!buf) return PTBGPU_E_NOMEM;
+    PTB_TRY(idempotency_enqueue(ctx, dP, dS, ndim, buf, buf + need - 1));
+    PTB_HIP(ctx, hipMemcpyAsync(result, buf + need - 1, sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PTBGPU_OK;
+}
+
 
 // host arrays -> two named device buffers
 static int upload2(ptbgpu_ctx *ctx, int64_t ndim, const double *A, const double *B, double **dA,
@@ -219,5 +246,6 @@ int ptbgpu_idempotency(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const dou
     PTB_TRY(ptb::upload2(ctx, ndim, P, S, &dP, &dS));
     return ptb::idempotency(ctx, dP, dS, ndim, deviation);
 }
+
 
 }  // extern "C"

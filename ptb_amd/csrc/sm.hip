@@ -283,7 +283,7 @@ int kmeans_groups(ptbgpu_ctx *ctx, int64_t ndim, int64_t n, const double *xyz,
 int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                     const QFactor &qf, const double *K, double *Z, double *W, double *X, double *R,
                     double *eig, const double *bis, double *wocc, int *nocc, int *oidx,
-                    double *thn) {
+                    double *thn, int lowp) {
     hipStream_t st = ctx->stream;
     const int nsm = (int)desc.size();
     int max_m = 0;
@@ -303,36 +303,40 @@ int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
     ptb::ProfScope prof_(ctx, "refine_all");
     PTB_TRY(ptb::apply_q(ctx, desc, qf, X, off, hn, false, "r1"));  // X = Q Zo
     std::vector<GemmDesc> g(nsm);
-    auto run = [&](const char *name, bool ta, auto make) -> int {
+    // prec: kLowpNone for the products whose rounding the result keeps (the
+    // residual, the Gram matrix of the Loewdin step, the final rotation), the
+    // low-precision arithmetic for the correction terms (their rounding
+    // scales with the correction itself)
+    auto run = [&](const char *name, bool ta, int prec, auto make) -> int {
         for (int s = 0; s < nsm; ++s) g[s] = make(desc[s], hn[s]);
         GemmDesc *dg;
         PTB_TRY(upload(ctx, name, g, &dg));
         ptb::TileList tl;
         tl.add(g.data(), nsm);
         PTB_TRY(tl.upload(ctx, (std::string(name) + "_w").c_str()));
-        return tl.launch(ctx, 0, ta, false, dg, "refine");
+        return tl.launch(ctx, 0, ta, false, dg, "refine", false, prec);
     };
     // R = K (Q Zo)
-    PTB_TRY(run("rf_g1", false, [&](const SubDesc &d, int n) {
+    PTB_TRY(run("rf_g1", false, kLowpNone, [&](const SubDesc &d, int n) {
         return GemmDesc{K + d.off, X + d.off, R + d.off, d.m, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
     PTB_TRY(ptb::apply_q(ctx, desc, qf, R, off, hn, true, "r2"));  // R = Q^T K Q Zo
     PTB_TRY(ptb::refine_resid(ctx, ddesc, nsm, max_n, nocc, oidx, eig, W, R));
     // X = Z^T R, then E in place and the Rayleigh quotients
-    PTB_TRY(run("rf_g2", true, [&](const SubDesc &d, int n) {
+    PTB_TRY(run("rf_g2", true, lowp, [&](const SubDesc &d, int n) {
         return GemmDesc{Z + d.off, R + d.off, X + d.off, d.m, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
     PTB_TRY(ptb::refine_corr(ctx, ddesc, nsm, max_n, nocc, oidx, eig, bis, X, thn));
     // W = Zo + Z E
-    PTB_TRY(run("rf_g3", false, [&](const SubDesc &d, int n) {
+    PTB_TRY(run("rf_g3", false, lowp, [&](const SubDesc &d, int n) {
         return GemmDesc{Z + d.off, X + d.off, W + d.off, d.m, n, d.m, d.ld, d.ld, d.ld, 1.0, 1.0, 0, 0};
     }));
     // R = (3 I - W^T W) / 2, X = W R
-    PTB_TRY(run("rf_g4", true, [&](const SubDesc &d, int n) {
+    PTB_TRY(run("rf_g4", true, kLowpNone, [&](const SubDesc &d, int n) {
         return GemmDesc{W + d.off, W + d.off, R + d.off, n, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
     PTB_TRY(ptb::refine_shift(ctx, ddesc, nsm, max_n, nocc, R));
-    PTB_TRY(run("rf_g5", false, [&](const SubDesc &d, int n) {
+    PTB_TRY(run("rf_g5", false, kLowpNone, [&](const SubDesc &d, int n) {
         return GemmDesc{W + d.off, R + d.off, X + d.off, d.m, n, n, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
     return ptb::refine_scatter(ctx, ddesc, nsm, max_n, nocc, oidx, X, thn, Z, eig);
@@ -361,10 +365,12 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     Timer tm;
     tm.init(st);
     tm.mark(0, st);
+    ctx->last_nsm = ctx->last_sum_m = 0;
     if (mode != 2)
         return ptb::fail(ctx, PTBGPU_E_MODE, "submatrix_mode %lld not implemented "
                          "(the reference stops, source/sm.f90:249-251,312-314)", (long long)mode);
-    if (opts.precision != PTBGPU_FP64 && opts.precision != PTBGPU_MIXED)
+    if (opts.precision != PTBGPU_FP64 && opts.precision != PTBGPU_MIXED &&
+        opts.precision != PTBGPU_MIXED_BF16)
         return ptb::fail(ctx, PTBGPU_E_ARG, "unknown precision %lld", (long long)opts.precision);
     const int nmol = (int)mols.size();
     if (nmol < 1) return ptb::fail(ctx, PTBGPU_E_ARG, "empty batch");
@@ -509,7 +515,11 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     // multiplies by K, mirrors it into the upper one).  A(i, j), i >= j, needs
     // Z(i, k) for k <= j only, so Z is formed on the tiles on and below the
     // diagonal (diagonal tiles whole: the second product reads them whole)
-    const bool mixed = opts.precision == PTBGPU_MIXED;
+    const bool mixed = opts.precision != PTBGPU_FP64;
+    // the BF16 matrix cores for the products whose rounding the FP64
+    // refinement removes: split-BF16x3 by default, plain BF16 on request
+    const int lowp = !mixed ? kLowpNone
+                            : (opts.precision == PTBGPU_MIXED_BF16 ? kLowpBF16 : kLowpBF16x3);
     PTB_TRY(ptb::inv_lower_batch(ctx, desc, ddesc, B, Li, X));
     {
         std::vector<GemmDesc> g1(nsm), g2(nsm);
@@ -546,7 +556,8 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         if (!K32 || !V || !R || !thn || !oidx) return PTBGPU_E_NOMEM;
     }
     QFactor qf;
-    PTB_TRY(ptb::eig_tri_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot, qf, K32, V));
+    PTB_TRY(ptb::eig_tri_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot, qf, K32, V,
+                               lowp));
     tm.mark(4, st);
     // ---- fe from the owned rows: F = Z^T Q^T [L(O,:)^T, L^-1 e_O]
     std::vector<int> hflags(nsm);
@@ -603,7 +614,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         // bisection again with the refined eigenpairs
         for (int sweep = 0; sweep < sweeps; ++sweep)
             PTB_TRY(refine_occupied(ctx, desc, ddesc, qf, A, Z, W, X, R, dvec, bis, wocc, nocc,
-                                    oidx, thn));
+                                    oidx, thn, lowp));
         PTB_TRY(tf.launch(ctx, 0, true, false, dgf, "fe"));
         PTB_TRY(ptb::fe_proj(ctx, ddesc, nsm, max_m, dxoff, F, fe));
         PTB_TRY(bisect_all());
@@ -644,6 +655,8 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     PTB_HIP(ctx, hipStreamSynchronize(st));
     tm.mark(8, st);
     PTB_HIP(ctx, hipStreamSynchronize(st));
+    ctx->last_nsm = nsm;
+    ctx->last_sum_m = rows;
     const int slots[] = {PTBGPU_T_COMBINATION, PTBGPU_T_STATS, PTBGPU_T_CONSTRUCTION, PTBGPU_T_EIGEN,
                          PTBGPU_T_FE, PTBGPU_T_MUITER, PTBGPU_T_FINAL};
     for (int i = 0; i < nmol; ++i) {
@@ -698,7 +711,7 @@ namespace ptb {
 // vectors and qf the blocked form of Q (apply_q multiplies by Q or Q^T).
 int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
-                  double *tau, double *rot, QFactor &qf, float *K32, double *V) {
+                  double *tau, double *rot, QFactor &qf, float *K32, double *V, int lowp) {
     hipStream_t st = ctx->stream;
     const int nsm = (int)desc.size();
     int max_m = 0;
@@ -754,7 +767,7 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
         tl.add(shape.data(), (int)shape.size());
         PTB_TRY(tl.upload(ctx, ("sm_lwork" + std::to_string(lv)).c_str()));
         PTB_TRY(dc_merge_level(ctx, ddesc, dnodes, (int)nodes.size(), max_n, evec, dvec, Z,
-                                    W, X, rot, dg, tl));
+                                    W, X, rot, dg, tl, lowp));
     }
     return PTBGPU_OK;
 }
@@ -885,8 +898,15 @@ int ptbgpu_sm_spectra(ptbgpu_ctx *ctx, int64_t nsm, int64_t sum_m, int64_t *m, d
     if (!ctx || nsm < 1 || sum_m < 1) return PTBGPU_E_ARG;
     auto it = ctx->bufs.find("sm_desc");
     auto iv = ctx->bufs.find("sm_vec");
-    if (it == ctx->bufs.end() || iv == ctx->bufs.end())
-        return ptb::fail(ctx, PTBGPU_E_ARG, "no density build on this context yet");
+    if (it == ctx->bufs.end() || iv == ctx->bufs.end() || ctx->last_nsm == 0)
+        return ptb::fail(ctx, PTBGPU_E_ARG, "no completed density build on this context");
+    // the descriptor table holds last_nsm entries: a different count would read
+    // past it (or stop short)
+    if (nsm != ctx->last_nsm || sum_m != ctx->last_sum_m)
+        return ptb::fail(ctx, PTBGPU_E_ARG,
+                         "nsm %lld / sum_m %lld do not match the last build (%lld / %lld)",
+                         (long long)nsm, (long long)sum_m, (long long)ctx->last_nsm,
+                         (long long)ctx->last_sum_m);
     PTB_HIP(ctx, hipSetDevice(ctx->device));
     std::vector<SubDesc> desc(nsm);
     PTB_HIP(ctx, hipMemcpyAsync(desc.data(), it->second.ptr, sizeof(SubDesc) * nsm,

@@ -101,6 +101,13 @@ struct RhsLayout {
     std::vector<int> cols;
 };
 
+// Arithmetic of a batched product: FP64 MFMA (the plain path), or the BF16
+// matrix cores with FP32 accumulation on split-BF16x3 or plain BF16 operands
+// (lowp.hip; the mixed-precision build).
+constexpr int kLowpNone = 0;
+constexpr int kLowpBF16x3 = 1;
+constexpr int kLowpBF16 = 2;
+
 namespace ptb {
 
 // pattern.hip: one launch per phase for every molecule of a batch.  mols
@@ -117,6 +124,11 @@ int group_map(ptbgpu_ctx *ctx, const uint32_t *uni, const MolDev *mols, const in
 int gather(ptbgpu_ctx *ctx, const MolDev *mols, const SubDesc *ddesc, int nsm, int max_m,
            const int *dmap, double *Hd, double *Sd);
 
+// lowp.hip: the BF16 matrix-core form of gemm_tiles (same descriptors and tile
+// lists); f32: operands stored as float, else double
+int gemm_tiles_bf(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, const GemmTile *dwork,
+                  int64_t nwork, int lowp, bool f32);
+
 // dense.hip
 // f32: A, B, C are float arrays behind the descriptor's pointers (A B^T only).
 // gemm_vbatch launches count x tiles(max_m, max_n) workgroups (shapes may be
@@ -125,7 +137,8 @@ int gather(ptbgpu_ctx *ctx, const MolDev *mols, const SubDesc *ddesc, int nsm, i
 int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int count, int max_m,
                 int max_n, const char *site = nullptr, bool f32 = false);
 int gemm_tiles(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, const GemmTile *dwork,
-               int64_t nwork, const char *site = nullptr, bool f32 = false, bool one = false);
+               int64_t nwork, const char *site = nullptr, bool f32 = false, bool one = false,
+               int lowp = kLowpNone);
 
 // Tile lists of batched GEMMs whose shapes the host knows.  Each launch gets
 // exactly the tiles that hold output (lower-triangle tiles for uplo), dealt to
@@ -141,7 +154,7 @@ class TileList {
     // builds (if the shapes changed) and uploads the lists of every launch
     int upload(ptbgpu_ctx *ctx, const char *name);
     int launch(ptbgpu_ctx *ctx, int i, bool ta, bool tb, const GemmDesc *ddesc,
-               const char *site = nullptr, bool f32 = false) const;
+               const char *site = nullptr, bool f32 = false, int lowp = kLowpNone) const;
 
   private:
     void build(std::vector<GemmTile> &work);
@@ -188,10 +201,11 @@ int sytrd_batch_f32(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
 // sm.hip
 // With K32 and V given, the tridiagonalisation runs on float storage (K32) and
 // the FP64 reflectors go to V; A (the FP64 matrix) is left intact.
+// lowp: arithmetic of the divide-and-conquer merge products (kLowp*)
 int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
                   double *tau, double *rot, QFactor &qf, float *K32 = nullptr,
-                  double *V = nullptr);
+                  double *V = nullptr, int lowp = 0);
 int eig_sym_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
                   double *tau, double *rot);
@@ -208,9 +222,10 @@ int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int 
 // below the split: n1 x n and (n - n1) x n bounds, n = hi - lo; the merge
 // kernel writes N = the node's non-deflated count <= n); dgemm holds
 // 2 x count descriptors
+// lowp: arithmetic of the merged-vector products (kLowp*)
 int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, int count,
                    int max_n, const double *e, double *D, double *Z, double *W, double *X,
-                   double *rot, GemmDesc *dgemm, const TileList &tiles);
+                   double *rot, GemmDesc *dgemm, const TileList &tiles, int lowp = 0);
 
 // density.hip
 // XA = [L(O,:)^T, Li(:,O)] per submatrix (Li = L^-1)

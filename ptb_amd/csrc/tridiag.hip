@@ -1,10 +1,10 @@
 // Householder reduction of each K = L^-1 H L^-T to tridiagonal form, K = Q T Q^T,
 // Q = H_0 ... H_{m-3},  H_k = I - tau_k v_k v_k^T  (LAPACK dsytrd, lower).
 //
-// One 256-thread workgroup per submatrix, blocked by 32 columns (dlatrd scheme):
-// inside a panel the trailing matrix stays as it was at the panel start and the
-// pending rank-2 terms are applied on the fly as V (W^T v) + W (V^T v); at the
-// end of the panel one rank-64 update refreshes the lower triangle of the
+// One 256-thread workgroup per submatrix, blocked by kNB = 16 columns (dlatrd
+// scheme): inside a panel the trailing matrix stays as it was at the panel start
+// and the pending rank-2 terms are applied on the fly as V (W^T v) + W (V^T v);
+// at the end of the panel one rank-32 update refreshes the lower triangle of the
 // trailing matrix (the upper triangle is never read again).  The per-column
 // product p = A22 v is a symmetric GEMV over the lower triangle: lanes own
 // rows, so a wavefront reads 64 consecutive elements of one column, and the
@@ -14,7 +14,7 @@
 //
 // Output: d (diagonal), e (subdiagonal, e[k] couples k and k+1), tau, and v_k
 // stored in column k rows k+1.. of A (unit at row k+1).  W is a work panel
-// (m x 32, leading dimension ld) in a separate panel buffer.
+// (m x kNB, leading dimension ld) in a separate panel buffer.
 //
 // The matrix and the panel buffer are stored as T: double for the plain
 // solve, float for the mixed-precision one (half the bytes of the GEMV).
@@ -418,9 +418,9 @@ __global__ void k_sytrd_finish(const SubDesc *__restrict__ desc, const T *__rest
 
 namespace ptb {
 
-// Panel by panel: the panel kernel reduces 32 columns of every submatrix, then
-// one batched GEMM applies the rank-64 update A(s:,s:) -= [W V] [V W]^T to the
-// lower triangle.
+// Panel by panel: the panel kernel reduces kNB = 16 columns of every submatrix,
+// then one batched GEMM applies the rank-32 update A(s:,s:) -= [W V] [V W]^T to
+// the lower triangle.
 namespace {
 template <typename T>
 int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc, T *A,

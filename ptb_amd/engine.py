@@ -29,7 +29,8 @@ I64 = ctypes.c_int64
 F64 = ctypes.c_double
 VP = ctypes.c_void_p
 
-PRECISION = {"fp64": 0, "mixed": 1}
+PRECISION = {"fp64": 0, "mixed": 1, "bf16x3": 1, "mixed_bf16": 2, "bf16": 2}
+GEMM_PRECISION = {"fp64": 0, "bf16x3": 1, "bf16": 2}
 COLUMNS = {"single_columns": 1, "atoms_columns": 2, "full_matrix": 3, "kmeans_r_heuristic": 4}
 
 PHASES = ["alloc", "submatrix combination", "submatrix stats", "submatrix construction",
@@ -130,6 +131,12 @@ def _lib():
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [VP, ctypes.POINTER(PurSettings), I64, VP, VP, VP,
                                            ctypes.POINTER(PurReport)]
+    for name in ("ptbgpu_mlpop2", "ptbgpu_mlpop2_dev"):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = [VP, I64, I64, VP, VP, VP, VP, VP, VP, VP]
+    if hasattr(lib, "ptbgpu_gemm_batch_dev"):
+        lib.ptbgpu_gemm_batch_dev.argtypes = [VP, I64, I64, I64, I64, VP, VP, VP, VP, VP, VP, VP,
+                                              VP, VP, F64, F64]
     lib._ptb_typed = True
     return lib
 
@@ -148,6 +155,31 @@ def _f64(a, n, name):
     if a.size < n:
         raise ValueError(f"{name} has {a.size} elements, needs {n}")
     return a
+
+
+def _dev_f64(t, n, name, device):
+    """A device operand of a ``*_dev`` call: a contiguous float64 torch tensor on
+    the engine's device with at least ``n`` elements.  The kernels index it up
+    to ``n``; anything else would make them read or write outside it."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor on cuda:{device}")
+    if not t.is_cuda or t.device.index != device:
+        raise ValueError(f"{name} is on {t.device}, the engine runs on cuda:{device}")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name} must be float64, not {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.numel() < n:
+        raise ValueError(f"{name} has {t.numel()} elements, needs {n}")
+    return t
+
+
+def _producers_done(device):
+    """The engine's stream does not wait on other streams: work queued on
+    torch's current stream (the tensors' producers) must be complete first."""
+    import torch
+    torch.cuda.current_stream(device).synchronize()
 
 
 def fermismear(norbs, nel, t, eig):
@@ -228,6 +260,8 @@ class Engine:
         return int(c[0]), int(c[1])
 
     def filter_matrix_sym_dev(self, A, n: int, filter_: float):
+        _dev_f64(A, n * (n + 1) // 2, "A", self.device)
+        _producers_done(self.device)
         c = np.zeros(2, dtype=np.int64)
         self._check(self.lib.ptbgpu_filter_sym_dev(self.ctx, n, _ptr(A), filter_, _ptr(c)))
         return int(c[0]), int(c[1])
@@ -238,6 +272,8 @@ class Engine:
             _f64(A, n * (n + 1) // 2, "A")
             self._check(self.lib.ptbgpu_analyse_sym(self.ctx, n, _ptr(A), _ptr(c)))
         else:
+            _dev_f64(A, n * (n + 1) // 2, "A", self.device)
+            _producers_done(self.device)
             self._check(self.lib.ptbgpu_analyse_sym_dev(self.ctx, n, _ptr(A), _ptr(c)))
         tot = int(c[0])
         return {"dimension": n, "nonzero": tot, "sparsity": tot / float(n) ** 2 if n else 0.0,
@@ -277,8 +313,14 @@ class Engine:
                              strict=False, verbose=False, refine_steps=0,
                              kmeans_stream=None) -> dict:
         """Same with device-resident packed H, S, P (torch CUDA tensors)."""
+        npk = ndim * (ndim + 1) // 2
+        for t, name in ((dH, "dH"), (dS, "dS"), (dP, "dP")):
+            _dev_f64(t, npk, name, self.device)
         x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(n, 3)
         a = np.ascontiguousarray(aoat, dtype=np.int64)
+        if a.size != ndim:
+            raise ValueError("aoat must have ndim entries")
+        _producers_done(self.device)
         rep = Report()
         opts = self._opts(precision, strict, verbose, refine_steps, kmeans_stream)
         self._check(self.lib.ptbgpu_sm_density_dev(
@@ -301,10 +343,11 @@ class Engine:
         ptrs = {name: (VP * k)() for name in ("H", "S", "P", "xyz", "aoat")}
         for i, (nd, _, H, S, P, n, xyz, aoat) in enumerate(mols):
             npk = nd * (nd + 1) // 2
-            if not device:
-                _f64(H, npk, "H")
-                _f64(S, npk, "S")
-                _f64(P, npk, "P")
+            check = _f64 if not device else \
+                (lambda t, k, name: _dev_f64(t, k, f"{name}[{i}]", self.device))
+            check(H, npk, "H")
+            check(S, npk, "S")
+            check(P, npk, "P")
             x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(n, 3)
             a = np.ascontiguousarray(aoat, dtype=np.int64)
             if a.size != nd:
@@ -315,6 +358,8 @@ class Engine:
             ptrs["P"][i] = _ptr(P).value
             ptrs["xyz"][i] = x.ctypes.data
             ptrs["aoat"][i] = a.ctypes.data
+        if device:
+            _producers_done(self.device)
         reps = (Report * k)()
         opts = self._opts(precision, strict, verbose, refine_steps, kmeans_stream)
         fn = self.lib.ptbgpu_sm_density_batch_dev if device else self.lib.ptbgpu_sm_density_batch
@@ -349,6 +394,10 @@ class Engine:
 
     def purification_wrapper_dev(self, pur: PurSettings, ndim: int, dH, dS, dP) -> dict:
         """Same with device-resident packed H, S, P (torch CUDA tensors)."""
+        npk = ndim * (ndim + 1) // 2
+        for t, name in ((dH, "dH"), (dS, "dS"), (dP, "dP")):
+            _dev_f64(t, npk, name, self.device)
+        _producers_done(self.device)
         rep = PurReport()
         self._check(self.lib.ptbgpu_purify_dev(self.ctx, ctypes.byref(pur), ndim, _ptr(dH),
                                                _ptr(dS), _ptr(dP), ctypes.byref(rep)))
@@ -387,6 +436,70 @@ class Engine:
                                            shift[1], _ptr(H), _ptr(S), _ptr(P), _ptr(e),
                                            _ptr(focc), _ptr(U), ctypes.byref(fail)))
         return P, e, focc, np.ascontiguousarray(U.T), bool(fail.value)
+
+    # ---- populations ---------------------------------------------------------
+    def mlpop2(self, n, nao, P, s1, s2, aoat, shell2ao):
+        """mlpop2 (source/mlpop.f:198-238): (q(n), qsh(10, n)) from packed P
+        and the float32 S^3/4 (s1), S^1/4 (s2), nao x nao.  numpy arrays (host)
+        or torch CUDA tensors for P, s1, s2 (device); s1/s2 are read
+        column-major (pass a symmetric matrix or its transpose)."""
+        a = np.ascontiguousarray(aoat, dtype=np.int64)
+        sh = np.ascontiguousarray(shell2ao, dtype=np.int64)
+        if a.size != nao or sh.size != nao:
+            raise ValueError("aoat and shell2ao must have nao entries")
+        q = np.zeros(n)
+        qsh = np.zeros((n, 10))      # qsh(10, n) column-major
+        if isinstance(P, np.ndarray):
+            _f64(P, nao * (nao + 1) // 2, "P")
+            s1 = np.asfortranarray(s1, dtype=np.float32)
+            s2 = np.asfortranarray(s2, dtype=np.float32)
+            fn = self.lib.ptbgpu_mlpop2
+        else:
+            import torch
+            _dev_f64(P, nao * (nao + 1) // 2, "P", self.device)
+            for t, name in ((s1, "s1"), (s2, "s2")):
+                if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32
+                        and t.is_contiguous() and t.numel() >= nao * nao):
+                    raise TypeError(f"{name} must be a contiguous float32 CUDA tensor of nao^2")
+            _producers_done(self.device)
+            fn = self.lib.ptbgpu_mlpop2_dev
+        self._check(fn(self.ctx, n, nao, _ptr(P), _ptr(s1), _ptr(s2), _ptr(a), _ptr(sh),
+                       _ptr(q), _ptr(qsh)))
+        return q, qsh.T.copy()
+
+    # ---- matrix-core products --------------------------------------------------
+    def gemm_batch_dev(self, precision, transa, transb, As, Bs, Cs, alpha=1.0, beta=0.0):
+        """C_i = alpha op(A_i) op(B_i) + beta C_i for lists of 2-D float64 torch
+        CUDA tensors (column-major views: pass A.t().contiguous() layouts as
+        torch row-major arrays of the transposed shape; see tests).  Each tensor
+        is read as a column-major matrix of shape (cols, rows) of its torch
+        shape, i.e. a torch tensor X of shape (c, r) holds the r x c matrix X^T."""
+        k = len(As)
+        if not (len(Bs) == len(Cs) == k) or k < 1:
+            raise ValueError("A, B, C lists must have the same nonzero length")
+        M = np.zeros(k, dtype=np.int64)
+        N = np.zeros(k, dtype=np.int64)
+        K = np.zeros(k, dtype=np.int64)
+        lda, ldb, ldc = (np.zeros(k, dtype=np.int64) for _ in range(3))
+        pa, pb, pc = (VP * k)(), (VP * k)(), (VP * k)()
+        for i, (a, b, c) in enumerate(zip(As, Bs, Cs)):
+            for t, name in ((a, "A"), (b, "B"), (c, "C")):
+                _dev_f64(t, t.numel(), f"{name}[{i}]", self.device)
+                if t.dim() != 2:
+                    raise ValueError(f"{name}[{i}] must be 2-D")
+            ar, ac = a.shape[1], a.shape[0]      # stored (column-major) rows, cols
+            br, bc = b.shape[1], b.shape[0]
+            M[i], K[i] = (ac, ar) if transa else (ar, ac)
+            kb, N[i] = (bc, br) if transb else (br, bc)
+            if kb != K[i] or c.shape[1] != M[i] or c.shape[0] != N[i]:
+                raise ValueError(f"product {i}: shapes do not conform")
+            lda[i], ldb[i], ldc[i] = ar, br, M[i]
+            pa[i], pb[i], pc[i] = a.data_ptr(), b.data_ptr(), c.data_ptr()
+        _producers_done(self.device)
+        prec = GEMM_PRECISION[precision] if isinstance(precision, str) else int(precision)
+        self._check(self.lib.ptbgpu_gemm_batch_dev(
+            self.ctx, prec, 1 if transa else 0, 1 if transb else 0, k, _ptr(M), _ptr(N), _ptr(K),
+            pa, _ptr(lda), pb, _ptr(ldb), pc, _ptr(ldc), alpha, beta))
 
     # ---- metrics -----------------------------------------------------------
     def _scalar(self, fn, ndim, A, B):

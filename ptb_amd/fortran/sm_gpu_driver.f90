@@ -3,7 +3,8 @@
 ! then evaluate N_el and Tr(HP).  Inputs come from a raw binary file written by
 ! the test-suite: ndim, nel, n, columns, mode (int64), filter (double), xyz(3,n),
 ! aoat(ndim) (int64), then packed H and S (double).  Packed P and the two
-! traces are written to the output file.
+! traces are written to the output file (stream access) and, as two Fortran
+! sequential records, to <output>.seq.
 program sm_gpu_driver
    use iso_c_binding
    use sm, only: sm_stupid_simple
@@ -42,6 +43,12 @@ program sm_gpu_driver
    print *, "Tr(H P)=", eband
 
    open (newunit=u, file=trim(fout), access="stream", form="unformatted", status="replace")
+   write (u) P
+   write (u) nelp, eband
+   close (u)
+   ! the same two records with the framing of PTB's ptb_dump (sequential
+   ! unformatted, source/pgtb.f90:275-290), for the format readers' tests
+   open (newunit=u, file=trim(fout)//".seq", form="unformatted", status="replace")
    write (u) P
    write (u) nelp, eband
    close (u)

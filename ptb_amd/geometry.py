@@ -29,6 +29,10 @@ ANGSTROM_TO_BOHR = 1.0 / 0.52917726
 
 # element -> (nuclear charge, valence electrons, AO count)
 ELEMENTS = {"H": (1, 1, 5), "C": (6, 4, 13), "O": (8, 6, 13)}
+# element -> shell (1-based, within the atom) of each of its AOs:
+# H s, s, p(3); C and O s, s, p(3), p(3), d(5)
+SHELLS = {"H": [1, 2, 3, 3, 3], "C": [1, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5, 5, 5]}
+SHELLS["O"] = SHELLS["C"]
 
 
 @dataclass
@@ -57,6 +61,12 @@ class Molecule:
         """1-based atom index of every AO (the reference's ``aoat``)."""
         counts = [ELEMENTS[s][2] for s in self.symbols]
         return np.repeat(np.arange(1, self.n + 1, dtype=np.int64), counts)
+
+    @property
+    def shell2ao(self) -> np.ndarray:
+        """Shell (1-based, within its atom) of every AO (the reference's
+        ``shell2ao``, used by the population analysis)."""
+        return np.concatenate([SHELLS[s] for s in self.symbols]).astype(np.int64)
 
     @property
     def znum(self) -> np.ndarray:

@@ -92,6 +92,11 @@ def main():
         kmeans_groups()
     if want("kmeans_sm"):
         kmeans_density()
+    for name in LARGE_CASES:
+        if want("large") or want(name):
+            large_case(name)
+    if want("mlpop"):
+        mlpop_cases()
 
 
 def metrics_and_fermi():
@@ -204,6 +209,94 @@ def kmeans_groups():
             out[f"owner{tag}_{nsm}"] = owner
             out[f"effort{tag}_{nsm}"] = eff
     np.savez_compressed(os.path.join(HERE, "kmeans.npz"), **out)
+
+
+# The benchmark-sized systems, atoms_columns, submatrix_sygv.  The packed P of
+# these is too large to commit whole, so the fixture holds the reference's
+# scalars (positive counts, a digest of the filtered matrices, the mu trace,
+# tr(HP), tr(PS)) and the packed columns of a sample of column groups spread
+# over the range of submatrix dimensions: their nonzero rows (positions) and
+# values.  Every other entry of those columns is an exact zero in the
+# reference's P (rows outside the owner's map).
+LARGE_CASES = {
+    # name: (workload, filter, sampled groups)
+    "c0_h2o64": ("h2o_64", 1e-4, 24),
+    "c1_h2o512": ("h2o_512", 1e-4, 32),
+    "c2_c400h802": ("c400h802", 1e-5, 24),
+}
+
+
+def group_dims(ndim, H, S, groups):
+    """m of every column group: rows i with H(i,j) > 0 or S(i,j) > 0 or i == j
+    for some owned j (source/sm.f90:165-188)."""
+    r, c = O._upper_index(ndim)
+    pat = np.zeros((ndim, ndim), dtype=bool)
+    pat[r, c] = (H > 0) | (S > 0)
+    pat |= pat.T
+    np.fill_diagonal(pat, True)
+    return np.array([int(pat[:, g].any(axis=1).sum()) for g in groups])
+
+
+def large_case(name):
+    import hashlib
+    import time
+    wl, filt, nsample = LARGE_CASES[name]
+    mol = geometry.config_molecule(wl)
+    H, S = surrogate.build_hs(mol)
+    n = mol.ndim
+    cnt = {}
+    cnt["S_before"] = O.analyse_matrix_sym(S, n)["nonzero"]
+    cnt["H_before"] = O.analyse_matrix_sym(H, n)["nonzero"]
+    O.filter_matrix_sym(S, n, filt)
+    O.filter_matrix_sym(H, n, filt)
+    cnt["S_after"] = O.analyse_matrix_sym(S, n)["nonzero"]
+    cnt["H_after"] = O.analyse_matrix_sym(H, n)["nonzero"]
+    groups = O.column_groups(2, n, mol.aoat)
+    mdim = group_dims(n, H, S, groups)
+    t0 = time.time()
+    log = []
+    P = R.sm_stupid_simple(n, mol.nel, H, S, mol.n, mol.xyz, mol.aoat, 2, 2, log=log)
+    t_ref = time.time() - t0
+    trace = parse_mu_trace(log[0])
+    order = np.argsort(mdim, kind="stable")
+    pick = np.unique(order[np.linspace(0, len(order) - 1, nsample).round().astype(int)])
+    cols, rows, vals, nnz = [], [], [], []
+    for g in pick:
+        for j in groups[g]:
+            col = P[j * (j + 1) // 2: j * (j + 1) // 2 + j + 1]   # P(0..j, j)
+            nz = np.flatnonzero(col)
+            cols.append(j)
+            nnz.append(len(nz))
+            rows.append(nz)
+            vals.append(col[nz])
+    np.savez_compressed(
+        os.path.join(HERE, f"sm_{name}.npz"), workload=wl, filter=filt, ndim=n, nel=mol.nel,
+        n=mol.n, cols=2, nsm=len(groups), group_m=mdim, sampled_groups=pick,
+        sampled_cols=np.array(cols), col_nnz=np.array(nnz), col_rows=np.concatenate(rows),
+        col_vals=np.concatenate(vals), mu_trace=trace,
+        counts=np.array([cnt["S_before"], cnt["S_after"], cnt["H_before"], cnt["H_after"]]),
+        sha_S=hashlib.sha256(S.tobytes()).hexdigest(), sha_H=hashlib.sha256(H.tobytes()).hexdigest(),
+        eband=R.band_structure(n, P, H), nelp=R.get_nel(n, P, S), t_ref=t_ref)
+    print(name, n, "nsm", len(groups), "m", mdim.min(), mdim.max(), "muiter", len(trace),
+          "mu", trace[-1, 1], f"ref {t_ref:.1f} s", "columns", len(cols))
+
+
+def mlpop_cases():
+    """mlpop14 + mlpop2 (source/mlpop.f:142-238) on the unfiltered overlap and
+    the reference's own atoms-mode density of two small systems."""
+    out = {}
+    for tag, fixture in (("h2o8", "sm_h2o8_atoms.npz"), ("c8h18", "sm_c8h18_atoms.npz")):
+        d = np.load(os.path.join(HERE, fixture))
+        mol = geometry.water_box((2, 2, 2), 3.1, 0.2, 0) if tag == "h2o8" else geometry.alkane(8)
+        assert mol.ndim == int(d["ndim"])
+        _, S = surrogate.build_hs(mol)
+        s34, s14 = R.mlpop14(mol.ndim, S)
+        q, qsh = R.mlpop2(mol.n, mol.ndim, d["P"], s34, s14, mol.aoat, mol.shell2ao)
+        out.update({f"{tag}_P": d["P"], f"{tag}_s34": s34, f"{tag}_s14": s14,
+                    f"{tag}_aoat": mol.aoat, f"{tag}_shell2ao": mol.shell2ao, f"{tag}_q": q,
+                    f"{tag}_qsh": qsh, f"{tag}_n": mol.n, f"{tag}_nao": mol.ndim})
+        print("mlpop", tag, mol.ndim, "sum q", q.sum())
+    np.savez_compressed(os.path.join(HERE, "mlpop.npz"), **out)
 
 
 if __name__ == "__main__":

@@ -75,3 +75,32 @@ def test_batch_device_and_mixed(engine):
     for (mol, H, S), r, x in zip(batch, ref, mixed):
         assert np.abs(r[4] - x[4]).max() <= 1e-6
         assert abs(O.band_structure(mol.ndim, r[4], H) - O.band_structure(mol.ndim, x[4], H)) <= 1e-6
+
+
+def test_batch256_seed2_equals_single_and_oracle(engine):
+    """The benchmark's own batch: 256 molecules from seed 2 (128 water clusters
+    of 4-40 H2O at filter 1e-4, 128 alkanes C3..C40 at filter 1e-5) in one
+    build.  Every molecule's density is bit-identical to its single-molecule
+    build; 16 molecules spread over the batch match the oracle to 1e-8 with
+    identical mu and pass count."""
+    from oracle import ptb_oracle as O
+    batch = _batch(256, seed=2)
+    args = _args(batch)
+    reps = engine.sm_stupid_simple_batch(args, 2, 2)
+    worst = 0.0
+    for (mol, H, S), a, rep in zip(batch, args, reps):
+        P1 = np.full_like(H, np.nan)
+        r1 = engine.sm_stupid_simple(mol.ndim, mol.nel, H, S, P1, mol.n, mol.xyz, mol.aoat, 2, 2)
+        assert rep["mu"] == r1["mu"] and rep["muiter"] == r1["muiter"]
+        worst = max(worst, float(np.abs(a[4] - P1).max()))
+    print(f"\n256 molecules ({sum(r['nsm'] for r in reps)} submatrices): "
+          f"max |P_batch - P_single| = {worst:.3e}")
+    assert worst == 0.0
+    for i in np.linspace(0, 255, 16).round().astype(int):
+        mol, H, S = batch[i]
+        res = O.sm_stupid_simple(mol.ndim, mol.nel, H, S, mol.n, mol.xyz, mol.aoat, 2, 2)
+        err = float(np.abs(args[i][4] - res.P).max())
+        print(f"{i:3d} {mol.name:10s} ndim {mol.ndim:4d} mu {reps[i]['mu']:+.6f} "
+              f"max|P - P_oracle| {err:.2e}")
+        assert reps[i]["mu"] == res.mu and reps[i]["muiter"] == res.muiter
+        assert err <= 1e-8

@@ -1,0 +1,161 @@
+"""The benchmark's own workloads on the GPU against the compiled reference.
+
+For the 64-water box (filter 1e-4), the 512-water box the throughput is quoted
+on (filter 1e-4, 1536 submatrices with m = 203..999) and C400H802 (filter
+1e-5), all atoms_columns / submatrix_sygv, the fixtures under tests/golden/
+hold what the reference's own sm_stupid_simple (source/sm.f90:34-409) and
+filter_matrix_sym / analyse_matrix_sym (source/pgtb.f90:1532-1579) give on the
+same surrogate pencils: positive counts before and after filtering, a SHA-256
+digest of the filtered packed S and H, the mu trace, tr(HP), tr(PS), and the
+packed columns of sampled column groups spread over the range of m.
+
+Bars (FP64 drop-in): filter bytes and counts bit-exact; identical mu and pass
+count; the sampled columns within 1e-8 per element with the same exact-zero
+rows; |dE_band| <= 1e-7 Eh; |dN_el| <= 1e-10.  Mixed precision against the FP64
+build: max |dP| <= 1e-6 and |dE_band| <= 1e-6 Eh.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+CONFIGS = ["c0_h2o64", "c1_h2o512", "c2_c400h802"]
+_cache = {}
+
+
+def _fixture(name):
+    path = os.path.join(GOLD, f"sm_{name}.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{path} not generated")
+    return np.load(path)
+
+
+def _system(engine, name):
+    """The workload's pencil, filtered on the GPU (S, then H, as twoscf does,
+    source/pgtb.f90:566-578), checked byte for byte against the reference's."""
+    if name in _cache:
+        return _cache[name]
+    from ptb_amd import geometry, surrogate
+    d = _fixture(name)
+    mol = geometry.config_molecule(str(d["workload"]))
+    H, S = surrogate.build_hs(mol)
+    n = mol.ndim
+    assert n == int(d["ndim"]) and mol.nel == int(d["nel"])
+    filt = float(d["filter"])
+    cs = engine.filter_matrix_sym(S, n, filt)
+    ch = engine.filter_matrix_sym(H, n, filt)
+    print(f"\n{name}: ndim {n}, positive S {cs}, H {ch}")
+    assert list(cs) + list(ch) == [int(x) for x in d["counts"][[0, 1, 2, 3]]]
+    assert hashlib.sha256(S.tobytes()).hexdigest() == str(d["sha_S"])
+    assert hashlib.sha256(H.tobytes()).hexdigest() == str(d["sha_H"])
+    _cache.clear()
+    _cache[name] = (mol, H, S)
+    return _cache[name]
+
+
+def _build(engine, mol, H, S, **kw):
+    P = np.full_like(H, np.nan)
+    rep = engine.sm_stupid_simple(mol.ndim, mol.nel, H, S, P, mol.n, mol.xyz, mol.aoat, 2, 2,
+                                  **kw)
+    return P, rep
+
+
+def _check_columns(d, P):
+    """The sampled packed columns: same nonzero rows, values within 1e-8."""
+    cols, nnz = d["sampled_cols"], d["col_nnz"]
+    rows, vals = d["col_rows"], d["col_vals"]
+    at, worst = 0, 0.0
+    for j, k in zip(cols, nnz):
+        j = int(j)
+        col = P[j * (j + 1) // 2: j * (j + 1) // 2 + j + 1]
+        nz = np.flatnonzero(col)
+        assert np.array_equal(nz, rows[at:at + k]), f"zero pattern of column {j}"
+        worst = max(worst, float(np.abs(col[nz] - vals[at:at + k]).max()))
+        at += k
+    return worst
+
+
+@pytest.mark.parametrize("name", CONFIGS)
+def test_fp64_matches_reference(engine, name):
+    d = _fixture(name)
+    mol, H, S = _system(engine, name)
+    n = mol.ndim
+    P, rep = _build(engine, mol, H, S)
+    trace = d["mu_trace"]
+    m = engine.sm_spectra(rep["nsm"], rep["sum_m"])[0]
+    print(f"nsm {rep['nsm']} m {m.min()}..{m.max()} mu {rep['mu']} ({trace[-1, 1]}) "
+          f"muiter {rep['muiter']} ({len(trace)})")
+    assert rep["nsm"] == int(d["nsm"])
+    assert np.array_equal(m, d["group_m"])
+    assert rep["muiter"] == len(trace)
+    assert rep["mu"] == trace[-1, 1]
+    assert rep["nel"] == pytest.approx(trace[-1, 2], abs=1e-9)
+    err = _check_columns(d, P)
+    eb = engine.band_structure(n, P, H)
+    ne = engine.get_nel(n, P, S)
+    print(f"{len(d['sampled_cols'])} sampled columns: max|P - P_ref| {err:.3e}; "
+          f"dE_band {eb - float(d['eband']):.3e}, dN_el {ne - float(d['nelp']):.3e}")
+    assert err <= 1e-8
+    assert abs(eb - float(d["eband"])) <= 1e-7
+    assert abs(ne - float(d["nelp"])) <= 1e-10
+    _cache[name + "_P"] = P
+
+
+@pytest.mark.parametrize("name", CONFIGS)
+def test_mixed_matches_fp64(engine, name):
+    mol, H, S = _system(engine, name)
+    n = mol.ndim
+    P64 = _cache.get(name + "_P")
+    if P64 is None:
+        P64, _ = _build(engine, mol, H, S)
+    Pm, rm = _build(engine, mol, H, S, precision="mixed")
+    dP = float(np.abs(Pm - P64).max())
+    dE = abs(engine.band_structure(n, Pm, H) - engine.band_structure(n, P64, H))
+    print(f"mixed: max|P - P_fp64| {dP:.3e}, |dE_band| {dE:.3e}, mu {rm['mu']}")
+    assert dP <= 1e-6 and dE <= 1e-6
+    assert np.array_equal(Pm == 0.0, P64 == 0.0)
+
+
+def test_h2o64_full_density_matches_oracle(engine):
+    """Every element of the 64-water density (ndim 1472, 192 submatrices up to
+    m = 811) against the oracle, not a sample."""
+    from oracle import ptb_oracle as O
+    mol, H, S = _system(engine, "c0_h2o64")
+    P, rep = _build(engine, mol, H, S)
+    res = O.sm_stupid_simple(mol.ndim, mol.nel, H, S, mol.n, mol.xyz, mol.aoat, 2, 2)
+    err = float(np.abs(P - res.P).max())
+    print(f"max|P - P_oracle| {err:.3e} over {P.size} packed entries")
+    assert rep["mu"] == res.mu and rep["muiter"] == res.muiter
+    assert err <= 1e-8
+    assert np.array_equal(P == 0.0, res.P == 0.0)
+
+
+@pytest.mark.parametrize("name", ["c2_c400h802", "c1_h2o512"])
+def test_precision_ladder(engine, name):
+    """The mixed build's accuracy per matrix-core precision, against the FP64
+    build: split-BF16x3 products (the default mixed build) with one FP64
+    refinement sweep meet max |dP| <= 1e-6 and |dE_band| <= 1e-6 Eh; plain BF16
+    products (8-bit operands) are reported with one and with three sweeps and
+    must reach the same bar with three."""
+    mol, H, S = _system(engine, name)
+    n = mol.ndim
+    P64 = _cache.get(name + "_P")
+    if P64 is None:
+        P64, _ = _build(engine, mol, H, S)
+    e64 = engine.band_structure(n, P64, H)
+    rows = {}
+    for label, prec, sweeps in (("bf16x3", "mixed", 1), ("bf16", "mixed_bf16", 1),
+                                ("bf16+3", "mixed_bf16", 3), ("none", "mixed", -1)):
+        P, rep = _build(engine, mol, H, S, precision=prec, refine_steps=sweeps)
+        rows[label] = (float(np.abs(P - P64).max()), abs(engine.band_structure(n, P, H) - e64))
+        print(f"{name} {label:7s} sweeps {sweeps:2d}: max|dP| {rows[label][0]:.3e} "
+              f"|dE_band| {rows[label][1]:.3e} mu {rep['mu']}")
+    assert rows["bf16x3"][0] <= 1e-6 and rows["bf16x3"][1] <= 1e-6
+    assert rows["bf16+3"][0] <= 1e-6 and rows["bf16+3"][1] <= 1e-6
+    assert rows["none"][0] > rows["bf16x3"][0]    # the refinement does the work

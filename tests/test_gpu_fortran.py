@@ -42,6 +42,28 @@ def test_fortran_drop_in(tmp_path, cols):
     assert np.abs(P - res.P).max() <= 1e-8
     assert abs(eband - O.band_structure(n, res.P, Hf)) <= 1e-7
     assert abs(nelp - O.get_nel(n, res.P, Sf)) <= 1e-9
+    # the log reads like the reference's: the keys the reference's own log
+    # scraper uses (collect_and_plot_results.py:42-57: a line starting
+    # "Submatrix " carries the build time as field 2, " number of
+    # submatrices=" the count as field 4) and the timer block of
+    # source/timer.f90:165-214 with the labels of source/sm.f90:76-386
+    lines = out.stdout.splitlines()
+    tsm = [float(ln.split()[1]) for ln in lines if ln.startswith("Submatrix ")]
+    nsm = [int(ln.split()[3]) for ln in lines if ln.startswith(" number of submatrices=")]
+    assert len(tsm) == 1 and tsm[0] > 0.0
+    assert nsm == [len(res.groups)]
+    tags = [ln[2:42].rstrip() for ln in lines if ln.startswith("* ")]
+    assert tags[:12] == ["alloc", "submatrix combination", "submatrix stats", "mu iter",
+                         "submatrix construction", "submatrix matrix function",
+                         "submatrix eigendecomp", "submatrix fe", "submatrix occ",
+                         "submatrix Nel", "submatrix final iter", "submatrix dealloc"]
+    assert len(tags) == 20 and all(t == "" for t in tags[12:])
+    # the sequential-record copy reads back through the ptb_dump record reader
+    from ptb_amd import formats
+    recs = formats.read_records(str(fout) + ".seq")
+    assert len(recs) == 2
+    assert np.frombuffer(recs[0], dtype="<f8").tobytes() == P.tobytes()
+    assert np.frombuffer(recs[1], dtype="<f8").tolist() == [nelp, eband]
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no flang)")
