@@ -95,6 +95,82 @@ def traffic_per_launch(kernel, workload, mixed=False):
     return None
 
 
+BF16_PEAK_TFS = 2500.0   # dense BF16 matrix-core peak; split-BF16x3 runs 3 products per step
+
+
+def cpu_info():
+    """CPU model, the cores this process may use, and the BLAS numpy calls."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = "unknown"
+    try:
+        from threadpoolctl import threadpool_info
+        for lib in threadpool_info():
+            if lib.get("user_api") == "blas":
+                blas = f"{lib.get('internal_api')} {lib.get('version')} ({os.path.basename(lib.get('filepath', ''))})"
+                break
+    except Exception:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"cpu_model": model, "cores_available": avail, "blas": blas}
+
+
+def gemm_site_flops(site, ms, cs, nos, mixed):
+    """Algorithmic flops of one build's products at a GEMM site, from the
+    submatrix dimensions m, owned columns c and occupied states n_o."""
+    m = np.asarray(ms, dtype=np.float64)
+    c = np.asarray(cs, dtype=np.float64)
+    no = np.asarray(nos, dtype=np.float64)
+    if site == "gemm_potrf":
+        return float(np.sum(m ** 3) / 3.0)
+    if site == "gemm_linv":
+        return float(np.sum(m ** 3) / 3.0)
+    if site == "gemm_kform":
+        return float(np.sum(m ** 3))
+    if site == "gemm_sytrd":
+        return float(np.sum(2.0 * m ** 3 / 3.0))
+    if site == "gemm_dc":
+        return float(np.sum(2.0 * m ** 3 / 3.0))
+    if site == "gemm_u":
+        return float(np.sum(m * m * no))
+    if site == "gemm_fe":
+        return float(np.sum(2.0 * m * m * 2.0 * c)) * (2.0 if mixed else 1.0)
+    if site == "gemm_q":
+        f = np.sum(4.0 * m * m * (2.0 * c + no))
+        return float(f + (np.sum(8.0 * m * m * no) if mixed else 0.0))
+    if site == "gemm_refine":
+        return float(np.sum(3 * 2.0 * m * m * no + 2 * 2.0 * m * no * no))
+    return None
+
+
+def cpu_baseline_full_h2o64(budget_note=""):
+    """One complete oracle build (numpy/scipy restatement of sm_stupid_simple)
+    of the 64-water box at filter 1e-4, atoms_columns: the reference's own
+    CPU-runnable case."""
+    from ptb_amd import geometry, surrogate
+    from oracle import ptb_oracle as O
+    mol = geometry.config_molecule("h2o_64")
+    H, S = surrogate.build_hs(mol)
+    O.filter_matrix_sym(S, mol.ndim, 1e-4)
+    O.filter_matrix_sym(H, mol.ndim, 1e-4)
+    t0 = time.perf_counter()
+    O.sm_stupid_simple(mol.ndim, mol.nel, H, S, mol.n, mol.xyz, mol.aoat, 2, 2)
+    t = time.perf_counter() - t0
+    return {"value": 1.0 / t, "unit": "builds/s", "seconds_per_build": round(t, 2),
+            "workload": "h2o_64 (64 H2O, ndim 1472, 192 submatrices), filter 1e-4, atoms_columns",
+            "kind": "port", "sample": "one complete build, mu bisection, scatter and pack included"}
+
+
 def cpu_baseline(mol, H, S, cols, budget_s=20.0):
     """The oracle (numpy/scipy restatement of source/sm.f90) timed on a bounded
     sample of the same workload: every submatrix's gather + dsygvd + fe +
@@ -134,10 +210,12 @@ def cpu_baseline(mol, H, S, cols, budget_s=20.0):
                 break
     per_sm = t_sm / done
     est = t_setup + per_sm * nsm
+    info = cpu_info()
     return {"value": 1.0 / est, "unit": "builds/s", "cores": cores, "kind": "port",
-            "sample": f"oracle (scipy dsygvd, OpenBLAS, {cores} threads): {done} of {nsm} "
-                      f"submatrices timed ({t_sm:.1f} s) + dense setup ({t_setup:.1f} s), "
-                      f"scaled to all submatrices: {est:.1f} s per build"}
+            "sample": f"extrapolated: oracle (scipy dsygvd, {cores} threads) timed on {done} of "
+                      f"{nsm} submatrices ({t_sm:.1f} s) + dense setup ({t_setup:.1f} s), scaled "
+                      f"to all submatrices: {est:.1f} s per build (no bisection, scatter or pack)",
+            **info}
 
 
 def run_batch(args):
@@ -296,7 +374,7 @@ def main():
         m, _, _ = eng.sm_spectra(rep["nsm"], rep["sum_m"])
         results[prec] = {"ms_per_step": ms_step, "gpu_ms_per_step": gpu_ms / args.steps,
                          "rep": rep, "kernels": kern, "m": m,
-                         "P": dP.cpu().numpy().copy()}
+                         "P": dP.clone() if n > 20000 else dP.cpu().numpy().copy()}
         log(f"[{prec}] {ms_step:.1f} ms/build (stream {gpu_ms / args.steps:.1f} ms), mu {rep['mu']}, "
             f"nsm {rep['nsm']}, max m {rep['max_m']}")
         for k, (ms, cnt) in sorted(kern.items(), key=lambda kv: -kv[1][0]):
@@ -331,6 +409,53 @@ def main():
                 "algorithmic_bytes_per_launch": round(units / max(cnt, 1), 0),
                 "ms_per_launch": round(per_launch_ms, 3), "share_of_build": round(ms / r["ms_per_step"], 3)}
         break
+    # per-site rooflines of the main build: every GEMM site against the peak of
+    # its arithmetic (FP64 matrix cores, or the BF16 matrix cores for the
+    # split-BF16x3 sites of the mixed build), the HBM-bound kernels in GB/s
+    starts = np.r_[0, np.flatnonzero(np.diff(mol.aoat)) + 1, n]
+    cs = np.diff(starts) if wl["columns"] == 2 else None
+    sites = []
+    if cs is not None and len(cs) == len(r["m"]):
+        _, eigs, _ = eng.sm_spectra(r["rep"]["nsm"], r["rep"]["sum_m"])
+        nos = np.array([int(np.sum(e < r["rep"]["mu"])) for e in eigs])
+        lowp_sites = {"gemm_dc"} if main_prec == "mixed" else set()
+        for site, (ms, cnt) in sorted(r["kernels"].items(), key=lambda kv: -kv[1][0]):
+            fl = gemm_site_flops(site, r["m"], cs, nos, main_prec == "mixed")
+            if fl is None or ms <= 0:
+                continue
+            tf = fl / (ms * 1e-3) / 1e12
+            if site in lowp_sites:
+                sites.append({"site": site, "bound": "mfma", "arith": "bf16x3 (3 bf16 products)",
+                              "algorithmic_tflop": round(fl / 1e12, 4), "ms": round(ms, 3),
+                              "achieved": round(tf, 2), "peak": round(BF16_PEAK_TFS / 3.0, 1),
+                              "unit": "TFLOP/s", "frac": round(tf / (BF16_PEAK_TFS / 3.0), 4)})
+            else:
+                sites.append({"site": site, "bound": "mfma", "arith": "fp64",
+                              "algorithmic_tflop": round(fl / 1e12, 4), "ms": round(ms, 3),
+                              "achieved": round(tf, 2), "peak": FP64_PEAK_TFS,
+                              "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFS, 4)})
+        for name in ("sytrd_panel", "gather", "pattern"):
+            if name in r["kernels"]:
+                ms, cnt = r["kernels"][name]
+                bound, units, unit = roofline_model(name, r["m"], n, mixed=main_prec == "mixed")
+                gbs = units / (ms * 1e-3) / 1e9
+                sites.append({"site": name, "bound": bound, "algorithmic_gb": round(units / 1e9, 4),
+                              "ms": round(ms, 3), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                              "unit": unit, "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        if "dmat_scatter" in r["kernels"]:
+            ms, _ = r["kernels"]["dmat_scatter"]
+            mm = np.asarray(r["m"], dtype=np.float64)
+            # owned columns of U_o read (m x n_o per submatrix... the c rows of
+            # it the columns need) and the packed P written once
+            units = float(np.sum(8.0 * mm * nos)) + 8.0 * n * (n + 1) / 2.0
+            gbs = units / (ms * 1e-3) / 1e9
+            sites.append({"site": "dmat_scatter", "bound": "hbm", "algorithmic_gb": round(units / 1e9, 4),
+                          "ms": round(ms, 3), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        gbs = 16.0 * n * (n + 1) / 2.0 / (filt_ms / max(filt_n, 1) * 1e-3) / 1e9
+        sites.append({"site": "filter", "bound": "hbm", "algorithmic_gb": round(16.0 * n * (n + 1) / 2.0 / 1e9, 4),
+                      "ms": round(filt_ms / max(filt_n, 1), 3), "achieved": round(gbs, 1),
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)})
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -351,23 +476,44 @@ def main():
         "fp64_builds_per_s": round(1000.0 / results["fp64"]["ms_per_step"], 4) if "fp64" in results else None,
         "filter_ms_per_matrix": round(filt_ms / max(filt_n, 1), 4),
         "roofline": roof,
+        "roofline_sites": sites,
     }
-    Hf = dH.cpu().numpy()
-    Sf = dS.cpu().numpy()
-    # the drop-in on host arrays (PCIe transfers of packed H, S in and P out
-    # included) and the filter + build figure; neither is `value`
-    Ph = np.empty_like(Hf)
-    t0 = time.perf_counter()
-    eng.sm_stupid_simple(n, mol.nel, Hf, Sf, Ph, mol.n, mol.xyz, mol.aoat, wl["columns"], 2,
-                         precision=main_prec)
-    out["host_arrays_ms_per_build"] = round((time.perf_counter() - t0) * 1000.0, 3)
+    big = n > 20000   # the 2048-water box: no host copies of its 9 GB matrices
+    Hf = dH.cpu().numpy() if not big else None
     out["filter_plus_build_ms"] = round(2.0 * filt_ms / max(filt_n, 1) + r["ms_per_step"], 3)
+    if not big:
+        # the drop-in on host arrays (PCIe transfers of packed H, S in and P
+        # out included); not `value`
+        Sf = dS.cpu().numpy()
+        Ph = np.empty_like(Hf)
+        t0 = time.perf_counter()
+        eng.sm_stupid_simple(n, mol.nel, Hf, Sf, Ph, mol.n, mol.xyz, mol.aoat, wl["columns"], 2,
+                             precision=main_prec)
+        out["host_arrays_ms_per_build"] = round((time.perf_counter() - t0) * 1000.0, 3)
     if "fp64" in results and "mixed" in results:
         P64, Pm = results["fp64"]["P"], results["mixed"]["P"]
+        if big:
+            e64 = eng.band_structure_dev(n, P64, dH)
+            em = eng.band_structure_dev(n, Pm, dH)
+            dpmax = float((P64 - Pm).abs().max().item())
+        else:
+            e64 = eng.band_structure(n, P64, Hf)
+            em = eng.band_structure(n, Pm, Hf)
+            dpmax = float(np.abs(P64 - Pm).max())
         out["mixed_vs_fp64"] = {
             "speedup": round(results["fp64"]["ms_per_step"] / results["mixed"]["ms_per_step"], 3),
-            "dP_max": float(np.abs(P64 - Pm).max()),
-            "dE_band": float(abs(eng.band_structure(n, Pm, Hf) - eng.band_structure(n, P64, Hf)))}
+            "dP_max": dpmax, "dE_band": float(abs(em - e64))}
+        # the precision ladder: the same build with plain BF16 products
+        # (8-bit operands) in place of split-BF16x3, against FP64
+        if not big:
+            eng.sm_stupid_simple_dev(n, mol.nel, dH, dS, dP, mol.n, mol.xyz, mol.aoat,
+                                     wl["columns"], 2, precision="mixed_bf16")
+            Pb = dP.cpu().numpy()
+            out["precision_errors"] = {
+                "bf16x3": {"dP_max": out["mixed_vs_fp64"]["dP_max"],
+                           "dE_band": out["mixed_vs_fp64"]["dE_band"]},
+                "bf16": {"dP_max": float(np.abs(P64 - Pb).max()),
+                         "dE_band": float(abs(eng.band_structure(n, Pb, Hf) - e64))}}
         # the metric is quoted at |dE_band| <= 1e-6 Eh (and max |dP| <= 1e-6):
         # a mixed build outside that bound posts no throughput
         acc = out["mixed_vs_fp64"]
@@ -375,11 +521,17 @@ def main():
             out["value"] = None
             out["accuracy_gate"] = "failed: mixed build outside |dE_band|, max|dP| <= 1e-6"
             log(f"accuracy gate failed: dP_max {acc['dP_max']:.3e}, dE_band {acc['dE_band']:.3e}")
-    if not args.no_cpu:
+    if not args.no_cpu and not big:
         try:
             out["cpu_baseline"] = cpu_baseline(mol, Hf, Sf, wl["columns"], args.cpu_budget)
         except Exception as exc:  # reported, never fatal
             out["cpu_baseline"] = {"error": str(exc)}
+        try:
+            from threadpoolctl import threadpool_limits
+            with threadpool_limits(limits=max(1, min(16, os.cpu_count() or 1))):
+                out["cpu_baseline_h2o64_full"] = cpu_baseline_full_h2o64()
+        except Exception as exc:
+            out["cpu_baseline_h2o64_full"] = {"error": str(exc)}
     print(json.dumps(out), flush=True)
 
 

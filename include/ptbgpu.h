@@ -243,6 +243,11 @@ int ptbgpu_band_energy(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const dou
 int ptbgpu_idempotency(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const double *S,
                        double *deviation);
 
+/* The same traces on device-resident packed matrices. */
+int ptbgpu_get_nel_dev(ptbgpu_ctx *ctx, int64_t ndim, const double *dP, const double *dS,
+                       double *nel);
+int ptbgpu_band_energy_dev(ptbgpu_ctx *ctx, int64_t ndim, const double *dP, const double *dH,
+                           double *energy);
 
 /* ---- purification (source/purification.f90) -------------------------------
  * The fields of tPurificationSet (source/purification_settings.f90:29-86) that

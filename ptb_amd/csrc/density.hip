@@ -465,12 +465,14 @@ int refine_scatter(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc,
 int dmat_compact(ptbgpu_ctx *ctx, const MolDev *mols, int nmol, int64_t max_packed,
                  const SubDesc *ddesc, int nsm, int max_nocc, int max_own, const int *dgcols,
                  const int *doloc, const int *dmap, const double *wocc, const int *nocc,
-                 const double *U) {
+                 const double *U, bool zero) {
     ProfScope prof_(ctx, "dmat_scatter");
-    const int zb = (int)std::max<int64_t>(1, std::min<int64_t>((max_packed + kT - 1) / kT,
-                                                              (int64_t)ctx->num_cus * 8 / nmol + 1));
-    hipLaunchKernelGGL(k_zero_packed, dim3(zb, nmol), dim3(kT), 0, ctx->stream, mols);
-    PTB_CHECK_LAUNCH(ctx);
+    if (zero) {
+        const int zb = (int)std::max<int64_t>(1, std::min<int64_t>((max_packed + kT - 1) / kT,
+                                                                  (int64_t)ctx->num_cus * 8 / nmol + 1));
+        hipLaunchKernelGGL(k_zero_packed, dim3(zb, nmol), dim3(kT), 0, ctx->stream, mols);
+        PTB_CHECK_LAUNCH(ctx);
+    }
     if (max_nocc <= 0 || max_own <= 0) return PTBGPU_OK;
     const size_t shmem = sizeof(double) * (size_t)kG * max_nocc;
     if (shmem > 150 * 1024)

@@ -239,6 +239,18 @@ int ptbgpu_band_energy(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const dou
     return ptb::weighted_dot(ctx, dP, dH, ndim, energy);
 }
 
+int ptbgpu_get_nel_dev(ptbgpu_ctx *ctx, int64_t ndim, const double *dP, const double *dS,
+                       double *nel) {
+    PTB_TRY(metric_args(ctx, ndim, dP, dS, nel));
+    return ptb::weighted_dot(ctx, dP, dS, ndim, nel);
+}
+
+int ptbgpu_band_energy_dev(ptbgpu_ctx *ctx, int64_t ndim, const double *dP, const double *dH,
+                           double *energy) {
+    PTB_TRY(metric_args(ctx, ndim, dP, dH, energy));
+    return ptb::weighted_dot(ctx, dP, dH, ndim, energy);
+}
+
 int ptbgpu_idempotency(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const double *S,
                        double *deviation) {
     PTB_TRY(metric_args(ctx, ndim, P, S, deviation));

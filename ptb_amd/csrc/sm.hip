@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "sm_internal.hpp"
@@ -277,7 +278,7 @@ int kmeans_groups(ptbgpu_ctx *ctx, int64_t ndim, int64_t n, const double *xyz,
 // tridiagonalisation of K; A holds K in FP64, both triangles):
 //   Zo = occupied columns of Z;  R = Q^T K Q Zo - Zo diag(theta_o);  C = Z^T R;
 //   E = C / (theta_j - theta_i) on unoccupied rows;  Zo += Z E;
-//   Zo <- Zo (3 I - Zo^T Zo) / 2;  theta_j += C(j, j)
+//   Zo <- Zo (3 I - Zo^T Zo) / 2, twice;  theta_j += C(j, j)
 // and the refined columns and eigenvalues are written back into Z and eig.
 // Uses W, X and R as scratch.
 int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
@@ -331,7 +332,10 @@ int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
     PTB_TRY(run("rf_g3", false, lowp, [&](const SubDesc &d, int n) {
         return GemmDesc{Z + d.off, X + d.off, W + d.off, d.m, n, d.m, d.ld, d.ld, d.ld, 1.0, 1.0, 0, 0};
     }));
-    // R = (3 I - W^T W) / 2, X = W R
+    // Loewdin orthonormalisation by two Newton-Schulz steps of the inverse
+    // square root, R = (3 I - W^T W) / 2, X = W R, then the same on X (each
+    // step squares the departure from orthonormality: the low-precision
+    // merge products leave the columns of Z orthonormal only to about 1e-5)
     PTB_TRY(run("rf_g4", true, kLowpNone, [&](const SubDesc &d, int n) {
         return GemmDesc{W + d.off, W + d.off, R + d.off, n, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
@@ -339,7 +343,32 @@ int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
     PTB_TRY(run("rf_g5", false, kLowpNone, [&](const SubDesc &d, int n) {
         return GemmDesc{W + d.off, R + d.off, X + d.off, d.m, n, n, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
-    return ptb::refine_scatter(ctx, ddesc, nsm, max_n, nocc, oidx, X, thn, Z, eig);
+    PTB_TRY(run("rf_g6", true, kLowpNone, [&](const SubDesc &d, int n) {
+        return GemmDesc{X + d.off, X + d.off, R + d.off, n, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
+    }));
+    PTB_TRY(ptb::refine_shift(ctx, ddesc, nsm, max_n, nocc, R));
+    PTB_TRY(run("rf_g7", false, kLowpNone, [&](const SubDesc &d, int n) {
+        return GemmDesc{X + d.off, R + d.off, W + d.off, d.m, n, n, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
+    }));
+    return ptb::refine_scatter(ctx, ddesc, nsm, max_n, nocc, oidx, W, thn, Z, eig);
+}
+
+// Device bytes the per-submatrix arenas of one chunk may take: the free
+// device memory plus what this context already holds (its buffers are
+// reused), with a reserve for everything else.  PTBGPU_ARENA_BUDGET_MB, when
+// set, caps it (tests use it to force the chunked passes on small systems).
+size_t arena_budget(ptbgpu_ctx *ctx) {
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) freeb = (size_t)64 << 30;
+    size_t held = 0;
+    for (const auto &kv : ctx->bufs) held += kv.second.bytes;
+    const size_t reserve = (size_t)4 << 30;
+    size_t budget = freeb + held > reserve ? (size_t)(0.85 * (double)(freeb + held - reserve)) : 0;
+    if (const char *env = std::getenv("PTBGPU_ARENA_BUDGET_MB")) {
+        const long long mb = std::atoll(env);
+        if (mb > 0) budget = std::min(budget, (size_t)mb << 20);
+    }
+    return std::max<size_t>(budget, (size_t)64 << 20);
 }
 
 // One molecule of a batch: its packed H, S (filtered) and P on the device and
@@ -473,128 +502,63 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     if (!dmap || !doloc) return PTBGPU_E_NOMEM;
     PTB_TRY(ptb::group_map(ctx, uni, dmols, dsmol, ddesc, nsm, dmap, dgptr, dgcols, doloc));
     tm.mark(2, st);
-    // ---- construction
-    const size_t abytes = sizeof(double) * (size_t)arena;
-    auto *A = static_cast<double *>(ptb::ws(ctx, "sm_A", abytes));
-    auto *B = static_cast<double *>(ptb::ws(ctx, "sm_B", abytes));
-    auto *Z = static_cast<double *>(ptb::ws(ctx, "sm_Z", abytes));
-    auto *W = static_cast<double *>(ptb::ws(ctx, "sm_W", abytes));
-    auto *X = static_cast<double *>(ptb::ws(ctx, "sm_X", abytes));
-    auto *vec = static_cast<double *>(ptb::ws(ctx, "sm_vec", sizeof(double) * rows * 9));
-    auto *bis = static_cast<double *>(ptb::ws(ctx, "sm_bis", sizeof(double) * kBisStride * nmol));
-    auto *flags = static_cast<int *>(ptb::ws(ctx, "sm_flags", sizeof(int) * nsm));
-    auto *nocc = static_cast<int *>(ptb::ws(ctx, "sm_nocc", sizeof(int) * nsm));
-    if (!A || !B || !Z || !W || !X || !vec || !bis || !flags || !nocc) return PTBGPU_E_NOMEM;
-    double *dvec = vec, *evec = vec + rows, *tau = vec + 2 * rows, *fe = vec + 3 * rows;
-    double *rot = vec + 4 * rows;   // 4 * rows
-    double *wocc = vec + 8 * rows;  // rows
-    // owned-row projection buffers: XA = [L(O,:)^T, L^-1 e_O] (m x 2c),
-    // F = Z^T Q^T XA (m x 2c)
-    std::vector<int64_t> xoff(nsm);
-    std::vector<int> c2(nsm);
-    int64_t xtot = 0;
-    for (int s = 0; s < nsm; ++s) {
-        const int c = desc[s].gend - desc[s].gbeg;
-        xoff[s] = xtot;
-        c2[s] = 2 * c;
-        xtot += (int64_t)desc[s].ld * 2 * c;
-    }
-    auto *XA = static_cast<double *>(ptb::ws(ctx, "sm_xa", sizeof(double) * std::max<int64_t>(xtot, 1)));
-    auto *F = static_cast<double *>(ptb::ws(ctx, "sm_f", sizeof(double) * std::max<int64_t>(xtot, 1)));
-    auto *Li = static_cast<double *>(ptb::ws(ctx, "sm_Li", abytes));
-    int64_t *dxoff;
-    PTB_TRY(upload(ctx, "sm_xoff", xoff, &dxoff));
-    if (!XA || !F || !Li) return PTBGPU_E_NOMEM;
-    PTB_TRY(ptb::gather(ctx, dmols, ddesc, nsm, max_m, dmap, A, B));
-    tm.mark(3, st);
-    // ---- eigendecomposition (every submatrix of every molecule)
-    PTB_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int) * nsm, st));
-    PTB_TRY(ptb::potrf_batch(ctx, desc, ddesc, B, flags));
-    // K = L^-1 H L^-T from Li = L^-1: Z = Li H, A = Z Li^T (the lower triangle,
-    // which the tridiagonalisation reads; the mixed solve, whose refinement
-    // multiplies by K, mirrors it into the upper one).  A(i, j), i >= j, needs
-    // Z(i, k) for k <= j only, so Z is formed on the tiles on and below the
-    // diagonal (diagonal tiles whole: the second product reads them whole)
+    // ---- the numeric phases, in chunks of submatrices that fit the device
+    // arenas.  One chunk (every workload up to the 512-water box) runs each
+    // phase once.  With several chunks the build is done in passes, each
+    // chunk recomputed in every pass (the arithmetic is deterministic, so a
+    // recomputed chunk is bit-identical): pass 1 gives every submatrix's
+    // eigenvalues and fe, the bisection runs over all of them; for the mixed
+    // solve a second pass refines the states occupied at that mu and the
+    // bisection runs again; the last pass writes the density.
     const bool mixed = opts.precision != PTBGPU_FP64;
     // the BF16 matrix cores for the products whose rounding the FP64
     // refinement removes: split-BF16x3 by default, plain BF16 on request
     const int lowp = !mixed ? kLowpNone
                             : (opts.precision == PTBGPU_MIXED_BF16 ? kLowpBF16 : kLowpBF16x3);
-    PTB_TRY(ptb::inv_lower_batch(ctx, desc, ddesc, B, Li, X));
+    // bytes per arena element: A, B, Z, W, X, Li in FP64; the mixed solve's
+    // float copy of K, FP64 reflectors and refinement residual
+    const size_t per_elem = sizeof(double) * 6 + (mixed ? sizeof(float) + 2 * sizeof(double) : 0);
+    std::vector<int> cbeg{0};
     {
-        std::vector<GemmDesc> g1(nsm), g2(nsm);
+        const size_t budget = arena_budget(ctx);
+        size_t used = 0;
         for (int s = 0; s < nsm; ++s) {
-            const SubDesc &d = desc[s];
-            g1[s] = {Li + d.off, A + d.off, Z + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 2,
-                     kTriALower};
-            g2[s] = {Z + d.off, Li + d.off, A + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 1,
-                     kTriBUpper};
+            // the arenas, the owned-row projections, the tridiagonalisation's
+            // panel buffer (ld x 96) and the 32/64/128-reflector T factors
+            const size_t c = per_elem * (size_t)desc[s].ld * desc[s].m +
+                             2 * sizeof(double) * (size_t)desc[s].ld * 2 * (desc[s].gend - desc[s].gbeg) +
+                             sizeof(double) * (size_t)desc[s].ld * (96 + 32 + 64 + 128);
+            if (used > 0 && used + c > budget) {
+                cbeg.push_back(s);
+                used = 0;
+            }
+            used += c;
         }
-        GemmDesc *dg1, *dg2;
-        PTB_TRY(upload(ctx, "sm_gk1", g1, &dg1));
-        PTB_TRY(upload(ctx, "sm_gk2", g2, &dg2));
-        ptb::TileList t1, t2;
-        t1.add(g1.data(), nsm);
-        t2.add(g2.data(), nsm);
-        PTB_TRY(t1.upload(ctx, "sm_wk1"));
-        PTB_TRY(t2.upload(ctx, "sm_wk2"));
-        PTB_TRY(t1.launch(ctx, 0, false, false, dg1, "kform"));
-        PTB_TRY(t2.launch(ctx, 0, false, true, dg2, "kform"));
-        if (mixed) PTB_TRY(ptb::mirror_lower_batch(ctx, ddesc, nsm, max_m, A));
+        cbeg.push_back(nsm);
     }
-    // mixed precision: the tridiagonalisation on float storage, its reflectors
-    // in V, and K (FP64) kept in A for the refinement below
-    float *K32 = nullptr;
-    double *V = nullptr, *R = nullptr, *thn = nullptr;
+    const int nchunk = (int)cbeg.size() - 1;
+    // per-row vectors are global (indexed by mapoff): d, e, tau, fe, rotations,
+    // occupation weights
+    auto *vec = static_cast<double *>(ptb::ws(ctx, "sm_vec", sizeof(double) * rows * 9));
+    auto *bis = static_cast<double *>(ptb::ws(ctx, "sm_bis", sizeof(double) * kBisStride * nmol));
+    auto *bis1 = static_cast<double *>(ptb::ws(ctx, "sm_bis1", sizeof(double) * kBisStride * nmol));
+    auto *flags = static_cast<int *>(ptb::ws(ctx, "sm_flags", sizeof(int) * nsm));
+    auto *nocc = static_cast<int *>(ptb::ws(ctx, "sm_nocc", sizeof(int) * nsm));
+    double *thn = nullptr;
     int *oidx = nullptr;
     if (mixed) {
-        K32 = static_cast<float *>(ptb::ws(ctx, "sm_K32", sizeof(float) * (size_t)arena));
-        V = static_cast<double *>(ptb::ws(ctx, "sm_V", abytes));
-        R = static_cast<double *>(ptb::ws(ctx, "sm_R", abytes));
         thn = static_cast<double *>(ptb::ws(ctx, "sm_thn", sizeof(double) * rows));
         oidx = static_cast<int *>(ptb::ws(ctx, "sm_oidx", sizeof(int) * rows));
-        if (!K32 || !V || !R || !thn || !oidx) return PTBGPU_E_NOMEM;
+        if (!thn || !oidx) return PTBGPU_E_NOMEM;
     }
-    QFactor qf;
-    PTB_TRY(ptb::eig_tri_batch(ctx, desc, ddesc, A, Z, W, X, dvec, evec, tau, rot, qf, K32, V,
-                               lowp));
-    tm.mark(4, st);
-    // ---- fe from the owned rows: F = Z^T Q^T [L(O,:)^T, L^-1 e_O]
-    std::vector<int> hflags(nsm);
-    PTB_HIP(ctx, hipMemcpyAsync(hflags.data(), flags, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
-    PTB_TRY(ptb::xall_build(ctx, ddesc, nsm, max_own, doloc, B, Li, dxoff, XA));
-    PTB_TRY(ptb::apply_q(ctx, desc, qf, XA, xoff, c2, true, "x"));           // Q^T XA
-    GemmDesc *dgf;
-    ptb::TileList tf;
-    {
-        std::vector<GemmDesc> gf(nsm);
-        for (int s = 0; s < nsm; ++s) {
-            const SubDesc &d = desc[s];
-            gf[s] = {Z + d.off, XA + xoff[s], F + xoff[s], d.m, c2[s], d.m, d.ld, d.ld, d.ld, 1.0, 0.0};
-        }
-        PTB_TRY(upload(ctx, "sm_gf", gf, &dgf));
-        tf.add(gf.data(), nsm);
-        PTB_TRY(tf.upload(ctx, "sm_wf"));
-        PTB_TRY(tf.launch(ctx, 0, true, false, dgf, "fe"));
-    }
-    PTB_HIP(ctx, hipStreamSynchronize(st));
+    if (!vec || !bis || !bis1 || !flags || !nocc) return PTBGPU_E_NOMEM;
+    double *dvec = vec, *evec = vec + rows, *tau = vec + 2 * rows, *fe = vec + 3 * rows;
+    double *rot = vec + 4 * rows;   // 4 * rows
+    double *wocc = vec + 8 * rows;  // rows
+    PTB_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int) * nsm, st));
     std::vector<int64_t> not_spd(nmol, 0);
     int64_t not_spd_all = 0;
-    for (int s = 0; s < nsm; ++s)
-        if (hflags[s] & 1) {
-            ++not_spd[desc[s].mol];
-            ++not_spd_all;
-            desc[s].flags |= 1;
-        }
-    if (not_spd_all) {
-        if (opts.strict)
-            return ptb::fail(ctx, PTBGPU_E_NOT_SPD, "%lld overlap submatrices are not positive definite",
-                             (long long)not_spd_all);
-        PTB_TRY(upload(ctx, "sm_desc", desc, &ddesc));
-    }
-    PTB_TRY(ptb::fe_proj(ctx, ddesc, nsm, max_m, dxoff, F, fe));
-    tm.mark(5, st);
-    // ---- mu bisection, per molecule over its own states
+    // mu bisection, per molecule over its own states
     int64_t max_rows = 0;
     std::vector<int64_t> nelrefs(nmol);
     for (int i = 0; i < nmol; ++i) {
@@ -607,48 +571,238 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     auto bisect_all = [&]() -> int {
         return ptb::bisect_batch(ctx, dvec, fe, drbeg, dnelref, nmol, max_rows, bis);
     };
-    PTB_TRY(bisect_all());
     const int sweeps = !mixed || opts.refine_steps < 0 ? 0 : (int)std::max<int64_t>(1, opts.refine_steps);
-    if (sweeps > 0) {
-        // FP64 refinement of the states occupied at this mu, then fe and the
-        // bisection again with the refined eigenpairs
-        for (int sweep = 0; sweep < sweeps; ++sweep)
-            PTB_TRY(refine_occupied(ctx, desc, ddesc, qf, A, Z, W, X, R, dvec, bis, wocc, nocc,
-                                    oidx, thn, lowp));
-        PTB_TRY(tf.launch(ctx, 0, true, false, dgf, "fe"));
-        PTB_TRY(ptb::fe_proj(ctx, ddesc, nsm, max_m, dxoff, F, fe));
-        PTB_TRY(bisect_all());
-    }
-    tm.mark(6, st);
-    // ---- final density: occupied columns only, Y_o = Q Z_o, U_o = L^-T Y_o
-    PTB_TRY(ptb::occ_compact(ctx, ddesc, nsm, max_m, dvec, bis, Z, W, wocc, nocc));
-    std::vector<int> hnocc(nsm);
-    PTB_HIP(ctx, hipMemcpyAsync(hnocc.data(), nocc, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
-    PTB_HIP(ctx, hipStreamSynchronize(st));
-    int max_nocc = 0;
-    std::vector<int64_t> aoff(nsm);
-    for (int s = 0; s < nsm; ++s) {
-        max_nocc = std::max(max_nocc, hnocc[s]);
-        aoff[s] = desc[s].off;
-    }
-    PTB_TRY(ptb::apply_q(ctx, desc, qf, W, aoff, hnocc, false, "o"));        // W = Q Z_o
-    {
-        // X = L^-T Q Z_o = Li^T W
-        std::vector<GemmDesc> gu(nsm);
-        for (int s = 0; s < nsm; ++s) {
-            const SubDesc &d = desc[s];
-            gu[s] = {Li + d.off, W + d.off, X + d.off, d.m, hnocc[s], d.m, d.ld, d.ld, d.ld, 1.0, 0.0,
-                     0, kTriAUpper};
+
+    // The arenas and tables of one chunk: submatrices [c0, c1) at offsets from
+    // the chunk's arena base.
+    struct Chunk {
+        int c0 = 0, c1 = 0, n = 0, max_m = 0, max_own = 0;
+        std::string sfx;  // suffix of the chunk's table names
+        std::vector<SubDesc> desc;
+        SubDesc *ddesc = nullptr;
+        std::vector<int64_t> xoff;
+        std::vector<int> c2;
+        int64_t *dxoff = nullptr;
+        double *A, *B, *Z, *W, *X, *Li, *XA, *F;
+        float *K32 = nullptr;
+        double *V = nullptr, *R = nullptr;
+        QFactor qf;
+        GemmDesc *dgf = nullptr;
+        ptb::TileList tf;
+    };
+    auto setup_chunk = [&](int k, Chunk &c) -> int {
+        c = Chunk{};
+        c.c0 = cbeg[k];
+        c.c1 = cbeg[k + 1];
+        c.n = c.c1 - c.c0;
+        c.desc.assign(desc.begin() + c.c0, desc.begin() + c.c1);
+        int64_t off = 0, xtot = 0;
+        c.xoff.resize(c.n);
+        c.c2.resize(c.n);
+        for (int i = 0; i < c.n; ++i) {
+            SubDesc &d = c.desc[i];
+            d.off = off;
+            off += (int64_t)d.ld * d.m;
+            c.max_m = std::max(c.max_m, d.m);
+            const int own = d.gend - d.gbeg;
+            c.max_own = std::max(c.max_own, own);
+            c.xoff[i] = xtot;
+            c.c2[i] = 2 * own;
+            xtot += (int64_t)d.ld * 2 * own;
         }
-        GemmDesc *dgu;
-        PTB_TRY(upload(ctx, "sm_gu", gu, &dgu));
-        ptb::TileList tu;
-        tu.add(gu.data(), nsm);
-        PTB_TRY(tu.upload(ctx, "sm_wu"));
-        PTB_TRY(tu.launch(ctx, 0, true, false, dgu, "u"));
+        c.sfx = nchunk == 1 ? std::string() : "_c" + std::to_string(k);
+        PTB_TRY(upload(ctx, ("sm_cdesc" + c.sfx).c_str(), c.desc, &c.ddesc));
+        PTB_TRY(upload(ctx, ("sm_xoff" + c.sfx).c_str(), c.xoff, &c.dxoff));
+        const size_t abytes = sizeof(double) * (size_t)std::max<int64_t>(off, 1);
+        c.A = static_cast<double *>(ptb::ws(ctx, "sm_A", abytes));
+        c.B = static_cast<double *>(ptb::ws(ctx, "sm_B", abytes));
+        c.Z = static_cast<double *>(ptb::ws(ctx, "sm_Z", abytes));
+        c.W = static_cast<double *>(ptb::ws(ctx, "sm_W", abytes));
+        c.X = static_cast<double *>(ptb::ws(ctx, "sm_X", abytes));
+        c.Li = static_cast<double *>(ptb::ws(ctx, "sm_Li", abytes));
+        // owned-row projection buffers: XA = [L(O,:)^T, L^-1 e_O] (m x 2c),
+        // F = Z^T Q^T XA (m x 2c)
+        c.XA = static_cast<double *>(ptb::ws(ctx, "sm_xa", sizeof(double) * std::max<int64_t>(xtot, 1)));
+        c.F = static_cast<double *>(ptb::ws(ctx, "sm_f", sizeof(double) * std::max<int64_t>(xtot, 1)));
+        if (!c.A || !c.B || !c.Z || !c.W || !c.X || !c.Li || !c.XA || !c.F) return PTBGPU_E_NOMEM;
+        if (mixed) {
+            c.K32 = static_cast<float *>(ptb::ws(ctx, "sm_K32", sizeof(float) * (size_t)std::max<int64_t>(off, 1)));
+            c.V = static_cast<double *>(ptb::ws(ctx, "sm_V", abytes));
+            c.R = static_cast<double *>(ptb::ws(ctx, "sm_R", abytes));
+            if (!c.K32 || !c.V || !c.R) return PTBGPU_E_NOMEM;
+        }
+        return PTBGPU_OK;
+    };
+    // construction, eigendecomposition and fe of a chunk: eigenvalues into
+    // dvec, fe into fe (rows mapoff of each submatrix)
+    auto eigen_chunk = [&](Chunk &c, bool timed) -> int {
+        const int n = c.n;
+        const std::vector<SubDesc> &cd = c.desc;
+        PTB_TRY(ptb::gather(ctx, dmols, c.ddesc, n, c.max_m, dmap, c.A, c.B));
+        if (timed) tm.mark(3, st);
+        PTB_TRY(ptb::potrf_batch(ctx, cd, c.ddesc, c.B, flags + c.c0));
+        // K = L^-1 H L^-T from Li = L^-1: Z = Li H, A = Z Li^T (the lower
+        // triangle, which the tridiagonalisation reads; the mixed solve, whose
+        // refinement multiplies by K, mirrors it into the upper one).  A(i, j),
+        // i >= j, needs Z(i, k) for k <= j only, so Z is formed on the tiles
+        // on and below the diagonal (diagonal tiles whole: the second product
+        // reads them whole)
+        PTB_TRY(ptb::inv_lower_batch(ctx, cd, c.ddesc, c.B, c.Li, c.X));
+        {
+            std::vector<GemmDesc> g1(n), g2(n);
+            for (int s = 0; s < n; ++s) {
+                const SubDesc &d = cd[s];
+                g1[s] = {c.Li + d.off, c.A + d.off, c.Z + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0,
+                         0.0, 2, kTriALower};
+                g2[s] = {c.Z + d.off, c.Li + d.off, c.A + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0,
+                         0.0, 1, kTriBUpper};
+            }
+            GemmDesc *dg1, *dg2;
+            PTB_TRY(upload(ctx, "sm_gk1", g1, &dg1));
+            PTB_TRY(upload(ctx, "sm_gk2", g2, &dg2));
+            ptb::TileList t1, t2;
+            t1.add(g1.data(), n);
+            t2.add(g2.data(), n);
+            PTB_TRY(t1.upload(ctx, "sm_wk1"));
+            PTB_TRY(t2.upload(ctx, "sm_wk2"));
+            PTB_TRY(t1.launch(ctx, 0, false, false, dg1, "kform"));
+            PTB_TRY(t2.launch(ctx, 0, false, true, dg2, "kform"));
+            if (mixed) PTB_TRY(ptb::mirror_lower_batch(ctx, c.ddesc, n, c.max_m, c.A));
+        }
+        // mixed precision: the tridiagonalisation on float storage, its
+        // reflectors in V, and K (FP64) kept in A for the refinement
+        c.qf = QFactor{};
+        PTB_TRY(ptb::eig_tri_batch(ctx, cd, c.ddesc, c.A, c.Z, c.W, c.X, dvec, evec, tau, rot, c.qf,
+                                   c.K32, c.V, lowp));
+        if (timed) tm.mark(4, st);
+        // ---- fe from the owned rows: F = Z^T Q^T [L(O,:)^T, L^-1 e_O]
+        std::vector<int> hflags(n);
+        PTB_HIP(ctx, hipMemcpyAsync(hflags.data(), flags + c.c0, sizeof(int) * n,
+                                    hipMemcpyDeviceToHost, st));
+        PTB_TRY(ptb::xall_build(ctx, c.ddesc, n, c.max_own, doloc, c.B, c.Li, c.dxoff, c.XA));
+        PTB_TRY(ptb::apply_q(ctx, cd, c.qf, c.XA, c.xoff, c.c2, true, "x"));    // Q^T XA
+        {
+            std::vector<GemmDesc> gf(n);
+            for (int s = 0; s < n; ++s) {
+                const SubDesc &d = cd[s];
+                gf[s] = {c.Z + d.off, c.XA + c.xoff[s], c.F + c.xoff[s], d.m, c.c2[s], d.m, d.ld, d.ld,
+                         d.ld, 1.0, 0.0};
+            }
+            PTB_TRY(upload(ctx, "sm_gf", gf, &c.dgf));
+            c.tf = ptb::TileList{};
+            c.tf.add(gf.data(), n);
+            PTB_TRY(c.tf.upload(ctx, "sm_wf"));
+            PTB_TRY(c.tf.launch(ctx, 0, true, false, c.dgf, "fe"));
+        }
+        PTB_HIP(ctx, hipStreamSynchronize(st));
+        int bad = 0;
+        for (int s = 0; s < n; ++s)
+            if (hflags[s] & 1) {
+                ++bad;
+                c.desc[s].flags |= 1;
+                desc[c.c0 + s].flags |= 1;
+            }
+        if (bad) {
+            if (opts.strict)
+                return ptb::fail(ctx, PTBGPU_E_NOT_SPD, "%d overlap submatrices are not positive definite",
+                                 bad);
+            PTB_TRY(upload(ctx, ("sm_cdesc" + c.sfx).c_str(), c.desc, &c.ddesc));
+        }
+        PTB_TRY(ptb::fe_proj(ctx, c.ddesc, n, c.max_m, c.dxoff, c.F, fe));
+        if (timed) tm.mark(5, st);
+        return PTBGPU_OK;
+    };
+    // FP64 refinement of the states occupied at mu = mu_ref[mol], then fe again
+    auto refine_chunk = [&](Chunk &c, const double *mu_ref) -> int {
+        for (int sweep = 0; sweep < sweeps; ++sweep)
+            PTB_TRY(refine_occupied(ctx, c.desc, c.ddesc, c.qf, c.A, c.Z, c.W, c.X, c.R, dvec, mu_ref,
+                                    wocc, nocc + c.c0, oidx, thn, lowp));
+        PTB_TRY(c.tf.launch(ctx, 0, true, false, c.dgf, "fe"));
+        return ptb::fe_proj(ctx, c.ddesc, c.n, c.max_m, c.dxoff, c.F, fe);
+    };
+    // final density of a chunk: occupied columns only, Y_o = Q Z_o,
+    // U_o = L^-T Y_o, owned columns scattered into P (zeroed first when zero)
+    auto density_chunk = [&](Chunk &c, bool zero) -> int {
+        const int n = c.n;
+        PTB_TRY(ptb::occ_compact(ctx, c.ddesc, n, c.max_m, dvec, bis, c.Z, c.W, wocc, nocc + c.c0));
+        std::vector<int> hnocc(n);
+        PTB_HIP(ctx, hipMemcpyAsync(hnocc.data(), nocc + c.c0, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+        PTB_HIP(ctx, hipStreamSynchronize(st));
+        int max_nocc = 0;
+        std::vector<int64_t> aoff(n);
+        for (int s = 0; s < n; ++s) {
+            max_nocc = std::max(max_nocc, hnocc[s]);
+            aoff[s] = c.desc[s].off;
+        }
+        PTB_TRY(ptb::apply_q(ctx, c.desc, c.qf, c.W, aoff, hnocc, false, "o"));     // W = Q Z_o
+        {
+            // X = L^-T Q Z_o = Li^T W
+            std::vector<GemmDesc> gu(n);
+            for (int s = 0; s < n; ++s) {
+                const SubDesc &d = c.desc[s];
+                gu[s] = {c.Li + d.off, c.W + d.off, c.X + d.off, d.m, hnocc[s], d.m, d.ld, d.ld, d.ld,
+                         1.0, 0.0, 0, kTriAUpper};
+            }
+            GemmDesc *dgu;
+            PTB_TRY(upload(ctx, "sm_gu", gu, &dgu));
+            ptb::TileList tu;
+            tu.add(gu.data(), n);
+            PTB_TRY(tu.upload(ctx, "sm_wu"));
+            PTB_TRY(tu.launch(ctx, 0, true, false, dgu, "u"));
+        }
+        return ptb::dmat_compact(ctx, dmols, nmol, max_packed, c.ddesc, n, max_nocc, c.max_own, dgcols,
+                                 doloc, dmap, wocc, nocc + c.c0, c.X, zero);
+    };
+    auto save_mu = [&]() -> int {
+        PTB_HIP(ctx, hipMemcpyAsync(bis1, bis, sizeof(double) * kBisStride * nmol,
+                                    hipMemcpyDeviceToDevice, st));
+        return PTBGPU_OK;
+    };
+    Chunk ch;
+    if (nchunk == 1) {
+        PTB_TRY(setup_chunk(0, ch));
+        PTB_TRY(eigen_chunk(ch, true));
+        PTB_TRY(bisect_all());
+        if (sweeps > 0) {
+            // FP64 refinement of the states occupied at this mu, then fe and
+            // the bisection again with the refined eigenpairs
+            PTB_TRY(save_mu());
+            PTB_TRY(refine_chunk(ch, bis1));
+            PTB_TRY(bisect_all());
+        }
+        tm.mark(6, st);
+        PTB_TRY(density_chunk(ch, true));
+    } else {
+        tm.mark(3, st);
+        for (int k = 0; k < nchunk; ++k) {
+            PTB_TRY(setup_chunk(k, ch));
+            PTB_TRY(eigen_chunk(ch, false));
+        }
+        tm.mark(4, st);
+        tm.mark(5, st);
+        PTB_TRY(bisect_all());
+        if (sweeps > 0) {
+            PTB_TRY(save_mu());
+            for (int k = 0; k < nchunk; ++k) {
+                PTB_TRY(setup_chunk(k, ch));
+                PTB_TRY(eigen_chunk(ch, false));
+                PTB_TRY(refine_chunk(ch, bis1));
+            }
+            PTB_TRY(bisect_all());
+        }
+        tm.mark(6, st);
+        for (int k = 0; k < nchunk; ++k) {
+            PTB_TRY(setup_chunk(k, ch));
+            PTB_TRY(eigen_chunk(ch, false));
+            if (sweeps > 0) PTB_TRY(refine_chunk(ch, bis1));
+            PTB_TRY(density_chunk(ch, k == 0));
+        }
     }
-    PTB_TRY(ptb::dmat_compact(ctx, dmols, nmol, max_packed, ddesc, nsm, max_nocc, max_own, dgcols,
-                              doloc, dmap, wocc, nocc, X));
+    for (int s = 0; s < nsm; ++s)
+        if (desc[s].flags & 1) {
+            ++not_spd[desc[s].mol];
+            ++not_spd_all;
+        }
     tm.mark(7, st);
     std::vector<double> hb((size_t)kBisStride * nmol);
     PTB_HIP(ctx, hipMemcpyAsync(hb.data(), bis, sizeof(double) * hb.size(), hipMemcpyDeviceToHost, st));

@@ -250,11 +250,11 @@ int refine_shift(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, c
                  double *G);
 int refine_scatter(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_nocc, const int *nocc,
                    const int *oidx, const double *Xo, const double *thn, double *Z, double *eig);
-// packed P of every molecule zeroed, then the owned columns of every
-// submatrix written into its molecule's P
+// packed P of every molecule zeroed (when zero), then the owned columns of
+// every submatrix of ddesc written into its molecule's P
 int dmat_compact(ptbgpu_ctx *ctx, const MolDev *mols, int nmol, int64_t max_packed,
                  const SubDesc *ddesc, int nsm, int max_nocc, int max_own, const int *dgcols,
                  const int *doloc, const int *dmap, const double *wocc, const int *nocc,
-                 const double *U);
+                 const double *U, bool zero = true);
 
 }  // namespace ptb

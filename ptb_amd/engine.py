@@ -115,7 +115,8 @@ def _lib():
             getattr(lib, name).argtypes = batch_args
     if hasattr(lib, "ptbgpu_sm_spectra"):
         lib.ptbgpu_sm_spectra.argtypes = [VP, I64, I64, VP, VP, VP]
-    for name in ("ptbgpu_get_nel", "ptbgpu_band_energy", "ptbgpu_idempotency"):
+    for name in ("ptbgpu_get_nel", "ptbgpu_band_energy", "ptbgpu_idempotency",
+                 "ptbgpu_get_nel_dev", "ptbgpu_band_energy_dev"):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [VP, I64, VP, VP, ctypes.POINTER(F64)]
     for name in ("ptbgpu_matrix_sign", "ptbgpu_matrix_root"):
@@ -509,6 +510,18 @@ class Engine:
 
     def get_nel(self, ndim, P, S) -> float:
         return self._scalar(self.lib.ptbgpu_get_nel, ndim, P, S)
+
+    def get_nel_dev(self, ndim, dP, dS) -> float:
+        for t, name in ((dP, "dP"), (dS, "dS")):
+            _dev_f64(t, ndim * (ndim + 1) // 2, name, self.device)
+        _producers_done(self.device)
+        return self._scalar(self.lib.ptbgpu_get_nel_dev, ndim, dP, dS)
+
+    def band_structure_dev(self, ndim, dP, dH) -> float:
+        for t, name in ((dP, "dP"), (dH, "dH")):
+            _dev_f64(t, ndim * (ndim + 1) // 2, name, self.device)
+        _producers_done(self.device)
+        return self._scalar(self.lib.ptbgpu_band_energy_dev, ndim, dP, dH)
 
     def band_structure(self, ndim, P, H) -> float:
         return self._scalar(self.lib.ptbgpu_band_energy, ndim, P, H)

@@ -43,3 +43,14 @@ def test_idempotency_of_exact_projector(engine):
     P = O.packsym(O.dmat(occ, C))
     assert engine.idempotency_deviation(n, P, S) < 1e-9
     assert abs(engine.get_nel(n, P, S) - mol.nel) < 1e-10
+
+
+def test_device_traces_equal_host_traces(engine):
+    """get_nel / band_structure on device-resident matrices give exactly the
+    host entry points' values (same kernels, same reduction order)."""
+    import torch
+    mol, H, S, P = _density()
+    n = mol.ndim
+    dP, dS, dH = (torch.from_numpy(x.copy()).cuda() for x in (P, S, H))
+    assert engine.get_nel_dev(n, dP, dS) == engine.get_nel(n, P, S)
+    assert engine.band_structure_dev(n, dP, dH) == engine.band_structure(n, P, H)
