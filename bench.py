@@ -260,11 +260,29 @@ def run_batch(args):
             eng.sm_stupid_simple_dev(*it, 2, 2, precision=prec)
             torch.cuda.synchronize()
             single.append((time.perf_counter() - t1) * 1000.0)
+        # the same batch with the eigen phase replayed from a captured hipGraph
+        # (one build to see the shapes, one to capture, then K timed replays)
+        eng.set_tuning("graph", 1)
+        for _ in range(2):
+            eng.sm_stupid_simple_batch(items, 2, 2, precision=prec, device=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            eng.sm_stupid_simple_batch(items, 2, 2, precision=prec, device=True)
+        torch.cuda.synchronize()
+        gms = (time.perf_counter() - t0) * 1000.0 / args.steps
+        same = all(np.array_equal(a, it[4].cpu().numpy()) for a, it in zip(Ps[prec], items))
+        held = eng.get_tuning("graphs")
+        eng.set_tuning("graph", 0)
         res[prec] = {"ms_per_batch": round(ms, 3), "molecules_per_s": round(len(items) * 1000.0 / ms, 2),
+                     "graph": {"ms_per_batch": round(gms, 3),
+                               "molecules_per_s": round(len(items) * 1000.0 / gms, 2),
+                               "graphs_held": held, "bit_identical_to_eager": same},
                      "single_p50_ms": round(float(np.percentile(single, 50)), 3),
                      "single_p99_ms": round(float(np.percentile(single, 99)), 3),
                      "single_sum_ms": round(float(np.sum(single)), 1),
-                     "nsm": int(sum(r["nsm"] for r in reps))}
+                     "nsm": int(sum(r["nsm"] for r in reps)),
+                     "dc_bucket_min": eng.get_tuning("dc_bucket_min")}
         log(f"[{prec}] {ms:.1f} ms per batch of {len(items)}, single-molecule p50 "
             f"{res[prec]['single_p50_ms']} ms p99 {res[prec]['single_p99_ms']} ms "
             f"(sum {res[prec]['single_sum_ms']} ms)")

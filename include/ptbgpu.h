@@ -125,6 +125,23 @@ int ptbgpu_sync(ptbgpu_ctx *ctx);
 /* The context's HIP stream (hipStream_t), for callers that time or order work. */
 int ptbgpu_stream(ptbgpu_ctx *ctx, void **stream);
 
+/* Tuning knobs of a context (no reference counterpart; set by the autotuner,
+ * ptb_amd/tune.py).  "dc_bucket_min": the divide-and-conquer merge nodes of
+ * one tree level are launched in power-of-two size classes, each with the LDS
+ * of its class; classes below this floor join the smallest launched one, and
+ * 0 launches a level at once (default 256).  "graph": 1 makes the
+ * construction and eigendecomposition phase of a build (one chunk) replay a
+ * captured hipGraph once the same shapes and buffers have been seen before
+ * (the first such build runs eagerly, the second captures); 0 (default) turns
+ * it off and drops the held graphs.  ptbgpu_get_tuning also reads "graphs",
+ * the number of graphs held, and "graph_failures", the captures that failed
+ * (those builds ran eagerly). */
+int ptbgpu_set_tuning(ptbgpu_ctx *ctx, const char *key, int64_t value);
+int ptbgpu_get_tuning(ptbgpu_ctx *ctx, const char *key, int64_t *value);
+/* What the eigen phase of the last build did with graphs on: "replayed",
+ * "captured", "first sight of the shapes: eager", or why a capture failed. */
+const char *ptbgpu_graph_note(const ptbgpu_ctx *ctx);
+
 /* Kernel timing (no reference counterpart; used by the benchmark).  With
  * profiling on, the engine brackets each launch of its hot kernels with HIP
  * events on the context's stream; the query syncs and returns the summed

@@ -19,6 +19,10 @@ int fail(ptbgpu_ctx *ctx, int status, const char *fmt, ...) {
 void *ws(ptbgpu_ctx *ctx, const char *name, size_t bytes) {
     DevBuf &b = ctx->bufs[name];
     if (b.bytes >= bytes && b.ptr) return b.ptr;
+    if (ctx->capturing) {  // no allocation inside a graph capture
+        fail(ctx, PTBGPU_E_NOMEM, "buffer '%s' would grow during a graph capture", name);
+        return nullptr;
+    }
     ctx->shadow.erase(name);  // new memory: no table is known to be in it
     if (b.ptr) {
         (void)hipStreamSynchronize(ctx->stream);
@@ -37,13 +41,21 @@ void *ws(ptbgpu_ctx *ctx, const char *name, size_t bytes) {
     return b.ptr;
 }
 
-void *upload_table(ptbgpu_ctx *ctx, const char *name, const void *src, size_t bytes) {
+void *upload_table(ptbgpu_ctx *ctx, const char *name_in, const void *src, size_t bytes) {
+    // tables of a graph-held phase live under the graph's own prefix, so that
+    // other builds do not overwrite what its replays read
+    const std::string full = ctx->table_prefix + name_in;
+    const char *name = full.c_str();
     void *d = ws(ctx, name, bytes ? bytes : 1);
     if (!d || bytes == 0) return d;
     auto it = ctx->shadow.find(name);
     if (it != ctx->shadow.end() && it->second.size() == bytes &&
         memcmp(it->second.data(), src, bytes) == 0)
         return d;
+    if (ctx->capturing) {  // a graph must not hold a copy from host memory
+        fail(ctx, PTBGPU_E_ARG, "table '%s' changed during a graph capture", name);
+        return nullptr;
+    }
     if (hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
         ctx->shadow.erase(name);
         fail(ctx, PTBGPU_E_HIP, "upload of '%s' failed: %s", name, hipGetErrorString(hipGetLastError()));
@@ -96,6 +108,10 @@ const char *ptbgpu_strerror(int64_t status) {
     }
 }
 
+const char *ptbgpu_graph_note(const ptbgpu_ctx *ctx) {
+    return ctx ? ctx->graph_note.c_str() : "no context";
+}
+
 const char *ptbgpu_last_error(const ptbgpu_ctx *ctx) {
     return ctx ? ctx->last_error.c_str() : "no context";
 }
@@ -122,6 +138,7 @@ int ptbgpu_finalize(ptbgpu_ctx *ctx) {
     if (!ctx) return PTBGPU_E_ARG;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx->graphs.clear();
     prof_clear(ctx);
     for (auto &kv : ctx->bufs)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
@@ -130,6 +147,47 @@ int ptbgpu_finalize(ptbgpu_ctx *ctx) {
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PTBGPU_OK;
+}
+
+int ptbgpu_set_tuning(ptbgpu_ctx *ctx, const char *key, int64_t value) {
+    if (!ctx || !key) return PTBGPU_E_ARG;
+    const std::string k(key);
+    if (k == "dc_bucket_min") {
+        if (value < 0 || value > 4096) return ptb::fail(ctx, PTBGPU_E_ARG, "dc_bucket_min out of range");
+        ctx->dc_bucket_min = value;
+        return PTBGPU_OK;
+    }
+    if (k == "graph") {
+        ctx->graph_mode = value != 0;
+        if (!ctx->graph_mode) {
+            ctx->graphs.clear();
+            ctx->graph_seen.clear();
+        }
+        return PTBGPU_OK;
+    }
+    return ptb::fail(ctx, PTBGPU_E_ARG, "unknown tuning key '%s'", key);
+}
+
+int ptbgpu_get_tuning(ptbgpu_ctx *ctx, const char *key, int64_t *value) {
+    if (!ctx || !key || !value) return PTBGPU_E_ARG;
+    const std::string k(key);
+    if (k == "dc_bucket_min") {
+        *value = ctx->dc_bucket_min;
+        return PTBGPU_OK;
+    }
+    if (k == "graph") {
+        *value = ctx->graph_mode;
+        return PTBGPU_OK;
+    }
+    if (k == "graphs") {  // instantiated graphs held by the context
+        *value = (int64_t)ctx->graphs.size();
+        return PTBGPU_OK;
+    }
+    if (k == "graph_failures") {  // captures or instantiations that did not succeed
+        *value = ctx->graph_failures;
+        return PTBGPU_OK;
+    }
+    return ptb::fail(ctx, PTBGPU_E_ARG, "unknown tuning key '%s'", key);
 }
 
 int ptbgpu_sync(ptbgpu_ctx *ctx) {

@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <memory>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -37,6 +39,20 @@ struct ptbgpu_ctx {
     // submatrices and their summed dimensions; 0 while none or during a build
     int64_t last_nsm = 0;
     int64_t last_sum_m = 0;
+    // tuning knobs (ptbgpu_set_tuning): the smallest size class of the
+    // divide-and-conquer merge launches (0: one launch per tree level)
+    int64_t dc_bucket_min = 256;
+    // "graph": the eigen phase of repeated one-chunk builds replays a captured
+    // hipGraph; the cache of instantiated graphs (keyed by shapes and
+    // buffers), the keys seen once, and whether a capture is in progress (no
+    // allocation or table upload may happen then)
+    int64_t graph_mode = 0;
+    std::map<uint64_t, std::shared_ptr<void>> graphs;
+    std::set<uint64_t> graph_seen;
+    bool capturing = false;
+    std::string graph_note;     // what the last eigen phase did (replayed, captured, why not)
+    std::string table_prefix;   // prefix of table names while a graph-held phase runs
+    int64_t graph_failures = 0;
 };
 
 namespace ptb {

@@ -633,20 +633,52 @@ int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int 
 
 int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, int count,
                    int max_n, const double *e, double *D, double *Z, double *W, double *X,
-                   double *rot, GemmDesc *dgemm, const TileList &tiles, int lowp) {
+                   double *rot, GemmDesc *dgemm, const TileList &tiles, int lowp,
+                   const std::vector<int> *sizes, int nfp64) {
     if (count <= 0) return PTBGPU_OK;
-    int npow = 1;
-    while (npow < max_n) npow <<= 1;
-    const size_t shmem = (size_t)npow * (6 * sizeof(double) + 4 * sizeof(int));
-    if (shmem > 150 * 1024)
+    auto pow2 = [](int n) {
+        int p = 1;
+        while (p < n) p <<= 1;
+        return p;
+    };
+    const int npow_max = pow2(max_n);
+    if ((size_t)npow_max * (6 * sizeof(double) + 4 * sizeof(int)) > 150 * 1024)
         return fail(ctx, PTBGPU_E_ARG, "merge of %d rows exceeds the LDS budget", max_n);
+    // Size buckets: the nodes come largest first, and each run of nodes whose
+    // power-of-two size class is the same is one launch with LDS for that
+    // class, so that small nodes are not held to the largest node's LDS (and
+    // occupancy).  Classes below the tuned floor (ctx->dc_bucket_min) join the
+    // smallest launched class; a floor of 0 launches the level at once.
+    std::vector<std::pair<int, int>> runs;  // (first node, npow)
+    const int floor = (int)ctx->dc_bucket_min;
+    if (!sizes || floor <= 0) {
+        runs.push_back({0, npow_max});
+    } else {
+        for (int i = 0; i < count; ++i) {
+            const int np = std::max(pow2((*sizes)[i]), std::min(pow2(floor), npow_max));
+            if (runs.empty() || runs.back().second != np) runs.push_back({i, np});
+        }
+    }
     {
         ProfScope prof_(ctx, "dc_merge");
-        hipLaunchKernelGGL(k_dc_merge, dim3(count), dim3(kMT), shmem, ctx->stream, ddesc, dnodes,
-                           npow, e, D, Z, W, X, rot, dgemm);
-        PTB_CHECK_LAUNCH(ctx);
+        for (size_t r = 0; r < runs.size(); ++r) {
+            const int i0 = runs[r].first;
+            const int i1 = r + 1 < runs.size() ? runs[r + 1].first : count;
+            const int np = runs[r].second;
+            const size_t shmem = (size_t)np * (6 * sizeof(double) + 4 * sizeof(int));
+            hipLaunchKernelGGL(k_dc_merge, dim3(i1 - i0), dim3(kMT), shmem, ctx->stream, ddesc,
+                               dnodes + i0, np, e, D, Z, W, X, rot, dgemm + 2 * i0);
+            PTB_CHECK_LAUNCH(ctx);
+        }
     }
-    return tiles.launch(ctx, 0, false, false, dgemm, "dc", false, lowp);
+    // the first nfp64 nodes' products in FP64 (launch 0 of the tile list),
+    // the rest in the low-precision arithmetic (the next launch)
+    nfp64 = std::min(std::max(nfp64, 0), count);
+    int launch = 0;
+    if (nfp64 > 0) PTB_TRY(tiles.launch(ctx, launch++, false, false, dgemm, "dc", false, kLowpNone));
+    if (nfp64 < count)
+        PTB_TRY(tiles.launch(ctx, launch, false, false, dgemm + 2 * nfp64, "dc", false, lowp));
+    return PTBGPU_OK;
 }
 
 }  // namespace ptb

@@ -940,7 +940,9 @@ void TileList::build(std::vector<GemmTile> &work) {
     }
 }
 
-int TileList::upload(ptbgpu_ctx *ctx, const char *name) {
+int TileList::upload(ptbgpu_ctx *ctx, const char *name_in) {
+    const std::string full = ctx->table_prefix + name_in;  // see upload_table
+    const char *name = full.c_str();
     // the shadow entry of `name` holds the shapes of the table in its buffer
     // (ws drops it when the buffer is reallocated); "<name>#meta" holds the
     // launch starts and flags built from them
@@ -971,6 +973,8 @@ int TileList::upload(ptbgpu_ctx *ctx, const char *name) {
         ctx->shadow[mkey] = meta;
         return PTBGPU_OK;
     }
+    if (ctx->capturing)  // a graph must not hold a copy from host memory
+        return fail(ctx, PTBGPU_E_ARG, "tile list '%s' changed during a graph capture", name);
     const size_t bytes = sizeof(GemmTile) * work.size();
     void *d = ws(ctx, name, bytes);
     if (!d) return PTBGPU_E_NOMEM;

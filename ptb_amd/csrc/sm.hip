@@ -37,9 +37,14 @@ struct Timer {
     void mark(int slot, hipStream_t s) {
         if (ok) (void)hipEventRecord(ev[slot], s);
     }
+    // 0 for a pair that was not recorded; the failed query's error is
+    // cleared so that the next launch check does not report it
     float between(int a, int b) {
         float ms = 0.f;
-        if (ok) (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
+        if (ok && hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) {
+            (void)hipGetLastError();
+            ms = 0.f;
+        }
         return ms;
     }
     ~Timer() {
@@ -371,6 +376,80 @@ size_t arena_budget(ptbgpu_ctx *ctx) {
     return std::max<size_t>(budget, (size_t)64 << 20);
 }
 
+// A captured eigen phase: the graph and the host-side state its launches
+// leave behind (the blocked reflector factors and the fe product's table).
+struct EigenGraph {
+    hipGraphExec_t exec = nullptr;
+    QFactor qf;
+    ptb::TileList tf;
+    GemmDesc *dgf = nullptr;
+    ~EigenGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
+// Runs the eigen phase of a one-chunk build through a hipGraph keyed by its
+// shapes and buffers: the first build with a key runs eagerly (and uploads
+// every table), the second captures the launches (every table and buffer is
+// then already in place, so the capture holds kernels only) and replays the
+// graph, later builds replay it.  replayed = false tells the caller to run the
+// launches eagerly (first sight of the key, or a capture that could not be
+// made).
+template <typename F>
+int eigen_graph_run(ptbgpu_ctx *ctx, uint64_t key, F launches, QFactor &qf, ptb::TileList &tf,
+                    GemmDesc *&dgf, bool &replayed) {
+    replayed = false;
+    auto it = ctx->graphs.find(key);
+    if (it != ctx->graphs.end()) {
+        auto *g = static_cast<EigenGraph *>(it->second.get());
+        PTB_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+        qf = g->qf;
+        tf = g->tf;
+        dgf = g->dgf;
+        replayed = true;
+        ctx->graph_note = "replayed";
+        return PTBGPU_OK;
+    }
+    if (!ctx->graph_seen.count(key)) {
+        ctx->graph_seen.insert(key);
+        ctx->graph_note = "first sight of the shapes: eager";
+        return PTBGPU_OK;
+    }
+    PTB_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    const int rc = launches();
+    ctx->capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != PTBGPU_OK || ec != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        ctx->graph_seen.erase(key);
+        ctx->graph_note = std::string("capture failed: ") +
+                          (rc != PTBGPU_OK ? ctx->last_error : std::string(hipGetErrorString(ec)));
+        ++ctx->graph_failures;
+        return PTBGPU_OK;  // not capturable now: run eagerly
+    }
+    auto g = std::make_shared<EigenGraph>();
+    const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        g->exec = nullptr;
+        (void)hipGetLastError();
+        ctx->graph_note = std::string("instantiation failed: ") + hipGetErrorString(ei);
+        ++ctx->graph_failures;
+        return PTBGPU_OK;
+    }
+    ctx->graph_note = "captured";
+    g->qf = qf;
+    g->tf = tf;
+    g->dgf = dgf;
+    ctx->graphs[key] = g;
+    PTB_HIP(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    replayed = true;
+    return PTBGPU_OK;
+}
+
 // One molecule of a batch: its packed H, S (filtered) and P on the device and
 // the host arrays the column grouping reads.
 struct MolIn {
@@ -635,7 +714,9 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     };
     // construction, eigendecomposition and fe of a chunk: eigenvalues into
     // dvec, fe into fe (rows mapoff of each submatrix)
-    auto eigen_chunk = [&](Chunk &c, bool timed) -> int {
+    // the launches of the construction, eigendecomposition and fe products
+    // (no host synchronisation inside: the phase a graph can hold)
+    auto eigen_launches = [&](Chunk &c, bool timed) -> int {
         const int n = c.n;
         const std::vector<SubDesc> &cd = c.desc;
         PTB_TRY(ptb::gather(ctx, dmols, c.ddesc, n, c.max_m, dmap, c.A, c.B));
@@ -676,9 +757,6 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
                                    c.K32, c.V, lowp));
         if (timed) tm.mark(4, st);
         // ---- fe from the owned rows: F = Z^T Q^T [L(O,:)^T, L^-1 e_O]
-        std::vector<int> hflags(n);
-        PTB_HIP(ctx, hipMemcpyAsync(hflags.data(), flags + c.c0, sizeof(int) * n,
-                                    hipMemcpyDeviceToHost, st));
         PTB_TRY(ptb::xall_build(ctx, c.ddesc, n, c.max_own, doloc, c.B, c.Li, c.dxoff, c.XA));
         PTB_TRY(ptb::apply_q(ctx, cd, c.qf, c.XA, c.xoff, c.c2, true, "x"));    // Q^T XA
         {
@@ -694,6 +772,53 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             PTB_TRY(c.tf.upload(ctx, "sm_wf"));
             PTB_TRY(c.tf.launch(ctx, 0, true, false, c.dgf, "fe"));
         }
+        return PTBGPU_OK;
+    };
+    // construction, eigendecomposition and fe of a chunk: eigenvalues into
+    // dvec, fe into fe (rows mapoff of each submatrix).  With graphs on
+    // (ptbgpu_set_tuning "graph"), a one-chunk build whose shapes and buffers
+    // were seen before replays the launches as one captured hipGraph.
+    auto eigen_chunk = [&](Chunk &c, bool timed) -> int {
+        const int n = c.n;
+        bool replayed = false;
+        if (ctx->graph_mode && nchunk == 1 && !ctx->profiling) {
+            uint64_t key = 1469598103934665603ull;
+            auto mix = [&](const void *p, size_t b) {
+                const unsigned char *q = static_cast<const unsigned char *>(p);
+                for (size_t i = 0; i < b; ++i) key = (key ^ q[i]) * 1099511628211ull;
+            };
+            // the shapes name the graph's private tables; the shapes and
+            // the addresses of every shared buffer key the graph itself
+            mix(c.desc.data(), sizeof(SubDesc) * c.desc.size());
+            const int64_t knobs[] = {(int64_t)lowp, (int64_t)mixed, ctx->dc_bucket_min};
+            mix(knobs, sizeof(knobs));
+            char pfx[32];
+            snprintf(pfx, sizeof(pfx), "g%016llx:", (unsigned long long)key);
+            for (const auto &kv : ctx->bufs) {
+                if (kv.first.compare(0, 1, "g") == 0 && kv.first.find(':') != std::string::npos)
+                    continue;  // the graphs' own tables
+                mix(kv.first.data(), kv.first.size());
+                mix(&kv.second.ptr, sizeof(void *));
+                mix(&kv.second.bytes, sizeof(size_t));
+            }
+            ctx->table_prefix = pfx;
+            // a replay records the construction and eigen marks around the
+            // graph (the eager launches record them themselves)
+            if (timed) tm.mark(3, st);
+            const int rc = eigen_graph_run(ctx, key, [&]() { return eigen_launches(c, false); }, c.qf,
+                                           c.tf, c.dgf, replayed);
+            if (rc != PTBGPU_OK || replayed) ctx->table_prefix.clear();
+            PTB_TRY(rc);
+            if (replayed && timed) tm.mark(4, st);
+        }
+        // eager launches (with graphs on, the first sight of the shapes runs
+        // under the graph's table prefix, so that the capture finds them)
+        const int erc = replayed ? PTBGPU_OK : eigen_launches(c, timed);
+        ctx->table_prefix.clear();
+        PTB_TRY(erc);
+        std::vector<int> hflags(n);
+        PTB_HIP(ctx, hipMemcpyAsync(hflags.data(), flags + c.c0, sizeof(int) * n,
+                                    hipMemcpyDeviceToHost, st));
         PTB_HIP(ctx, hipStreamSynchronize(st));
         int bad = 0;
         for (int s = 0; s < n; ++s)
@@ -896,11 +1021,23 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
     PTB_TRY(upload(ctx, "sm_leaves", leaves, &dleaves));
     PTB_TRY(dc_leaves(ctx, ddesc, dleaves, (int)leaves.size(), dvec, evec, Z));
     for (size_t lv = 0; lv < levels.size(); ++lv) {
-        // largest nodes first, so that the level's last workgroups are small
+        // The low-precision merge products carry an error of about
+        // sqrt(n) 2^-17 per vector element at a node of n rows, and the tree's
+        // levels add up.  Submatrices of up to 1024 rows keep the error within
+        // what one FP64 refinement sweep removes; larger ones (the 64-water
+        // box as one block, the largest of the 2048-water box) keep FP64
+        // merge products.  Per level: the nodes of large submatrices first,
+        // then the others, each group largest first (so that the level's last
+        // workgroups are small); the two groups' products are two launches.
         auto &nodes = levels[lv];
-        std::stable_sort(nodes.begin(), nodes.end(), [](const DcNode &a, const DcNode &b) {
+        auto big = [&](const DcNode &nd) { return lowp != kLowpNone && desc[nd.s].m > 1024; };
+        std::stable_sort(nodes.begin(), nodes.end(), [&](const DcNode &a, const DcNode &b) {
+            if (big(a) != big(b)) return big(a);
             return a.hi - a.lo > b.hi - b.lo;
         });
+        int nbig = 0;
+        for (const auto &nd : nodes) nbig += big(nd) ? 1 : 0;
+        if (lowp == kLowpNone) nbig = (int)nodes.size();
         int max_n = 0;
         for (const auto &nd : nodes) max_n = std::max(max_n, nd.hi - nd.lo);
         std::string nm = "sm_level" + std::to_string(lv);
@@ -918,10 +1055,13 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
             shape[2 * i].N = shape[2 * i + 1].N = n;
         }
         TileList tl;
-        tl.add(shape.data(), (int)shape.size());
+        if (nbig > 0) tl.add(shape.data(), 2 * nbig);
+        if (nbig < (int)nodes.size()) tl.add(shape.data() + 2 * nbig, 2 * ((int)nodes.size() - nbig));
         PTB_TRY(tl.upload(ctx, ("sm_lwork" + std::to_string(lv)).c_str()));
+        std::vector<int> sizes(nodes.size());
+        for (size_t i = 0; i < nodes.size(); ++i) sizes[i] = nodes[i].hi - nodes[i].lo;
         PTB_TRY(dc_merge_level(ctx, ddesc, dnodes, (int)nodes.size(), max_n, evec, dvec, Z,
-                                    W, X, rot, dg, tl, lowp));
+                                    W, X, rot, dg, tl, lowp, &sizes, nbig));
     }
     return PTBGPU_OK;
 }

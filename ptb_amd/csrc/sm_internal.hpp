@@ -223,9 +223,13 @@ int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int 
 // kernel writes N = the node's non-deflated count <= n); dgemm holds
 // 2 x count descriptors
 // lowp: arithmetic of the merged-vector products (kLowp*)
+// sizes (host, may be NULL): the nodes' row counts, for the size-bucketed
+// merge launches.  The products of the first nfp64 nodes run in FP64 (launch 0
+// of tiles), the rest in lowp (the next launch); nfp64 >= count: all FP64.
 int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, int count,
                    int max_n, const double *e, double *D, double *Z, double *W, double *X,
-                   double *rot, GemmDesc *dgemm, const TileList &tiles, int lowp = 0);
+                   double *rot, GemmDesc *dgemm, const TileList &tiles, int lowp = 0,
+                   const std::vector<int> *sizes = nullptr, int nfp64 = 1 << 30);
 
 // density.hip
 // XA = [L(O,:)^T, Li(:,O)] per submatrix (Li = L^-1)

@@ -135,6 +135,14 @@ def _lib():
     for name in ("ptbgpu_mlpop2", "ptbgpu_mlpop2_dev"):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [VP, I64, I64, VP, VP, VP, VP, VP, VP, VP]
+    for name in ("ptbgpu_set_tuning",):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = [VP, ctypes.c_char_p, I64]
+    if hasattr(lib, "ptbgpu_graph_note"):
+        lib.ptbgpu_graph_note.restype = ctypes.c_char_p
+        lib.ptbgpu_graph_note.argtypes = [VP]
+    if hasattr(lib, "ptbgpu_get_tuning"):
+        lib.ptbgpu_get_tuning.argtypes = [VP, ctypes.c_char_p, ctypes.POINTER(I64)]
     if hasattr(lib, "ptbgpu_gemm_batch_dev"):
         lib.ptbgpu_gemm_batch_dev.argtypes = [VP, I64, I64, I64, I64, VP, VP, VP, VP, VP, VP, VP,
                                               VP, VP, F64, F64]
@@ -207,6 +215,33 @@ class Engine:
         if st != 0:
             raise PtbGpuError(st, self.lib.ptbgpu_strerror(st).decode())
         self.device = device
+        self.apply_tuning()
+
+    def set_tuning(self, key: str, value: int):
+        """A tuning knob of this context (ptbgpu_set_tuning)."""
+        self._check(self.lib.ptbgpu_set_tuning(self.ctx, key.encode(), int(value)))
+
+    def get_tuning(self, key: str) -> int:
+        v = I64(0)
+        self._check(self.lib.ptbgpu_get_tuning(self.ctx, key.encode(), ctypes.byref(v)))
+        return v.value
+
+    def graph_note(self) -> str:
+        return self.lib.ptbgpu_graph_note(self.ctx).decode()
+
+    def apply_tuning(self, path: str | None = None):
+        """Apply the autotuner's choices (ptb_amd/tuning.json by default, written
+        by ``python -m ptb_amd.tune``); a missing file leaves the defaults."""
+        import json
+        import os
+        path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+        if not os.path.exists(path):
+            return {}
+        with open(path) as fh:
+            chosen = json.load(fh).get("chosen", {})
+        for k, v in chosen.items():
+            self.set_tuning(k, v)
+        return chosen
 
     def close(self):
         if self.ctx:

@@ -104,3 +104,38 @@ def test_batch256_seed2_equals_single_and_oracle(engine):
               f"max|P - P_oracle| {err:.2e}")
         assert reps[i]["mu"] == res.mu and reps[i]["muiter"] == res.muiter
         assert err <= 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_is_bit_identical(engine, precision):
+    """With graphs on, the eigen phase of a repeated build is captured into a
+    hipGraph (second sight of the shapes) and replayed afterwards.  Every
+    replay, with other eager builds and synchronisations in between, gives
+    the eager density bit for bit."""
+    import torch
+    batch = _batch(10, seed=4)
+    items = [(m.ndim, m.nel, torch.from_numpy(H).cuda(), torch.from_numpy(S).cuda(),
+              torch.zeros(H.size, dtype=torch.float64, device="cuda"), m.n, m.xyz, m.aoat)
+             for m, H, S in batch]
+    other = _args(_batch(3, seed=9))
+    engine.sm_stupid_simple_batch(items, 2, 2, precision=precision, device=True)
+    torch.cuda.synchronize()
+    ref = [it[4].cpu().numpy().copy() for it in items]
+    engine.set_tuning("graph", 1)
+    try:
+        for rep in range(4):
+            for it in items:
+                it[4].fill_(float("nan"))
+            torch.cuda.synchronize()
+            engine.sm_stupid_simple_batch(items, 2, 2, precision=precision, device=True)
+            torch.cuda.synchronize()
+            print(f"build {rep}: {engine.graph_note()}")
+            for r, it in zip(ref, items):
+                assert np.array_equal(r, it[4].cpu().numpy()), f"build {rep}"
+            engine.sm_stupid_simple_batch(other, 2, 2)   # eager work in between
+            engine.sync()
+        held = engine.get_tuning("graphs")
+        print(f"\n{precision}: graphs held {held}")
+        assert held >= 1
+    finally:
+        engine.set_tuning("graph", 0)

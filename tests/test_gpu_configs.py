@@ -184,3 +184,44 @@ def test_chunked_passes_are_bit_identical(engine, precision):
           f"max|dP| {np.abs(Pc - P1).max():.3e}")
     assert rc["mu"] == r1["mu"] and rc["muiter"] == r1["muiter"]
     assert np.array_equal(Pc, P1)
+
+
+def test_h2o2048_full_size(engine):
+    """The 2048-water box (6144 atoms, ndim 47104, filter 1e-5): its submatrix
+    arenas exceed the GPU's memory, so the build runs in chunks and passes.
+    At full size: every atom gives a submatrix, the FP64 and the mixed build
+    converge to the same mu, the mixed density is within 1e-6 of the FP64 one
+    (max per element) and 1e-6 Eh in band energy, and tr(P S) is the electron
+    count to the accuracy of the submatrix method (0.1 %)."""
+    import torch
+    from ptb_amd import geometry, surrogate
+    mol = geometry.config_molecule("h2o_2048")
+    H, S = surrogate.build_hs(mol)
+    n = mol.ndim
+    dH, dS = torch.from_numpy(H).cuda(), torch.from_numpy(S).cuda()
+    del H, S
+    torch.cuda.synchronize()
+    engine.filter_matrix_sym_dev(dS, n, 1e-5)
+    engine.filter_matrix_sym_dev(dH, n, 1e-5)
+    import time
+    dP64 = torch.zeros_like(dH)
+    t0 = time.perf_counter()
+    r64 = engine.sm_stupid_simple_dev(n, mol.nel, dH, dS, dP64, mol.n, mol.xyz, mol.aoat, 2, 2)
+    t64 = time.perf_counter() - t0
+    dPm = torch.zeros_like(dH)
+    t0 = time.perf_counter()
+    rm = engine.sm_stupid_simple_dev(n, mol.nel, dH, dS, dPm, mol.n, mol.xyz, mol.aoat, 2, 2,
+                                     precision="mixed")
+    tm = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    dp = float((dP64 - dPm).abs().max().item())
+    e64 = engine.band_structure_dev(n, dP64, dH)
+    em = engine.band_structure_dev(n, dPm, dH)
+    nel = engine.get_nel_dev(n, dP64, dS)
+    print(f"\nh2o_2048: nsm {r64['nsm']} max m {r64['max_m']} mu {r64['mu']} / {rm['mu']}, "
+          f"muiter {r64['muiter']}; fp64 {t64:.2f} s, mixed {tm:.2f} s; "
+          f"max|dP| {dp:.3e} |dE| {abs(em - e64):.3e}; tr(PS) {nel:.6f} of {mol.nel}")
+    assert r64["nsm"] == mol.n
+    assert rm["mu"] == r64["mu"] and rm["muiter"] == r64["muiter"]
+    assert dp <= 1e-6 and abs(em - e64) <= 1e-6
+    assert abs(nel - mol.nel) <= 1e-3 * mol.nel
