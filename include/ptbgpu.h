@@ -266,6 +266,20 @@ int ptbgpu_get_nel_dev(ptbgpu_ctx *ctx, int64_t ndim, const double *dP, const do
 int ptbgpu_band_energy_dev(ptbgpu_ctx *ctx, int64_t ndim, const double *dP, const double *dH,
                            double *energy);
 
+/* The idempotency deviation without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Device inputs dP, dS (packed, ndim*(ndim+1)/2 doubles); the deviation is
+ * written to the device double *d_deviation.  The call only enqueues work on
+ * the context's stream and returns without waiting (read the result after
+ * ptbgpu_sync or an event on ptbgpu_stream); it calls no hipMalloc, hipFree
+ * or memset.  Its scratch is one static 40 MiB device array per GPU, which
+ * limits ndim to about 51000 (larger returns PTBGPU_E_ARG); every context on a
+ * GPU shares that array, so calls that overlap on one GPU (two contexts or
+ * streams at once) race on it. */
+int ptbgpu_idempotency_noalloc(ptbgpu_ctx *ctx, int64_t ndim, const double *dP, const double *dS,
+                               double *d_deviation);
+
 /* ---- purification (source/purification.f90) -------------------------------
  * The fields of tPurificationSet (source/purification_settings.f90:29-86) that
  * purification_wrapper reads, with the same codes and defaults:

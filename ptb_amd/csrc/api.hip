@@ -130,6 +130,7 @@ int ptbgpu_init(ptbgpu_ctx **out, int64_t device, int64_t flags) {
     hipDeviceProp_t prop;
     PTB_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
     ctx->num_cus = prop.multiProcessorCount;
+    PTB_TRY(ptb::noalloc_scratch_init(ctx));
     *out = ctx;
     return PTBGPU_OK;
 }

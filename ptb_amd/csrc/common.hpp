@@ -149,4 +149,7 @@ __host__ __device__ __forceinline__ int64_t pk(int64_t i, int64_t j) {
 namespace ptb {
 int filter_count(ptbgpu_ctx *ctx, double *dA, int64_t ndim, double filter, bool do_filter,
                  int64_t counts[2]);
+// metrics.hip: device address of the static scratch of the allocation-free
+// entry point, looked up once per device at context creation
+int noalloc_scratch_init(ptbgpu_ctx *ctx);
 }

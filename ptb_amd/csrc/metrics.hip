@@ -13,12 +13,26 @@
 //
 // All reductions are deterministic: fixed per-thread strides, wavefront then
 // block sums into per-block partials, and one ordered final sum.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace {
 
 constexpr int kThreads = 256;
 
+// Scratch of ptbgpu_idempotency_noalloc: a file-scope device array, so that
+// the call allocates nothing.  Every GPU the library runs on holds its own
+// copy from the first context on it until the process ends.  40 MiB covers
+// the idempotency check's row-sum partials up to ndim ~ 51000 (2048 waters:
+// 47104).  Host code reaches it only through the address hipGetSymbolAddress
+// returns for the device (kept per device below); the array's name itself is
+// not a device address on the host.
+constexpr size_t kScratchDoubles = (size_t)5 << 20;
+__device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
 
 // packed index k -> column j (largest j with j(j+1)/2 <= k), exact for k < 2^52
 __device__ __forceinline__ int64_t col_of(int64_t k) {
@@ -200,6 +214,20 @@ int idempotency(ptbgpu_ctx *ctx, const double *dP, const double *dS, int64_t ndi
     return PTBGPU_OK;
 }
 
+// The static scratch's device address on the context's device, looked up once
+// per device (the context's device is current when ptbgpu_init calls this).
+int noalloc_scratch_init(ptbgpu_ctx *ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices)
+        return fail(ctx, PTBGPU_E_ARG, "device %d beyond the scratch table", ctx->device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[ctx->device]) return PTBGPU_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(ctx, PTBGPU_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_dev[ctx->device] = static_cast<double *>(p);
+    return PTBGPU_OK;
+}
 
 // host arrays -> two named device buffers
 static int upload2(ptbgpu_ctx *ctx, int64_t ndim, const double *A, const double *B, double **dA,
@@ -259,5 +287,17 @@ int ptbgpu_idempotency(ptbgpu_ctx *ctx, int64_t ndim, const double *P, const dou
     return ptb::idempotency(ctx, dP, dS, ndim, deviation);
 }
 
+int ptbgpu_idempotency_noalloc(ptbgpu_ctx *ctx, int64_t ndim, const double *dP, const double *dS,
+                               double *d_deviation) {
+    PTB_TRY(metric_args(ctx, ndim, dP, dS, d_deviation));
+    double *scr = g_scratch_dev[ctx->device];
+    if (!scr) return ptb::fail(ctx, PTBGPU_E_ARG, "static scratch not set up on device %d", ctx->device);
+    const size_t need = ptb::idem_scratch_doubles(ctx, ndim);
+    if (need > kScratchDoubles)
+        return ptb::fail(ctx, PTBGPU_E_ARG,
+                         "ndim %lld needs %zu bytes of scratch, above the static %zu bytes",
+                         (long long)ndim, need * sizeof(double), kScratchDoubles * sizeof(double));
+    return ptb::idempotency_enqueue(ctx, dP, dS, ndim, scr, d_deviation);
+}
 
 }  // extern "C"
