@@ -70,9 +70,9 @@ def main(argv=None):
                 ref = P
             same = all(np.array_equal(a, b) for a, b in zip(ref, P))
             table[wl][str(cand)] = {"build_ms": round(best * 1e3, 3), "merge_ms": round(merge_ms, 3),
-                                    "merge_launches": launches, "density_bit_identical": same}
+                                    "merge_levels": launches, "density_bit_identical": same}
             print(f"{wl} dc_bucket_min {cand:5d}: build {best * 1e3:8.2f} ms, merge {merge_ms:7.2f} ms "
-                  f"in {launches} launches, identical {same}", file=sys.stderr, flush=True)
+                  f"over {launches} tree levels, identical {same}", file=sys.stderr, flush=True)
     # the floor with the least summed build time over the workloads (only
     # floors that leave every density bit-identical qualify)
     def total(c):

@@ -204,10 +204,12 @@ def test_h2o2048_full_size(engine):
     engine.filter_matrix_sym_dev(dS, n, 1e-5)
     engine.filter_matrix_sym_dev(dH, n, 1e-5)
     import time
+    print(f"\nh2o_2048: H, S filtered on the GPU (ndim {n})", flush=True)
     dP64 = torch.zeros_like(dH)
     t0 = time.perf_counter()
     r64 = engine.sm_stupid_simple_dev(n, mol.nel, dH, dS, dP64, mol.n, mol.xyz, mol.aoat, 2, 2)
     t64 = time.perf_counter() - t0
+    print(f"h2o_2048: FP64 build {t64:.2f} s, nsm {r64['nsm']}, max m {r64['max_m']}", flush=True)
     dPm = torch.zeros_like(dH)
     t0 = time.perf_counter()
     rm = engine.sm_stupid_simple_dev(n, mol.nel, dH, dS, dPm, mol.n, mol.xyz, mol.aoat, 2, 2,
