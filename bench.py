@@ -295,7 +295,8 @@ def run_batch(args):
            "data": "synthetic (deterministic geometries, seed 2; Gaussian/extended-Hueckel surrogate H,S)",
            "config": {"workload": "batch256 (128 water clusters of 4-40 H2O at filter 1e-4, 128 "
                                   "CnH2n+2 n=3-40 at filter 1e-5), atoms_columns, submatrix_sygv",
-                      "precision": main_prec},
+                      "precision": main_prec, "mixed_min_m": eng.get_tuning("mixed_min_m"),
+                      "dc_bucket_min": eng.get_tuning("dc_bucket_min")},
            "per_precision": res}
     if "fp64" in Ps and "mixed" in Ps:
         dp, de = 0.0, 0.0
@@ -315,6 +316,8 @@ def main():
     ap.add_argument("--config", default="h2o_512", choices=sorted(WORKLOADS) + ["batch256"])
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip the dense -check solve (solve2) of the 512-water box")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (calibrates HBM "
@@ -490,7 +493,9 @@ def main():
         "config": {"workload": f"{args.config} ({mol.name}, {mol.n} atoms, ndim {n}), filter {wl['filter']}, "
                                f"atoms_columns, submatrix_sygv",
                    "nsm": int(r["rep"]["nsm"]), "max_m": int(r["rep"]["max_m"]),
-                   "sum_m3": float(r["rep"]["sum_m3"]), "precision": main_prec},
+                   "sum_m3": float(r["rep"]["sum_m3"]), "precision": main_prec,
+                   "mixed_min_m": eng.get_tuning("mixed_min_m"),
+                   "dc_bucket_min": eng.get_tuning("dc_bucket_min")},
         "fp64_builds_per_s": round(1000.0 / results["fp64"]["ms_per_step"], 4) if "fp64" in results else None,
         "filter_ms_per_matrix": round(filt_ms / max(filt_n, 1), 4),
         "roofline": roof,
@@ -539,6 +544,23 @@ def main():
             out["value"] = None
             out["accuracy_gate"] = "failed: mixed build outside |dE_band|, max|dP| <= 1e-6"
             log(f"accuracy gate failed: dP_max {acc['dP_max']:.3e}, dE_band {acc['dE_band']:.3e}")
+    if args.config == "h2o_512" and not args.no_check:
+        # the -check reference density (solve2, source/pgtb.f90:1066-1170) of
+        # the whole box: one dense generalized eigenproblem of ndim 11776,
+        # from host arrays (U returned, as the reference keeps it).  On the
+        # unfiltered pencil: the filtered overlap of this box is indefinite.
+        try:
+            Hc, Sc = surrogate.build_hs(mol)
+            t0 = time.perf_counter()
+            Pc, ec, fc, Uc, fail = eng.solve2(1, n, mol.nel, 0, 300.0, Hc, Sc)
+            ms = (time.perf_counter() - t0) * 1000.0
+            out["check_solve2"] = {"ms": round(ms, 1), "ndim": n, "fail": bool(fail),
+                                   "tr_PS": round(eng.get_nel(n, Pc, Sc), 9), "nel": mol.nel,
+                                   "pencil": "unfiltered H, S of the same box"}
+            log(f"solve2 (-check) ndim {n}: {ms:.1f} ms, fail {fail}")
+            del Hc, Sc, Pc, Uc
+        except Exception as exc:  # reported, never fatal
+            out["check_solve2"] = {"error": str(exc)}
     if not args.no_cpu and not big:
         try:
             out["cpu_baseline"] = cpu_baseline(mol, Hf, Sf, wl["columns"], args.cpu_budget)

@@ -129,11 +129,14 @@ int ptbgpu_stream(ptbgpu_ctx *ctx, void **stream);
  * ptb_amd/tune.py).  "dc_bucket_min": the divide-and-conquer merge nodes of
  * one tree level are launched in power-of-two size classes, each with the LDS
  * of its class; classes below this floor join the smallest launched one, and
- * 0 launches a level at once (default 256).  "graph": 1 makes the
- * construction and eigendecomposition phase of a build (one chunk) replay a
- * captured hipGraph once the same shapes and buffers have been seen before
- * (the first such build runs eagerly, the second captures); 0 (default) turns
- * it off and drops the held graphs.  ptbgpu_get_tuning also reads "graphs",
+ * 0 launches a level at once (default 256).  "mixed_min_m": a mixed-precision
+ * build reduces the submatrices below this dimension in FP64 (no float
+ * storage, no refinement sweep) and the others on float storage with the FP64
+ * refinement; 0 puts every submatrix on float storage (default 288).
+ * "graph": 1 makes the construction and eigendecomposition phase of a build
+ * (one chunk) replay a captured hipGraph once the same shapes and buffers
+ * have been seen before (the first such build runs eagerly, the second
+ * captures); 0 (default) turns it off and drops the held graphs.  ptbgpu_get_tuning also reads "graphs",
  * the number of graphs held, and "graph_failures", the captures that failed
  * (those builds ran eagerly). */
 int ptbgpu_set_tuning(ptbgpu_ctx *ctx, const char *key, int64_t value);

@@ -41,6 +41,15 @@ void *ws(ptbgpu_ctx *ctx, const char *name, size_t bytes) {
     return b.ptr;
 }
 
+void ws_trim(ptbgpu_ctx *ctx, const char *name, size_t bytes) {
+    auto it = ctx->bufs.find(name);
+    if (it == ctx->bufs.end() || !it->second.ptr || it->second.bytes <= bytes) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(it->second.ptr);
+    ctx->bufs.erase(it);
+    ctx->shadow.erase(name);
+}
+
 void *upload_table(ptbgpu_ctx *ctx, const char *name_in, const void *src, size_t bytes) {
     // tables of a graph-held phase live under the graph's own prefix, so that
     // other builds do not overwrite what its replays read
@@ -158,6 +167,11 @@ int ptbgpu_set_tuning(ptbgpu_ctx *ctx, const char *key, int64_t value) {
         ctx->dc_bucket_min = value;
         return PTBGPU_OK;
     }
+    if (k == "mixed_min_m") {
+        if (value < 0 || value > 1 << 20) return ptb::fail(ctx, PTBGPU_E_ARG, "mixed_min_m out of range");
+        ctx->mixed_min_m = value;
+        return PTBGPU_OK;
+    }
     if (k == "graph") {
         ctx->graph_mode = value != 0;
         if (!ctx->graph_mode) {
@@ -174,6 +188,10 @@ int ptbgpu_get_tuning(ptbgpu_ctx *ctx, const char *key, int64_t *value) {
     const std::string k(key);
     if (k == "dc_bucket_min") {
         *value = ctx->dc_bucket_min;
+        return PTBGPU_OK;
+    }
+    if (k == "mixed_min_m") {
+        *value = ctx->mixed_min_m;
         return PTBGPU_OK;
     }
     if (k == "graph") {

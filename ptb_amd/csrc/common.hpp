@@ -42,6 +42,12 @@ struct ptbgpu_ctx {
     // tuning knobs (ptbgpu_set_tuning): the smallest size class of the
     // divide-and-conquer merge launches (0: one launch per tree level)
     int64_t dc_bucket_min = 256;
+    // mixed-precision builds reduce submatrices below this dimension in FP64
+    // (no float storage, no refinement sweep); 0: every submatrix in float.
+    // 288: the tuner's choice on MI355X (below it the float reduction saves
+    // less than the refinement costs, and small near-degenerate blocks leave
+    // the refined density outside 1e-6 of FP64)
+    int64_t mixed_min_m = 288;
     // "graph": the eigen phase of repeated one-chunk builds replays a captured
     // hipGraph; the cache of instantiated graphs (keyed by shapes and
     // buffers), the keys seen once, and whether a capture is in progress (no
@@ -53,6 +59,8 @@ struct ptbgpu_ctx {
     std::string graph_note;     // what the last eigen phase did (replayed, captured, why not)
     std::string table_prefix;   // prefix of table names while a graph-held phase runs
     int64_t graph_failures = 0;
+    // the chunk bounds, precision and dimensions of the last chunked build
+    std::vector<int64_t> chunk_plan;
 };
 
 namespace ptb {
@@ -62,6 +70,8 @@ int fail(ptbgpu_ctx *ctx, int status, const char *fmt, ...);
 
 // Named device workspace of at least `bytes` (contents undefined after growth).
 void *ws(ptbgpu_ctx *ctx, const char *name, size_t bytes);
+// releases the named buffer if it holds more than `bytes` (0: any)
+void ws_trim(ptbgpu_ctx *ctx, const char *name, size_t bytes);
 // Named device table of `bytes` filled from host memory: the copy is skipped
 // when the buffer still holds exactly these bytes from the previous call
 // (descriptor tables of repeated builds with the same structure).  Returns the

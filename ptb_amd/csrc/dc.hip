@@ -241,6 +241,12 @@ __device__ int block_scan_excl(int v, int *wsum, int *total) {
 //   rows below it:         W(n1:n, dense+right) U(dense+right, :)
 // (about half the flops of W(:, 0:K) U; the structure dlaed3 exploits).  The
 // kernel writes both descriptors, gd[2 node] and gd[2 node + 1].
+//
+// The per-node vectors (6 doubles and 4 ints per row, padded to the power of
+// two npow) live in LDS up to npow = 2048; larger nodes (kGlobal) keep them
+// in a device scratch of 8 npow doubles per node at gscr, which only this
+// workgroup touches (its barriers order the accesses as they do in LDS).
+template <bool kGlobal>
 __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ desc,
                                                   const DcNode *__restrict__ nodes, int npow,
                                                   const double *__restrict__ e_all,
@@ -248,8 +254,11 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
                                                   double *__restrict__ Za, double *__restrict__ Wa,
                                                   double *__restrict__ Xa,
                                                   double *__restrict__ rot_all,
-                                                  GemmDesc *__restrict__ gd) {
-    extern __shared__ double shm[];
+                                                  GemmDesc *__restrict__ gd, double *gscr) {
+    extern __shared__ double shm_[];
+    double *shm;
+    if constexpr (kGlobal) shm = gscr + (size_t)blockIdx.x * 8 * (size_t)npow;
+    else shm = shm_;
     const DcNode nd = nodes[blockIdx.x];
     const SubDesc ds = desc[nd.s];
     const int n = nd.hi - nd.lo, n1 = nd.sp - nd.lo, ld = ds.ld;
@@ -293,16 +302,21 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     }
     const double rs2 = 1.0 / sqrt(2.0);
     double *zraw = tau;  // temporary: raw z by column before sorting
+    // Keys: the eigenvalues, a NaN as +inf; padding +inf.  Ties go by index,
+    // so the n real entries always sort ahead of the padding and the sorted
+    // positions 0..n-1 name only columns of this node, whatever the data (a
+    // NaN key would otherwise leave the order undefined).
     for (int i = threadIdx.x; i < npow; i += kMT) {
         if (i < n) {
-            Dk[i] = D[nd.lo + i];
+            const double dv = D[nd.lo + i];
+            Dk[i] = dv == dv ? dv : HUGE_VAL;
             ix[i] = i;
             const int col = nd.lo + i;
             const double zv = (i < n1) ? Z[(int64_t)col * ld + nd.sp - 1]
                                        : sgn * Z[(int64_t)col * ld + nd.sp];
             zraw[i] = zv * rs2;
         } else {
-            Dk[i] = DBL_MAX;
+            Dk[i] = HUGE_VAL;
             ix[i] = i;
         }
     }
@@ -642,8 +656,9 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
         return p;
     };
     const int npow_max = pow2(max_n);
-    if ((size_t)npow_max * (6 * sizeof(double) + 4 * sizeof(int)) > 150 * 1024)
-        return fail(ctx, PTBGPU_E_ARG, "merge of %d rows exceeds the LDS budget", max_n);
+    // classes whose vectors exceed the LDS budget run from device scratch
+    auto in_lds = [](int np) { return (size_t)np * (6 * sizeof(double) + 4 * sizeof(int)) <= 150 * 1024; };
+    if (max_n > (1 << 24)) return fail(ctx, PTBGPU_E_ARG, "merge of %d rows is too large", max_n);
     // Size buckets: the nodes come largest first, and each run of nodes whose
     // power-of-two size class is the same is one launch with LDS for that
     // class, so that small nodes are not held to the largest node's LDS (and
@@ -665,9 +680,18 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
             const int i0 = runs[r].first;
             const int i1 = r + 1 < runs.size() ? runs[r + 1].first : count;
             const int np = runs[r].second;
-            const size_t shmem = (size_t)np * (6 * sizeof(double) + 4 * sizeof(int));
-            hipLaunchKernelGGL(k_dc_merge, dim3(i1 - i0), dim3(kMT), shmem, ctx->stream, ddesc,
-                               dnodes + i0, np, e, D, Z, W, X, rot, dgemm + 2 * i0);
+            if (in_lds(np)) {
+                const size_t shmem = (size_t)np * (6 * sizeof(double) + 4 * sizeof(int));
+                hipLaunchKernelGGL(k_dc_merge<false>, dim3(i1 - i0), dim3(kMT), shmem, ctx->stream,
+                                   ddesc, dnodes + i0, np, e, D, Z, W, X, rot, dgemm + 2 * i0,
+                                   nullptr);
+            } else {
+                auto *scr = static_cast<double *>(
+                    ws(ctx, "dc_gscr", sizeof(double) * 8 * (size_t)np * (size_t)(i1 - i0)));
+                if (!scr) return PTBGPU_E_NOMEM;
+                hipLaunchKernelGGL(k_dc_merge<true>, dim3(i1 - i0), dim3(kMT), 0, ctx->stream, ddesc,
+                                   dnodes + i0, np, e, D, Z, W, X, rot, dgemm + 2 * i0, scr);
+            }
             PTB_CHECK_LAUNCH(ctx);
         }
     }

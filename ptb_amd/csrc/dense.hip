@@ -369,6 +369,21 @@ __global__ __launch_bounds__(kT) void k_potrf_panel(const SubDesc *__restrict__ 
     if (threadIdx.x == 0 && bad) flags[blockIdx.x] |= 1;
 }
 
+// L = I (lower triangle) for the submatrices whose Cholesky met a
+// non-positive pivot: the factorisation continued past it and its values can
+// overflow, so the products downstream (whose results the caller discards for
+// these submatrices) get finite operands.  grid (submatrix, column)
+__global__ __launch_bounds__(kT) void k_identity_if_bad(const SubDesc *__restrict__ desc,
+                                                        const int *__restrict__ flags,
+                                                        double *__restrict__ Aa) {
+    if (!(flags[blockIdx.x] & 1)) return;
+    const SubDesc d = desc[blockIdx.x];
+    const int c = blockIdx.y;
+    if (c >= d.m) return;
+    double *col = Aa + d.off + (int64_t)c * d.ld;
+    for (int r = c + threadIdx.x; r < d.m; r += kT) col[r] = r == c ? 1.0 : 0.0;
+}
+
 // out = in^T (or 0.5 (in + in^T)); grid.x: 32x32 tiles, grid.y: submatrix
 __global__ __launch_bounds__(kT) void k_transpose(const SubDesc *__restrict__ desc,
                                                   const double *__restrict__ ina,
@@ -442,6 +457,19 @@ __global__ __launch_bounds__(kT) void k_to_f32(const SubDesc *__restrict__ desc,
     float *dst = Fa + d.off + (int64_t)c * d.ld;
     // the lower triangle only: the float tridiagonalisation never reads above it
     for (int r = c + threadIdx.x; r < d.m; r += kT) dst[r] = (float)src[r];
+}
+
+// FP64 copy of the lower triangle (the mixed solve's FP64-reduced
+// submatrices: K stays in A for the refinement, the reduction runs on the copy)
+__global__ __launch_bounds__(kT) void k_copy_lower(const SubDesc *__restrict__ desc,
+                                                   const double *__restrict__ Aa,
+                                                   double *__restrict__ Ba) {
+    const SubDesc d = desc[blockIdx.x];
+    const int c = blockIdx.y;
+    if (c >= d.m) return;
+    const double *src = Aa + d.off + (int64_t)c * d.ld;
+    double *dst = Ba + d.off + (int64_t)c * d.ld;
+    for (int r = c + threadIdx.x; r < d.m; r += kT) dst[r] = src[r];
 }
 
 // Reflectors of the FP32 tridiagonalisation in FP64: V(:, c) = the stored
@@ -1139,6 +1167,9 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
         }
         PTB_TRY(tb.launch(ctx, b, false, true, dgb + sb[b], "potrf"));
     }
+    // a failed factorisation (flags bit 0) leaves L = I
+    hipLaunchKernelGGL(k_identity_if_bad, dim3(nsm, max_m), dim3(kT), 0, ctx->stream, ddesc, flags, A);
+    PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
 
@@ -1171,6 +1202,14 @@ int to_f32_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, cons
                  float *F) {
     ProfScope prof_(ctx, "to_f32");
     hipLaunchKernelGGL(k_to_f32, dim3(nsm, max_m), dim3(kT), 0, ctx->stream, ddesc, A, F);
+    PTB_CHECK_LAUNCH(ctx);
+    return PTBGPU_OK;
+}
+
+int copy_lower_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *A,
+                     double *B) {
+    ProfScope prof_(ctx, "copy_lower");
+    hipLaunchKernelGGL(k_copy_lower, dim3(nsm, max_m), dim3(kT), 0, ctx->stream, ddesc, A, B);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }

@@ -6,7 +6,8 @@
 //    alpha and beta occupations (FERMISMEAR, occu: source/fermi.f:5-124) and
 //    P = U diag(focc) U^T.
 // They run the batched eigensolver of the submatrix build on a batch of one
-// (ndim <= 2048).  Smearing is O(ndim) scalar work on the host, as in the
+// (above 2048 rows its multi-workgroup tridiagonalisation and device-scratch
+// merges; ndim <= kDenseMax, so that n^2 stays below 2^31).  Smearing is O(ndim) scalar work on the host, as in the
 // reference.
 #include <algorithm>
 #include <cmath>
@@ -98,11 +99,13 @@ int arena_setup(ptbgpu_ctx *ctx, int64_t n, const char *pfx, Arena &a) {
     return PTBGPU_OK;
 }
 
-// ascending order of the eigenvalues (dsyevd / dsygvd order)
+// ascending order of the eigenvalues (dsyevd / dsygvd order); a NaN (from a
+// failed factorisation) sorts as +inf, so that the order is always defined
 std::vector<int> ascending(const std::vector<double> &w) {
     std::vector<int> p(w.size());
     std::iota(p.begin(), p.end(), 0);
-    std::stable_sort(p.begin(), p.end(), [&](int a, int b) { return w[a] < w[b]; });
+    auto key = [&](int i) { return std::isnan(w[i]) ? HUGE_VAL : w[i]; };
+    std::stable_sort(p.begin(), p.end(), [&](int a, int b) { return key(a) < key(b); });
     return p;
 }
 
@@ -184,7 +187,8 @@ namespace ptb {
 // eigenvectors into V (n x n, leading dimension n, device) in the same order.
 int dense_eig(ptbgpu_ctx *ctx, const double *M, int64_t n, double *V, std::vector<double> &w,
               const char *pfx) {
-    if (n < 1 || n > 2048) return fail(ctx, PTBGPU_E_ARG, "dense eigensolver needs 1 <= n <= 2048");
+    if (n < 1 || n > kDenseMax)
+        return fail(ctx, PTBGPU_E_ARG, "dense eigensolver needs 1 <= n <= %d", kDenseMax);
     Arena a;
     PTB_TRY(arena_setup(ctx, n, pfx, a));
     hipLaunchKernelGGL(k_load_upper, dim3(grid_of(n)), dim3(kT), 0, ctx->stream, M, n, a.A,
@@ -244,8 +248,8 @@ int spectral_function(ptbgpu_ctx *ctx, int64_t n, const double *dA, double *dOut
 
 int spectral_host(ptbgpu_ctx *ctx, int64_t n, const double *A, double *out, int which) {
     if (!ctx) return PTBGPU_E_ARG;
-    if (n < 1 || n > 2048 || !A || !out)
-        return ptb::fail(ctx, PTBGPU_E_ARG, "matrix_sign/matrix_root need 1 <= ndim <= 2048");
+    if (n < 1 || n > kDenseMax || !A || !out)
+        return ptb::fail(ctx, PTBGPU_E_ARG, "matrix_sign/matrix_root need 1 <= ndim <= %d", kDenseMax);
     PTB_HIP(ctx, hipSetDevice(ctx->device));
     const size_t mb = sizeof(double) * (size_t)n * (size_t)n;
     double *dA = static_cast<double *>(ptb::ws(ctx, "ms_in", mb));
@@ -280,8 +284,8 @@ int ptbgpu_solve2(ptbgpu_ctx *ctx, int64_t mode, int64_t ndim, int64_t nel, int6
                   double et, double shift0, double shift1, const double *H, const double *S,
                   double *P, double *e, double *focc, double *U, int64_t *fail) {
     if (!ctx) return PTBGPU_E_ARG;
-    if (ndim < 1 || ndim > 2048 || nel < 0 || !H || !S || !P)
-        return ptb::fail(ctx, PTBGPU_E_ARG, "solve2 needs 1 <= ndim <= 2048 and H, S, P");
+    if (ndim < 1 || ndim > kDenseMax || nel < 0 || !H || !S || !P)
+        return ptb::fail(ctx, PTBGPU_E_ARG, "solve2 needs 1 <= ndim <= %d and H, S, P", kDenseMax);
     PTB_HIP(ctx, hipSetDevice(ctx->device));
     ptb::ProfScope prof_(ctx, "solve2");
     const int64_t n = ndim;

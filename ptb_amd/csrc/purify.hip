@@ -10,7 +10,7 @@
 // uploaded once into a small device table, so an iteration launches GEMMs
 // back to back and synchronises only where a convergence test needs a norm.
 // The metric S^-1 or S^-1/2 is the eigen-route V f(w) V^T through the batched
-// symmetric eigensolver (ndim <= 2048) or the reference's Newton-Schulz
+// symmetric eigensolver (ndim <= kDenseMax) or the reference's Newton-Schulz
 // iterations.
 #include <cmath>
 #include <cstring>
@@ -239,11 +239,11 @@ int ptbgpu_purify_dev(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *ps, int64_t n,
     if (ps->type != MCWEENY && ps->metric == SQRT_)
         return ptb::fail(ctx, PTBGPU_E_MODE, "sign purification needs metric inverse or inverse_sqrt");
     const bool need_eig = !iterative || ps->type == SIGN_DIAG;
-    if (need_eig && n > 2048)
+    if (need_eig && n > kDenseMax)
         return ptb::fail(ctx, PTBGPU_E_ARG,
-                         "eigen-route purification needs ndim <= 2048 (got %lld); use the iterative "
+                         "eigen-route purification needs ndim <= %d (got %lld); use the iterative "
                          "metric with McWeeny, Pade or Newton-Schulz",
-                         (long long)n);
+                         kDenseMax, (long long)n);
     PTB_HIP(ctx, hipSetDevice(ctx->device));
     ptb::ProfScope prof_(ctx, "purify");
     double thigh, tnorm, tlow;

@@ -26,6 +26,13 @@
 
 namespace {
 
+// nocc = 0 for the submatrices a mixed build reduced in FP64
+__global__ __launch_bounds__(256) void k_skip_exact(const SubDesc *__restrict__ desc, int nsm,
+                                                    int *__restrict__ nocc) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s < nsm && (desc[s].flags & kSubExact)) nocc[s] = 0;
+}
+
 struct Timer {
     hipEvent_t ev[PTBGPU_T_NSLOT + 1];
     bool ok = false;
@@ -296,6 +303,9 @@ int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
     for (const auto &d : desc) max_m = std::max(max_m, d.m);
     PTB_TRY(ptb::occ_compact(ctx, ddesc, nsm, max_m, eig, bis, Z, W, wocc, nocc, oidx));  // W = Zo
     PTB_TRY(ptb::occ_compact(ctx, ddesc, nsm, max_m, eig, bis, Z, X, wocc, nocc, nullptr));
+    // FP64-reduced submatrices are not refined: no occupied columns to correct
+    hipLaunchKernelGGL(k_skip_exact, dim3((nsm + 255) / 256), dim3(256), 0, st, ddesc, nsm, nocc);
+    PTB_CHECK_LAUNCH(ctx);
     std::vector<int> hn(nsm);
     PTB_HIP(ctx, hipMemcpyAsync(hn.data(), nocc, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
     PTB_HIP(ctx, hipStreamSynchronize(st));
@@ -358,16 +368,28 @@ int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
     return ptb::refine_scatter(ctx, ddesc, nsm, max_n, nocc, oidx, W, thn, Z, eig);
 }
 
+// The buffers whose sizes follow the chunk plan: the per-submatrix arenas,
+// the owned-row projections, the tridiagonalisation's panel buffers and the
+// reflector T factors.
+const char *const kChunkBufs[] = {"sm_A",      "sm_B",       "sm_Z",  "sm_W",   "sm_X",
+                                  "sm_Li",     "sm_K32",     "sm_V",  "sm_R",   "sm_xa",
+                                  "sm_f",      "sytrd_pw",   "sytrd_pw32", "q_T", "q_T64",
+                                  "q_T128",    "q_t128_gm"};
+
 // Device bytes the per-submatrix arenas of one chunk may take: the free
-// device memory plus what this context already holds (its buffers are
-// reused), with a reserve for everything else.  PTBGPU_ARENA_BUDGET_MB, when
-// set, caps it (tests use it to force the chunked passes on small systems).
+// device memory plus what this context holds in the chunk-sized buffers
+// (released when a new chunked plan starts), with a reserve for everything
+// else.  PTBGPU_ARENA_BUDGET_MB, when set, caps it (tests use it to force the
+// chunked passes on small systems).
 size_t arena_budget(ptbgpu_ctx *ctx) {
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) freeb = (size_t)64 << 30;
     size_t held = 0;
-    for (const auto &kv : ctx->bufs) held += kv.second.bytes;
-    const size_t reserve = (size_t)4 << 30;
+    for (const char *nm : kChunkBufs) {
+        auto it = ctx->bufs.find(nm);
+        if (it != ctx->bufs.end()) held += it->second.bytes;
+    }
+    const size_t reserve = ((size_t)4 << 30) + (size_t)(0.03 * (double)total);
     size_t budget = freeb + held > reserve ? (size_t)(0.85 * (double)(freeb + held - reserve)) : 0;
     if (const char *env = std::getenv("PTBGPU_ARENA_BUDGET_MB")) {
         const long long mb = std::atoll(env);
@@ -546,6 +568,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     PTB_HIP(ctx, hipMemcpyAsync(mcount.data(), dcount, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
     PTB_HIP(ctx, hipStreamSynchronize(st));
     std::vector<SubDesc> desc(nsm);
+    const bool mixed = opts.precision != PTBGPU_FP64;
     std::vector<int64_t> rbeg(nmol + 1, 0);
     std::vector<int> mol_max_m(nmol, 0);
     int64_t arena = 0, rows = 0;
@@ -560,7 +583,8 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             d.mapoff = (int)rows;
             d.gbeg = gptr[s];
             d.gend = gptr[s + 1];
-            d.flags = 0;
+            // a mixed build reduces the submatrices below mixed_min_m in FP64
+            d.flags = mixed && d.m < ctx->mixed_min_m ? kSubExact : 0;
             d.mol = i;
             arena += (int64_t)d.ld * d.m;
             rows += d.m;
@@ -570,8 +594,8 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         }
     }
     rbeg[nmol] = rows;
-    if (max_m > 2048)
-        return ptb::fail(ctx, PTBGPU_E_ARG, "submatrix dimension %d exceeds 2048", max_m);
+    if (max_m > kDenseMax)
+        return ptb::fail(ctx, PTBGPU_E_ARG, "submatrix dimension %d exceeds %d", max_m, kDenseMax);
     if (rows > INT32_MAX)
         return ptb::fail(ctx, PTBGPU_E_ARG, "batch too large (%lld submatrix rows)", (long long)rows);
     SubDesc *ddesc;
@@ -587,9 +611,9 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     // chunk recomputed in every pass (the arithmetic is deterministic, so a
     // recomputed chunk is bit-identical): pass 1 gives every submatrix's
     // eigenvalues and fe, the bisection runs over all of them; for the mixed
-    // solve a second pass refines the states occupied at that mu and the
-    // bisection runs again; the last pass writes the density.
-    const bool mixed = opts.precision != PTBGPU_FP64;
+    // solve a second pass refines the states occupied at that mu, writes the
+    // density at that mu, and the bisection runs again; the last pass writes
+    // the density (skipped when the second bisection returns the same mu).
     // the BF16 matrix cores for the products whose rounding the FP64
     // refinement removes: split-BF16x3 by default, plain BF16 on request
     const int lowp = !mixed ? kLowpNone
@@ -597,9 +621,8 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     // bytes per arena element: A, B, Z, W, X, Li in FP64; the mixed solve's
     // float copy of K, FP64 reflectors and refinement residual
     const size_t per_elem = sizeof(double) * 6 + (mixed ? sizeof(float) + 2 * sizeof(double) : 0);
-    std::vector<int> cbeg{0};
-    {
-        const size_t budget = arena_budget(ctx);
+    auto plan_chunks = [&](size_t budget) {
+        std::vector<int> cb{0};
         size_t used = 0;
         for (int s = 0; s < nsm; ++s) {
             // the arenas, the owned-row projections, the tridiagonalisation's
@@ -608,12 +631,50 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
                              2 * sizeof(double) * (size_t)desc[s].ld * 2 * (desc[s].gend - desc[s].gbeg) +
                              sizeof(double) * (size_t)desc[s].ld * (96 + 32 + 64 + 128);
             if (used > 0 && used + c > budget) {
-                cbeg.push_back(s);
+                cb.push_back(s);
                 used = 0;
             }
             used += c;
         }
-        cbeg.push_back(nsm);
+        cb.push_back(nsm);
+        return cb;
+    };
+    size_t budget = arena_budget(ctx);
+    std::vector<int> cbeg = plan_chunks(budget);
+    for (int attempt = 0; cbeg.size() > 2; ++attempt) {
+        // Several chunks: the arenas then take most of the device, and the
+        // budget counts the chunk-sized buffers already held as free.  When
+        // the plan differs from the last chunked one (other chunk bounds or
+        // precision) they are all released first; a repeated build of the
+        // same plan reuses them.  The largest chunk's arenas are allocated
+        // here; when the device cannot hold them (memory the budget could not
+        // see), the budget shrinks by a fifth and the plan is made again.
+        std::vector<int64_t> plan(cbeg.begin(), cbeg.end());
+        plan.push_back(mixed ? 1 : 0);
+        for (const auto &d : desc) plan.push_back((int64_t)d.m);
+        if (plan != ctx->chunk_plan) {
+            for (const char *nm : kChunkBufs) ptb::ws_trim(ctx, nm, 0);
+            ctx->chunk_plan = plan;
+        }
+        int64_t elems = 1;
+        for (size_t k = 0; k + 1 < cbeg.size(); ++k) {
+            int64_t e = 0;
+            for (int s = cbeg[k]; s < cbeg[k + 1]; ++s) e += (int64_t)desc[s].ld * desc[s].m;
+            elems = std::max(elems, e);
+        }
+        const size_t dbl = sizeof(double) * (size_t)elems;
+        bool ok = true;
+        for (const char *nm : {"sm_A", "sm_B", "sm_Z", "sm_W", "sm_X", "sm_Li"})
+            ok = ok && ptb::ws(ctx, nm, dbl) != nullptr;
+        if (mixed)
+            ok = ok && ptb::ws(ctx, "sm_K32", sizeof(float) * (size_t)elems) != nullptr &&
+                 ptb::ws(ctx, "sm_V", dbl) != nullptr && ptb::ws(ctx, "sm_R", dbl) != nullptr;
+        if (ok) break;
+        if (attempt == 4) return PTBGPU_E_NOMEM;
+        for (const char *nm : kChunkBufs) ptb::ws_trim(ctx, nm, 0);
+        ctx->chunk_plan.clear();
+        budget = budget / 5 * 4;
+        cbeg = plan_chunks(budget);
     }
     const int nchunk = (int)cbeg.size() - 1;
     // per-row vectors are global (indexed by mapoff): d, e, tau, fe, rotations,
@@ -650,7 +711,13 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     auto bisect_all = [&]() -> int {
         return ptb::bisect_batch(ctx, dvec, fe, drbeg, dnelref, nmol, max_rows, bis);
     };
-    const int sweeps = !mixed || opts.refine_steps < 0 ? 0 : (int)std::max<int64_t>(1, opts.refine_steps);
+    // the refinement sweeps correct the float-reduced submatrices (none when
+    // every submatrix of a mixed build is below mixed_min_m)
+    bool any_float = false;
+    for (const auto &d : desc) any_float = any_float || !(d.flags & kSubExact);
+    const int sweeps = !mixed || !any_float || opts.refine_steps < 0
+                           ? 0
+                           : (int)std::max<int64_t>(1, opts.refine_steps);
 
     // The arenas and tables of one chunk: submatrices [c0, c1) at offsets from
     // the chunk's arena base.
@@ -790,7 +857,8 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             // the shapes name the graph's private tables; the shapes and
             // the addresses of every shared buffer key the graph itself
             mix(c.desc.data(), sizeof(SubDesc) * c.desc.size());
-            const int64_t knobs[] = {(int64_t)lowp, (int64_t)mixed, ctx->dc_bucket_min};
+            const int64_t knobs[] = {(int64_t)lowp, (int64_t)mixed, ctx->dc_bucket_min,
+                                     ctx->mixed_min_m};
             mix(knobs, sizeof(knobs));
             char pfx[32];
             snprintf(pfx, sizeof(pfx), "g%016llx:", (unsigned long long)key);
@@ -906,17 +974,31 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         tm.mark(4, st);
         tm.mark(5, st);
         PTB_TRY(bisect_all());
+        bool final_done = false;
         if (sweeps > 0) {
+            // The refinement pass also writes the density at the first mu.
+            // The density reads only mu (and the refined pairs, which the
+            // last pass would recompute bit for bit), so when the bisection
+            // over the refined eigenvalues returns the same mu for every
+            // molecule that density is final and the last pass is skipped.
             PTB_TRY(save_mu());
             for (int k = 0; k < nchunk; ++k) {
                 PTB_TRY(setup_chunk(k, ch));
                 PTB_TRY(eigen_chunk(ch, false));
                 PTB_TRY(refine_chunk(ch, bis1));
+                PTB_TRY(density_chunk(ch, k == 0));
             }
             PTB_TRY(bisect_all());
+            std::vector<double> m1((size_t)kBisStride * nmol), m2((size_t)kBisStride * nmol);
+            PTB_HIP(ctx, hipMemcpyAsync(m1.data(), bis1, sizeof(double) * m1.size(), hipMemcpyDeviceToHost, st));
+            PTB_HIP(ctx, hipMemcpyAsync(m2.data(), bis, sizeof(double) * m2.size(), hipMemcpyDeviceToHost, st));
+            PTB_HIP(ctx, hipStreamSynchronize(st));
+            final_done = true;
+            for (int i = 0; i < nmol; ++i)
+                final_done = final_done && m1[(size_t)i * kBisStride] == m2[(size_t)i * kBisStride];
         }
         tm.mark(6, st);
-        for (int k = 0; k < nchunk; ++k) {
+        for (int k = 0; k < nchunk && !final_done; ++k) {
             PTB_TRY(setup_chunk(k, ch));
             PTB_TRY(eigen_chunk(ch, false));
             if (sweeps > 0) PTB_TRY(refine_chunk(ch, bis1));
@@ -1000,10 +1082,38 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
         arena = std::max(arena, (size_t)(x.off + (int64_t)x.ld * x.m));
     }
     const size_t abytes = sizeof(double) * arena;
-    if (K32 && V) {  // float storage, FP64 reflectors into V, A untouched
-        PTB_TRY(to_f32_batch(ctx, ddesc, nsm, max_m, A, K32));
-        PTB_TRY(sytrd_batch_f32(ctx, desc, ddesc, K32, dvec, evec, tau));
-        PTB_TRY(promote_reflectors(ctx, ddesc, nsm, max_m, K32, V, tau));
+    if (K32 && V) {
+        // mixed precision, A (K) untouched: float storage and FP64
+        // reflectors promoted into V for the submatrices the refinement
+        // corrects; the kSubExact ones are reduced in FP64 on a copy of K in V
+        std::vector<SubDesc> lo, ex;
+        int lo_m = 0, ex_m = 0;
+        for (const auto &x : desc) {
+            if (x.flags & kSubExact) {
+                ex.push_back(x);
+                ex_m = std::max(ex_m, x.m);
+            } else {
+                lo.push_back(x);
+                lo_m = std::max(lo_m, x.m);
+            }
+        }
+        SubDesc *dlo = const_cast<SubDesc *>(ddesc), *dex = const_cast<SubDesc *>(ddesc);
+        if (!lo.empty() && !ex.empty()) {
+            PTB_TRY(upload(ctx, "sm_desc_lo", lo, &dlo));
+            PTB_TRY(upload(ctx, "sm_desc_ex", ex, &dex));
+        }
+        if (!lo.empty()) {
+            const int n = (int)lo.size();
+            PTB_TRY(to_f32_batch(ctx, dlo, n, lo_m, A, K32));
+            PTB_TRY(sytrd_batch_f32(ctx, lo, dlo, K32, dvec, evec, tau));
+            PTB_TRY(promote_reflectors(ctx, dlo, n, lo_m, K32, V, tau));
+        }
+        if (!ex.empty()) {
+            const int n = (int)ex.size();
+            PTB_TRY(copy_lower_batch(ctx, dex, n, ex_m, A, V));
+            PTB_TRY(sytrd_batch(ctx, ex, dex, V, dvec, evec, tau));
+            PTB_TRY(zero_upper_batch(ctx, dex, n, ex_m, V));
+        }
         PTB_TRY(larft_all(ctx, desc, ddesc, V, tau, qf));
     } else {
         PTB_TRY(sytrd_batch(ctx, desc, ddesc, A, dvec, evec, tau));
@@ -1026,11 +1136,14 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
         // levels add up.  Submatrices of up to 1024 rows keep the error within
         // what one FP64 refinement sweep removes; larger ones (the 64-water
         // box as one block, the largest of the 2048-water box) keep FP64
-        // merge products.  Per level: the nodes of large submatrices first,
+        // merge products, and so do the FP64-reduced ones (kSubExact), which
+        // no refinement follows.  Per level: the nodes of large submatrices first,
         // then the others, each group largest first (so that the level's last
         // workgroups are small); the two groups' products are two launches.
         auto &nodes = levels[lv];
-        auto big = [&](const DcNode &nd) { return lowp != kLowpNone && desc[nd.s].m > 1024; };
+        auto big = [&](const DcNode &nd) {
+            return lowp != kLowpNone && (desc[nd.s].m > 1024 || (desc[nd.s].flags & kSubExact));
+        };
         std::stable_sort(nodes.begin(), nodes.end(), [&](const DcNode &a, const DcNode &b) {
             if (big(a) != big(b)) return big(a);
             return a.hi - a.lo > b.hi - b.lo;

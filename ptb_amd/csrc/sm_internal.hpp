@@ -13,10 +13,15 @@ struct SubDesc {
     int mapoff;
     int gbeg;   // owned columns: gcols[gbeg..gend)
     int gend;
-    int flags;  // bit 0: overlap block not positive definite
+    int flags;  // bit 0: overlap block not positive definite; bit 1 (kSubExact):
+                // reduced in FP64 in a mixed-precision build (no refinement)
     int mol;    // molecule of a batch (its chemical potential: mu[mol * kBisStride])
     int pad;
 };
+constexpr int kSubExact = 2;
+// the largest dense matrix or submatrix the eigensolver takes (n^2 < 2^31:
+// 32-bit element offsets inside a matrix)
+constexpr int kDenseMax = 32768;
 
 // One molecule of a batch on the device: its packed matrices and where its
 // pattern bitmap, its submatrices' row unions and its submatrices begin.
@@ -183,6 +188,9 @@ int larft_all(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
 // float copy of the arena blocks (same offsets and leading dimensions)
 int to_f32_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *A,
                  float *F);
+// B = the lower triangle of A (FP64), per submatrix
+int copy_lower_batch(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const double *A,
+                     double *B);
 // FP64 reflectors (zero above the subdiagonal) and orthogonal tau from the
 // float output of sytrd_batch_f32
 int promote_reflectors(ptbgpu_ctx *ctx, const SubDesc *ddesc, int nsm, int max_m, const float *F,

@@ -414,6 +414,242 @@ __global__ void k_sytrd_finish(const SubDesc *__restrict__ desc, const T *__rest
     ee[m - 1] = 0.0;
 }
 
+
+// ---- Large matrices: the same panel scheme, one launch per step of each
+// column over the whole GPU (the batched kernel gives one workgroup to a
+// submatrix and holds its vectors in LDS, which bounds m and, for a single
+// large matrix, leaves all but one CU idle).  Per column k of a panel:
+//   k_big_col      x = column k with the pending terms applied (rows > k),
+//                  d_k, and per-workgroup parts of sigma = |x(k+2:)|^2;
+//   k_big_reflect  the reflector from the summed parts (every workgroup
+//                  forms the same scalars), v stored in A and the panel
+//                  buffer, and per-workgroup parts of W^T v and V^T v;
+//   k_big_symv     A22 v from the lower triangle in 64 x 64 tiles: tile
+//                  (R, S) adds its row sums to PR[S][rows of R] and, for its
+//                  elements strictly below the diagonal, its column sums to
+//                  PQ[R][columns of S];
+//   k_big_p        p = tau (A22 v - V (W^T v) - W (V^T v)) from the partials
+//                  in fixed order, and parts of p^T v;
+//   k_big_w        w = p - tau/2 (p^T v) v into the panel buffer.
+// Every sum is formed in a fixed order, so the result is reproducible.  The
+// trailing rank-2 kNB update after each panel is the batched GEMM.
+constexpr int kBR = 256;  // rows per workgroup of the row kernels
+// the largest m whose vectors fit the batched kernel's 160 KB of LDS
+constexpr int kSmallMaxM = (160 * 1024 / 8 - kWaves - 4 * kNB - kWaves * 8) / (1 + kWaves);
+// a lone matrix above this goes through the multi-workgroup path
+constexpr int kLoneBigM = 2048;
+constexpr int kBT = 64;   // tile edge of k_big_symv
+
+template <typename T>
+__global__ __launch_bounds__(kBR) void k_big_zero_w(T *__restrict__ W, int m, int ld, int kb) {
+    for (int64_t l = (int64_t)blockIdx.x * kBR + threadIdx.x; l < (int64_t)(kNB - kb) * m;
+         l += (int64_t)gridDim.x * kBR) {
+        const int jj = kb + (int)(l / m), i = (int)(l % m);
+        W[(int64_t)jj * ld + i] = T(0);
+        W[(int64_t)(kNB + jj) * ld + i] = T(0);
+        W[(int64_t)(2 * kNB + jj) * ld + i] = T(0);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBR) void k_big_col(const T *__restrict__ a, int m, int ld,
+                                                 const T *__restrict__ W, int k0, int jj,
+                                                 double *__restrict__ vs, double *__restrict__ psig,
+                                                 double *__restrict__ scal, double *__restrict__ dd) {
+    __shared__ double c1[kNB], c2[kNB], red[kWaves];
+    const int k = k0 + jj;
+    if ((int)threadIdx.x < jj) {
+        c1[threadIdx.x] = W[(int64_t)threadIdx.x * ld + k];
+        c2[threadIdx.x] = a[(int64_t)(k0 + threadIdx.x) * ld + k];
+    }
+    __syncthreads();
+    const int i = k + 1 + blockIdx.x * kBR + threadIdx.x;
+    double part = 0.0;
+    if (i < m) {
+        double x = a[(int64_t)k * ld + i];
+        for (int t = 0; t < jj; ++t)
+            x -= a[(int64_t)(k0 + t) * ld + i] * c1[t] + W[(int64_t)t * ld + i] * c2[t];
+        vs[i] = x;
+        if (i > k + 1) part = x * x;
+        else scal[0] = x;  // alpha, read by every workgroup of the next kernel
+    }
+    const double tot = block_reduce(part, red);
+    if (threadIdx.x == 0) psig[blockIdx.x] = tot;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double dkk = a[(int64_t)k * ld + k];
+        for (int t = 0; t < jj; ++t) dkk -= 2.0 * c2[t] * c1[t];
+        dd[k] = dkk;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBR) void k_big_reflect(T *__restrict__ a, int m, int ld,
+                                                     T *__restrict__ W, int k0, int jj,
+                                                     double *__restrict__ vs,
+                                                     const double *__restrict__ psig, int nblk,
+                                                     double *__restrict__ scal,
+                                                     double *__restrict__ ee,
+                                                     double *__restrict__ tt,
+                                                     double *__restrict__ pq) {
+    __shared__ double sig, wred[kWaves][2 * kNB];
+    const int k = k0 + jj;
+    const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += psig[b];
+        sig = s;
+    }
+    __syncthreads();
+    const double sigma = sig, alpha = scal[0];
+    double beta, tk, scale;
+    if (sigma == 0.0) {
+        beta = alpha;
+        tk = 0.0;
+        scale = 0.0;
+    } else {
+        beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+        tk = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+    }
+    const int i = k + 1 + blockIdx.x * kBR + threadIdx.x;
+    double v = 0.0;
+    if (i < m) {
+        v = (i == k + 1) ? 1.0 : vs[i] * scale;
+        vs[i] = v;
+        a[(int64_t)k * ld + i] = (T)v;
+        W[(int64_t)(kNB + jj) * ld + i] = (T)v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ee[k] = beta;
+        tt[k] = tk;
+        scal[1] = tk;
+    }
+    // parts of q1[t] = W(:, t)^T v and q2[t] = V(:, k0 + t)^T v over this
+    // workgroup's rows: sum t of a wavefront ends in the lanes with
+    // (lane >> 2) & 15 == t
+    double p1[kNB], p2[kNB];
+#pragma unroll
+    for (int t = 0; t < kNB; ++t) {
+        const bool on = i < m && t < jj;
+        p1[t] = on ? (double)W[(int64_t)t * ld + i] * v : 0.0;
+        p2[t] = on ? (double)a[(int64_t)(k0 + t) * ld + i] * v : 0.0;
+    }
+    const double s1 = butterfly16(p1, lane);
+    const double s2 = butterfly16(p2, lane);
+    if ((lane & 3) == 0) {
+        wred[wid][(lane >> 2) & 15] = s1;
+        wred[wid][kNB + ((lane >> 2) & 15)] = s2;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < 2 * kNB) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) s += wred[w][threadIdx.x];
+        pq[(int64_t)blockIdx.x * 2 * kNB + threadIdx.x] = s;
+    }
+}
+
+// tile t of the lower triangle of block rows/columns [R0, R0 + nb): (R, S),
+// R >= S, in row-major order of the pairs
+template <typename T>
+__global__ __launch_bounds__(kBR) void k_big_symv(const T *__restrict__ a, int m, int ld, int k,
+                                                  const double *__restrict__ vs, int R0,
+                                                  double *__restrict__ PR, double *__restrict__ PQ) {
+    __shared__ double ysh[kWaves][kBT];
+    const int t = blockIdx.x;
+    int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((r + 1) * (r + 2) / 2 <= t) ++r;
+    while (r * (r + 1) / 2 > t) --r;
+    const int R = R0 + r, S = R0 + (t - r * (r + 1) / 2);
+    const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
+    const int row = R * kBT + lane;
+    const int c0 = S * kBT + wid * kNB;
+    const unsigned uld = (unsigned)ld, ir = (unsigned)min(row, m - 1);
+    T raw[kNB];
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) raw[q] = a[(unsigned)min(c0 + q, m - 1) * uld + ir];
+    const bool rok = row < m && row > k;
+    const double vi = rok ? vs[row] : 0.0;
+    double y = 0.0, z[kNB];
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) {
+        const int c = c0 + q;
+        const bool on = rok && c > k && c <= row;
+        const double x = on ? (double)raw[q] : 0.0;
+        y += x * vs[min(max(c, k + 1), m - 1)];
+        z[q] = c < row ? x * vi : 0.0;
+    }
+    ysh[wid][lane] = y;
+    const double zs = butterfly16(z, lane);
+    const int cq = c0 + ((lane >> 2) & 15);
+    if ((lane & 3) == 0 && cq < m && cq > k) PQ[(int64_t)R * m + cq] = zs;
+    __syncthreads();
+    if (wid == 0 && rok)
+        PR[(int64_t)S * m + row] = ((ysh[0][lane] + ysh[1][lane]) + ysh[2][lane]) + ysh[3][lane];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBR) void k_big_p(const T *__restrict__ a, int m, int ld,
+                                               T *__restrict__ W, int k0, int jj,
+                                               const double *__restrict__ vs,
+                                               const double *__restrict__ PR,
+                                               const double *__restrict__ PQ, int R0, int nbr,
+                                               const double *__restrict__ pq, int nblk,
+                                               const double *__restrict__ scal,
+                                               double *__restrict__ ppv) {
+    __shared__ double q[2 * kNB], red[kWaves];
+    const int k = k0 + jj;
+    if ((int)threadIdx.x < 2 * kNB) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += pq[(int64_t)b * 2 * kNB + threadIdx.x];
+        q[threadIdx.x] = s;
+    }
+    __syncthreads();
+    const double tk = scal[1];
+    const int i = k + 1 + blockIdx.x * kBR + threadIdx.x;
+    double part = 0.0;
+    if (i < m) {
+        const int Ri = i / kBT;
+        double s = 0.0;
+        for (int S = R0; S <= Ri; ++S) s += PR[(int64_t)S * m + i];
+        for (int R = Ri; R < R0 + nbr; ++R) s += PQ[(int64_t)R * m + i];
+        double p = s;
+        for (int t = 0; t < jj; ++t)
+            p -= a[(int64_t)(k0 + t) * ld + i] * q[t] + W[(int64_t)t * ld + i] * q[kNB + t];
+        p *= tk;
+        W[(int64_t)jj * ld + i] = (T)p;
+        part = p * vs[i];
+    }
+    const double tot = block_reduce(part, red);
+    if (threadIdx.x == 0) ppv[blockIdx.x] = tot;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBR) void k_big_w(T *__restrict__ W, int m, int ld, int k0, int jj,
+                                               const double *__restrict__ vs,
+                                               const double *__restrict__ ppv, int nblk,
+                                               const double *__restrict__ scal) {
+    __shared__ double sh;
+    const int k = k0 + jj;
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += ppv[b];
+        sh = s;
+    }
+    __syncthreads();
+    const double acoef = -0.5 * scal[1] * sh;
+    for (int i = blockIdx.x * kBR + threadIdx.x; i < m; i += gridDim.x * kBR) {
+        if (i > k) {
+            const double w = (double)W[(int64_t)jj * ld + i] + acoef * vs[i];
+            W[(int64_t)jj * ld + i] = (T)w;
+            W[(int64_t)(2 * kNB + jj) * ld + i] = (T)w;
+        } else {
+            W[(int64_t)jj * ld + i] = T(0);
+        }
+    }
+}
+
+
 }  // namespace
 
 namespace ptb {
@@ -422,12 +658,103 @@ namespace ptb {
 // then one batched GEMM applies the rank-32 update A(s:,s:) -= [W V] [V W]^T to
 // the lower triangle.
 namespace {
+// One large submatrix (descriptor x, its device descriptor dx) by the kernels
+// above; output as the batched path's.
+template <typename T>
+int sytrd_big(ptbgpu_ctx *ctx, const SubDesc &x, const SubDesc *dx, T *A, double *d, double *e,
+              double *tau) {
+    constexpr bool f32 = sizeof(T) == 4;
+    ProfScope kprof_(ctx, "sytrd_big");
+    const int m = x.m, ld = x.ld;
+    T *a = A + x.off;
+    double *dd = d + x.mapoff, *ee = e + x.mapoff, *tt = tau + x.mapoff;
+    const int nbt = (m + kBT - 1) / kBT;         // 64-row blocks
+    const int gmax = (m + kBR - 1) / kBR;        // row workgroups at most
+    const size_t nsc = (size_t)m + 2 * (size_t)nbt * m + (size_t)gmax * (2 + 2 * kNB) + 4;
+    auto *sc = static_cast<double *>(ws(ctx, "sybig_scr", sizeof(double) * nsc));
+    auto *W = static_cast<T *>(ws(ctx, f32 ? "sybig_pw32" : "sybig_pw", sizeof(T) * (size_t)ld * 3 * kNB));
+    if (!sc || !W) return PTBGPU_E_NOMEM;
+    double *vs = sc, *PR = vs + m, *PQ = PR + (size_t)nbt * m, *psig = PQ + (size_t)nbt * m;
+    double *ppv = psig + gmax, *pq = ppv + gmax, *scal = pq + (size_t)gmax * 2 * kNB;
+    const int npanel = m > 2 ? (m - 2 + kNB - 1) / kNB : 0;
+    std::vector<GemmDesc> g;
+    for (int p = 0; p < npanel; ++p) {
+        const int k0 = p * kNB, kb = std::min(kNB, m - 2 - k0), s0 = k0 + kb, n = m - s0;
+        T *C = a + (int64_t)s0 * ld + s0;
+        g.push_back({reinterpret_cast<const double *>(W + s0),
+                     reinterpret_cast<const double *>(W + (int64_t)kNB * ld + s0),
+                     reinterpret_cast<double *>(C), n, n, 2 * kNB, ld, ld, ld, -1.0, 1.0, 1, 0});
+    }
+    GemmDesc *dg = nullptr;
+    TileList tl;
+    if (npanel > 0) {
+        dg = static_cast<GemmDesc *>(upload_table(ctx, f32 ? "sybig_g32" : "sybig_g", g.data(),
+                                                  sizeof(GemmDesc) * g.size()));
+        if (!dg) return PTBGPU_E_NOMEM;
+        for (int p = 0; p < npanel; ++p) tl.add(g.data() + p, 1);
+        PTB_TRY(tl.upload(ctx, f32 ? "sybig_w32" : "sybig_w"));
+    }
+    hipStream_t st = ctx->stream;
+    for (int p = 0; p < npanel; ++p) {
+        const int k0 = p * kNB, kb = std::min(kNB, m - 2 - k0);
+        if (kb < kNB) {
+            hipLaunchKernelGGL(k_big_zero_w<T>, dim3(64), dim3(kBR), 0, st, W, m, ld, kb);
+            PTB_CHECK_LAUNCH(ctx);
+        }
+        for (int jj = 0; jj < kb; ++jj) {
+            const int k = k0 + jj;
+            const int nrow = m - k - 1;
+            const int G = (nrow + kBR - 1) / kBR;
+            const int R0 = (k + 1) / kBT, nbr = nbt - R0;
+            const unsigned ntile = (unsigned)(nbr * (nbr + 1) / 2);
+            hipLaunchKernelGGL(k_big_col<T>, dim3(G), dim3(kBR), 0, st, a, m, ld, W, k0, jj, vs, psig,
+                               scal, dd);
+            hipLaunchKernelGGL(k_big_reflect<T>, dim3(G), dim3(kBR), 0, st, a, m, ld, W, k0, jj, vs,
+                               psig, G, scal, ee, tt, pq);
+            hipLaunchKernelGGL(k_big_symv<T>, dim3(ntile), dim3(kBR), 0, st, a, m, ld, k, vs, R0, PR,
+                               PQ);
+            hipLaunchKernelGGL(k_big_p<T>, dim3(G), dim3(kBR), 0, st, a, m, ld, W, k0, jj, vs, PR, PQ,
+                               R0, nbr, pq, G, scal, ppv);
+            hipLaunchKernelGGL(k_big_w<T>, dim3(std::min(G + 1, 256)), dim3(kBR), 0, st, W, m, ld, k0,
+                               jj, vs, ppv, G, scal);
+            PTB_CHECK_LAUNCH(ctx);
+        }
+        PTB_TRY(tl.launch(ctx, p, false, true, dg + p, "sytrd", f32));
+    }
+    hipLaunchKernelGGL(k_sytrd_finish<T>, dim3(1), dim3(256), 0, st, dx, A, d, e, tau, 1);
+    PTB_CHECK_LAUNCH(ctx);
+    return PTBGPU_OK;
+}
+
 template <typename T>
 int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc, T *A,
                double *d, double *e, double *tau) {
     constexpr bool f32 = sizeof(T) == 4;
     ProfScope prof_(ctx, "sytrd");
     const int nsm = (int)desc.size();
+    // Submatrices whose vectors do not fit the batched kernel's LDS, and a
+    // lone matrix above kLoneBigM (one workgroup would leave the GPU idle),
+    // take the multi-workgroup path; the rest stay batched.
+    {
+        std::vector<SubDesc> rest;
+        int nbig = 0;
+        for (int si = 0; si < nsm; ++si) {
+            const int m = desc[si].m;
+            if (m > kSmallMaxM || (nsm == 1 && m > kLoneBigM)) {
+                PTB_TRY(sytrd_big<T>(ctx, desc[si], ddesc + si, A, d, e, tau));
+                ++nbig;
+            } else {
+                rest.push_back(desc[si]);
+            }
+        }
+        if (nbig == nsm) return PTBGPU_OK;
+        if (nbig > 0) {
+            auto *drest = static_cast<SubDesc *>(upload_table(ctx, f32 ? "sytrd_rest_f" : "sytrd_rest",
+                                                              rest.data(), sizeof(SubDesc) * rest.size()));
+            if (!drest) return PTBGPU_E_NOMEM;
+            return sytrd_impl(ctx, rest, drest, A, d, e, tau);
+        }
+    }
     int max_m = 0;
     for (const auto &x : desc) max_m = std::max(max_m, x.m);
     const size_t shmem = sizeof(double) * ((1 + kWaves) * (size_t)max_m + kWaves + 4 * kNB + kWaves * 8);
@@ -440,12 +767,12 @@ int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc 
         pwtot += (int64_t)desc[si].ld * 3 * kNB;
     }
     auto *PW = static_cast<T *>(ws(ctx, f32 ? "sytrd_pw32" : "sytrd_pw", sizeof(T) * std::max<int64_t>(pwtot, 1)));
-    auto *dpwoff = static_cast<int64_t *>(upload_table(ctx, "sytrd_pwoff", pwoff.data(), sizeof(int64_t) * nsm));
+    auto *dpwoff = static_cast<int64_t *>(upload_table(ctx, f32 ? "sytrd_pwoff_f" : "sytrd_pwoff", pwoff.data(), sizeof(int64_t) * nsm));
     if (!PW || !dpwoff) return PTBGPU_E_NOMEM;
     std::vector<int> order(nsm);
     for (int si = 0; si < nsm; ++si) order[si] = si;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return desc[a].m > desc[b].m; });
-    auto *dorder = static_cast<int *>(upload_table(ctx, "sytrd_order", order.data(), sizeof(int) * nsm));
+    auto *dorder = static_cast<int *>(upload_table(ctx, f32 ? "sytrd_order_f" : "sytrd_order", order.data(), sizeof(int) * nsm));
     if (!dorder) return PTBGPU_E_NOMEM;
     std::vector<GemmDesc> g1;
     std::vector<int> start(npanel + 1, 0);

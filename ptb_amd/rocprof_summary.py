@@ -7,6 +7,13 @@
         (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch), each scaled by what the
         same counter reads for the plain MIB-MiB device copy in the same run.
 
+    python -m ptb_amd.rocprof_summary counters DIR [--top N] [--cus 256]
+        per-kernel sums and per-dispatch averages of every counter in
+        DIR/*counter_collection.csv (one rocprofv3 --pmc run), with the MFMA
+        figures derived from them where their counters are present:
+        MFMA utilisation = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x CUs x 4)
+        and matrix-core work = SQ_INSTS_VALU_MFMA_MOPS_<type> x 512 flop.
+
 Prints markdown on stdout; the traffic mode also prints one line
 `bytes_per_launch=<float>` that bench.py reads back from the committed summary.
 """
@@ -86,6 +93,33 @@ def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_mib: int) -> None:
     print(f"bytes_per_launch={total:.6e}")
 
 
+def counters(d: str, top: int, cus: int) -> None:
+    per = {}  # kernel -> counter -> {dispatch: value}
+    for r in csv.DictReader(open(_one(os.path.join(d, "*counter_collection.csv")))):
+        k = short_name(r["Kernel_Name"])
+        c = per.setdefault(k, {}).setdefault(r["Counter_Name"], {})
+        did = int(r["Dispatch_Id"])
+        c[did] = c.get(did, 0.0) + float(r["Counter_Value"])
+    names = sorted({c for k in per.values() for c in k})
+    order = sorted(per, key=lambda k: -sum(per[k].get("GRBM_GUI_ACTIVE", {0: 0}).values()))
+    print("| kernel | dispatches | " + " | ".join(f"{n} / dispatch" for n in names) +
+          " | MFMA util | MFMA Tflop (F64, BF16) / dispatch |")
+    print("|---|---|" + "---|" * len(names) + "---|---|")
+    for k in order[:top]:
+        cs = per[k]
+        nd = max(len(v) for v in cs.values())
+        avg = {n: (sum(cs[n].values()) / len(cs[n]) if n in cs else float("nan")) for n in names}
+        util = "—"
+        if "SQ_VALU_MFMA_BUSY_CYCLES" in cs and "GRBM_GUI_ACTIVE" in cs and avg["GRBM_GUI_ACTIVE"] > 0:
+            util = f"{avg['SQ_VALU_MFMA_BUSY_CYCLES'] / (avg['GRBM_GUI_ACTIVE'] * cus * 4):.3f}"
+        fl = []
+        for t in ("F64", "BF16"):
+            n = f"SQ_INSTS_VALU_MFMA_MOPS_{t}"
+            fl.append(f"{avg[n] * 512 / 1e12:.4f}" if n in cs else "—")
+        print(f"| `{k}` | {nd} | " + " | ".join(f"{avg[n]:.4g}" for n in names) +
+              f" | {util} | {', '.join(fl)} |")
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="mode", required=True)
@@ -97,9 +131,15 @@ def main(argv=None) -> None:
     t.add_argument("write_dir")
     t.add_argument("--kernel", required=True)
     t.add_argument("--copy-mib", type=int, required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("dir")
+    c.add_argument("--top", type=int, default=20)
+    c.add_argument("--cus", type=int, default=256)
     a = ap.parse_args(argv)
     if a.mode == "stats":
         stats(a.dir, a.top)
+    elif a.mode == "counters":
+        counters(a.dir, a.top, a.cus)
     else:
         traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_mib)
 

@@ -152,12 +152,19 @@ def test_precision_ladder(engine, name):
         P64, _ = _build(engine, mol, H, S)
     e64 = engine.band_structure(n, P64, H)
     rows = {}
-    for label, prec, sweeps in (("bf16x3", "mixed", 1), ("bf16", "mixed_bf16", 1),
-                                ("bf16+3", "mixed_bf16", 3), ("none", "mixed", -1)):
-        P, rep = _build(engine, mol, H, S, precision=prec, refine_steps=sweeps)
-        rows[label] = (float(np.abs(P - P64).max()), abs(engine.band_structure(n, P, H) - e64))
-        print(f"{name} {label:7s} sweeps {sweeps:2d}: max|dP| {rows[label][0]:.3e} "
-              f"|dE_band| {rows[label][1]:.3e} mu {rep['mu']}")
+    # every submatrix on float storage (mixed_min_m 0): the precision of the
+    # low-precision path itself, not of the size policy
+    saved = engine.get_tuning("mixed_min_m")
+    engine.set_tuning("mixed_min_m", 0)
+    try:
+        for label, prec, sweeps in (("bf16x3", "mixed", 1), ("bf16", "mixed_bf16", 1),
+                                    ("bf16+3", "mixed_bf16", 3), ("none", "mixed", -1)):
+            P, rep = _build(engine, mol, H, S, precision=prec, refine_steps=sweeps)
+            rows[label] = (float(np.abs(P - P64).max()), abs(engine.band_structure(n, P, H) - e64))
+            print(f"{name} {label:7s} sweeps {sweeps:2d}: max|dP| {rows[label][0]:.3e} "
+                  f"|dE_band| {rows[label][1]:.3e} mu {rep['mu']}")
+    finally:
+        engine.set_tuning("mixed_min_m", saved)
     assert rows["bf16x3"][0] <= 1e-6 and rows["bf16x3"][1] <= 1e-6
     assert rows["bf16"][0] > 100 * rows["bf16x3"][0]
     assert rows["none"][0] > rows["bf16x3"][0]    # the refinement does the work
@@ -192,7 +199,8 @@ def test_h2o2048_full_size(engine):
     At full size: every atom gives a submatrix, the FP64 and the mixed build
     converge to the same mu, the mixed density is within 1e-6 of the FP64 one
     (max per element) and 1e-6 Eh in band energy, and tr(P S) is the electron
-    count to the accuracy of the submatrix method (0.1 %)."""
+    count to the accuracy of the submatrix method (the reference's own
+    density misses it by 0.7 to 1.1 % on the smaller workloads)."""
     import torch
     from ptb_amd import geometry, surrogate
     mol = geometry.config_molecule("h2o_2048")
@@ -226,4 +234,7 @@ def test_h2o2048_full_size(engine):
     assert r64["nsm"] == mol.n
     assert rm["mu"] == r64["mu"] and rm["muiter"] == r64["muiter"]
     assert dp <= 1e-6 and abs(em - e64) <= 1e-6
-    assert abs(nel - mol.nel) <= 1e-3 * mol.nel
+    # the submatrix method's own electron-count error: the compiled
+    # reference's P gives tr(P S) = 508.36 of 512 (64 waters), 4064.57 of
+    # 4096 (512 waters) and 2428.27 of 2402 (C400H802), 0.7 to 1.1 %
+    assert abs(nel - mol.nel) <= 1.5e-2 * mol.nel

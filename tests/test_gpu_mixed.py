@@ -76,3 +76,42 @@ def test_mixed_matches_oracle(engine, kind, size, seed, cols):
     print(f"\n{kind}{size} cols={cols}: max|dP| {dP:.3e} |dE| {dE:.3e} |dN| {dN:.3e}")
     assert dP <= P_TOL and dE <= E_TOL and dN <= 1e-8
     assert rm["mu"] == res.mu and rm["muiter"] == res.muiter
+
+
+def test_mixed_below_threshold_is_the_fp64_build(engine):
+    """With mixed_min_m above every submatrix dimension a mixed build reduces
+    every submatrix in FP64 and runs no refinement: the same arithmetic as
+    the FP64 build, so the same density bit for bit."""
+    mol, H, S = small_system("water", (3, 3, 2), 3)
+    P64, r64 = _build(engine, mol, H, S, 2, precision="fp64")
+    saved = engine.get_tuning("mixed_min_m")
+    engine.set_tuning("mixed_min_m", 1 << 16)
+    try:
+        Pm, rm = _build(engine, mol, H, S, 2, precision="mixed")
+    finally:
+        engine.set_tuning("mixed_min_m", saved)
+    assert rm["mu"] == r64["mu"] and rm["muiter"] == r64["muiter"]
+    assert np.array_equal(Pm, P64)
+
+
+def test_mixed_threshold_split_matches_fp64(engine):
+    """64 H2O with mixed_min_m in the middle of the submatrix dimensions: the
+    smaller submatrices are reduced in FP64 without refinement, the larger
+    ones in float and refined; the density is within the bar of the FP64
+    one (the all-float build's errors are printed beside it)."""
+    mol, H, S = _h2o64()
+    P64, r64 = _build(engine, mol, H, S, 2, precision="fp64")
+    saved = engine.get_tuning("mixed_min_m")
+    try:
+        engine.set_tuning("mixed_min_m", 0)
+        Pf, _ = _build(engine, mol, H, S, 2, precision="mixed")
+        engine.set_tuning("mixed_min_m", 500)
+        Pm, rm = _build(engine, mol, H, S, 2, precision="mixed")
+    finally:
+        engine.set_tuning("mixed_min_m", saved)
+    dP, dE, dN = _errors(mol, H, S, Pm, P64)
+    fP, fE, _ = _errors(mol, H, S, Pf, P64)
+    print(f"\nsplit at m=500: max|dP| {dP:.3e} |dE| {dE:.3e}; all float: max|dP| {fP:.3e} |dE| {fE:.3e}")
+    assert rm["mu"] == r64["mu"]
+    assert dP <= P_TOL and dE <= E_TOL and dN <= 1e-8
+    assert np.array_equal(Pm == 0.0, P64 == 0.0)
