@@ -40,7 +40,7 @@ PHASES = ["potrf", "trsm", "trsm_h", "trsm_k", "trsm_e", "trsm_u", "sytrd", "app
           "gemm_trsm", "gemm_dc", "gemm_q", "gemm_fe", "gemm_refine", "gemm_other", "refine_all",
           "inv_lower", "gemm_linv", "gemm_kform", "gemm_u"]
 KERNELS = ["pattern", "group_union", "group_map", "gather", "potrf_panel", "diag_inv",
-           "transpose", "sytrd_panel", "zero_upper", "dc_leaf", "dc_merge", "gemm", "larft",
+           "transpose", "sytrd_panel", "sytrd_panel_f32", "zero_upper", "dc_leaf", "dc_merge", "gemm", "larft",
            "fe_rows", "fe_finish", "bisect", "occ_compact", "dmat_scatter", "to_f32", "promote",
            "refine"]
 
@@ -61,14 +61,19 @@ def sytrd_bytes(ms, elem=8):
     return tot
 
 
-def roofline_model(name, ms, ndim, mixed=False):
+def roofline_model(name, ms, ndim, mixed=False, thr=0):
     """(bound, algorithmic units of one build summed over the kernel's launches,
     unit); divided by the kernel's bracketed time of one build this is the
-    per-launch figure averaged over its launches."""
+    per-launch figure averaged over its launches.  A mixed build reduces the
+    submatrices below `thr` (mixed_min_m) with the FP64 panel kernel
+    (`sytrd_panel`) and the others with the float one (`sytrd_panel_f32`)."""
     m = np.asarray(ms, dtype=np.float64)
     npk = ndim * (ndim + 1) / 2.0
+    if name == "sytrd_panel_f32":
+        return "hbm", sytrd_bytes([x for x in ms if x >= thr], 4), "GB/s"
     if name == "sytrd_panel":
-        return "hbm", sytrd_bytes(ms, 4 if mixed else 8), "GB/s"
+        sel = [x for x in ms if not mixed or x < thr]
+        return ("hbm", sytrd_bytes(sel, 8), "GB/s") if sel else None
     if name == "gather":   # dense Hsm and Ssm written + packed reads counted once
         return "hbm", float(np.sum(16.0 * m * m)) + 16.0 * npk, "GB/s"
     if name == "pattern":
@@ -404,12 +409,13 @@ def main():
     main_prec = "mixed" if "mixed" in results else "fp64"
     r = results[main_prec]
     value = 1000.0 / r["ms_per_step"]
+    thr = eng.get_tuning("mixed_min_m")
     # roofline of the dominant kernel that has a model
     roof = None
     for name, (ms, cnt) in sorted(r["kernels"].items(), key=lambda kv: -kv[1][0]):
         if name not in KERNELS:
             continue
-        model = roofline_model(name, r["m"], n, mixed=main_prec == "mixed")
+        model = roofline_model(name, r["m"], n, mixed=main_prec == "mixed", thr=thr)
         if model is None:
             continue
         bound, units, unit = model
@@ -422,9 +428,11 @@ def main():
         else:
             achieved = units / (ms * 1e-3) / 1e12
             peak = FP64_PEAK_TFS
-        kname = "k_" + name
-        traffic = traffic_per_launch(kname, args.config, mixed=main_prec == "mixed")
-        roof = {"kernel": kname, "bound": bound, "achieved": round(achieved, 2), "peak": peak,
+        kname = "k_sytrd_panel" if name.startswith("sytrd_panel") else "k_" + name
+        f32 = name == "sytrd_panel_f32"
+        traffic = traffic_per_launch(kname, args.config, mixed=f32)
+        roof = {"kernel": kname + ("<float>" if f32 else "<double>" if name == "sytrd_panel" else ""),
+                "bound": bound, "achieved": round(achieved, 2), "peak": peak,
                 "unit": unit, "frac": round(achieved / peak, 4),
                 "traffic": None if traffic is None else round(traffic, 0),
                 "algorithmic_bytes_per_launch": round(units / max(cnt, 1), 0),
@@ -455,10 +463,13 @@ def main():
                               "algorithmic_tflop": round(fl / 1e12, 4), "ms": round(ms, 3),
                               "achieved": round(tf, 2), "peak": FP64_PEAK_TFS,
                               "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFS, 4)})
-        for name in ("sytrd_panel", "gather", "pattern"):
+        for name in ("sytrd_panel_f32", "sytrd_panel", "gather", "pattern"):
             if name in r["kernels"]:
                 ms, cnt = r["kernels"][name]
-                bound, units, unit = roofline_model(name, r["m"], n, mixed=main_prec == "mixed")
+                model = roofline_model(name, r["m"], n, mixed=main_prec == "mixed", thr=thr)
+                if model is None:
+                    continue
+                bound, units, unit = model
                 gbs = units / (ms * 1e-3) / 1e9
                 sites.append({"site": name, "bound": bound, "algorithmic_gb": round(units / 1e9, 4),
                               "ms": round(ms, 3), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,

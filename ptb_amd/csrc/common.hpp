@@ -59,8 +59,10 @@ struct ptbgpu_ctx {
     std::string graph_note;     // what the last eigen phase did (replayed, captured, why not)
     std::string table_prefix;   // prefix of table names while a graph-held phase runs
     int64_t graph_failures = 0;
-    // the chunk bounds, precision and dimensions of the last chunked build
-    std::vector<int64_t> chunk_plan;
+    // the last chunked build: its precision and submatrix dimensions (the
+    // key) and its chunk bounds
+    std::vector<int64_t> chunk_key;
+    std::vector<int> chunk_cbeg;
 };
 
 namespace ptb {

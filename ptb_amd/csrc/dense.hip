@@ -372,16 +372,17 @@ __global__ __launch_bounds__(kT) void k_potrf_panel(const SubDesc *__restrict__ 
 // L = I (lower triangle) for the submatrices whose Cholesky met a
 // non-positive pivot: the factorisation continued past it and its values can
 // overflow, so the products downstream (whose results the caller discards for
-// these submatrices) get finite operands.  grid (submatrix, column)
+// these submatrices) get finite operands.  One workgroup per submatrix (the
+// unflagged ones return at once).
 __global__ __launch_bounds__(kT) void k_identity_if_bad(const SubDesc *__restrict__ desc,
                                                         const int *__restrict__ flags,
                                                         double *__restrict__ Aa) {
     if (!(flags[blockIdx.x] & 1)) return;
     const SubDesc d = desc[blockIdx.x];
-    const int c = blockIdx.y;
-    if (c >= d.m) return;
-    double *col = Aa + d.off + (int64_t)c * d.ld;
-    for (int r = c + threadIdx.x; r < d.m; r += kT) col[r] = r == c ? 1.0 : 0.0;
+    for (int c = 0; c < d.m; ++c) {
+        double *col = Aa + d.off + (int64_t)c * d.ld;
+        for (int r = c + threadIdx.x; r < d.m; r += kT) col[r] = r == c ? 1.0 : 0.0;
+    }
 }
 
 // out = in^T (or 0.5 (in + in^T)); grid.x: 32x32 tiles, grid.y: submatrix
@@ -1168,7 +1169,7 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
         PTB_TRY(tb.launch(ctx, b, false, true, dgb + sb[b], "potrf"));
     }
     // a failed factorisation (flags bit 0) leaves L = I
-    hipLaunchKernelGGL(k_identity_if_bad, dim3(nsm, max_m), dim3(kT), 0, ctx->stream, ddesc, flags, A);
+    hipLaunchKernelGGL(k_identity_if_bad, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, flags, A);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }

@@ -639,8 +639,17 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         cb.push_back(nsm);
         return cb;
     };
-    size_t budget = arena_budget(ctx);
-    std::vector<int> cbeg = plan_chunks(budget);
+    // a build of the same submatrices and precision as the last chunked one
+    // keeps its plan (and the arenas it holds): the free memory the budget
+    // reads moves a little from build to build, and a new plan would
+    // reallocate every arena
+    const char *cap = std::getenv("PTBGPU_ARENA_BUDGET_MB");
+    std::vector<int64_t> pkey{mixed ? 1 : 0, cap ? (int64_t)std::atoll(cap) : -1};
+    for (const auto &d : desc) pkey.push_back((int64_t)d.m);
+    std::vector<int> cbeg;
+    if (!ctx->chunk_cbeg.empty() && pkey == ctx->chunk_key) cbeg = ctx->chunk_cbeg;
+    size_t budget = cbeg.empty() ? arena_budget(ctx) : 0;
+    if (cbeg.empty()) cbeg = plan_chunks(budget);
     for (int attempt = 0; cbeg.size() > 2; ++attempt) {
         // Several chunks: the arenas then take most of the device, and the
         // budget counts the chunk-sized buffers already held as free.  When
@@ -649,12 +658,10 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         // same plan reuses them.  The largest chunk's arenas are allocated
         // here; when the device cannot hold them (memory the budget could not
         // see), the budget shrinks by a fifth and the plan is made again.
-        std::vector<int64_t> plan(cbeg.begin(), cbeg.end());
-        plan.push_back(mixed ? 1 : 0);
-        for (const auto &d : desc) plan.push_back((int64_t)d.m);
-        if (plan != ctx->chunk_plan) {
+        if (pkey != ctx->chunk_key || cbeg != ctx->chunk_cbeg) {
             for (const char *nm : kChunkBufs) ptb::ws_trim(ctx, nm, 0);
-            ctx->chunk_plan = plan;
+            ctx->chunk_key = pkey;
+            ctx->chunk_cbeg = cbeg;
         }
         int64_t elems = 1;
         for (size_t k = 0; k + 1 < cbeg.size(); ++k) {
@@ -672,7 +679,9 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         if (ok) break;
         if (attempt == 4) return PTBGPU_E_NOMEM;
         for (const char *nm : kChunkBufs) ptb::ws_trim(ctx, nm, 0);
-        ctx->chunk_plan.clear();
+        ctx->chunk_key.clear();
+        ctx->chunk_cbeg.clear();
+        if (budget == 0) budget = arena_budget(ctx);
         budget = budget / 5 * 4;
         cbeg = plan_chunks(budget);
     }

@@ -804,7 +804,7 @@ int sytrd_impl(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc 
     PTB_TRY(tl.upload(ctx, f32 ? "sytrd_w1f" : "sytrd_w1"));
     for (int p = 0; p < npanel; ++p) {
         {
-            ProfScope kprof_(ctx, "sytrd_panel");
+            ProfScope kprof_(ctx, f32 ? "sytrd_panel_f32" : "sytrd_panel");
             hipLaunchKernelGGL(k_sytrd_panel<T>, dim3(nsm), dim3(kT), shmem, ctx->stream, ddesc, A, PW,
                                d, e, tau, dpwoff, dorder, p);
             PTB_CHECK_LAUNCH(ctx);

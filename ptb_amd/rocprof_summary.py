@@ -2,17 +2,22 @@
 
     python -m ptb_amd.rocprof_summary stats DIR [--top N]
         kernel table from DIR/*_kernel_stats.csv (rocprofv3 --kernel-trace --stats)
-    python -m ptb_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR --kernel NAME --copy-mib MIB
+    python -m ptb_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR --kernel NAME --copy-mib MIB [--first N]
         HBM bytes per dispatch of kernel NAME from two rocprofv3 --pmc runs
         (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch), each scaled by what the
-        same counter reads for the plain MIB-MiB device copy in the same run.
+        same counter reads for the plain MIB-MiB device copy in the same run;
+        --first N averages only the kernel's first N dispatches (one build's
+        launches when the run makes several kinds of builds).
 
-    python -m ptb_amd.rocprof_summary counters DIR [--top N] [--cus 256]
-        per-kernel sums and per-dispatch averages of every counter in
+    python -m ptb_amd.rocprof_summary counters DIR [--top N] [--cus 256] [--xcds 8]
+        per-kernel per-dispatch averages of every counter in
         DIR/*counter_collection.csv (one rocprofv3 --pmc run), with the MFMA
         figures derived from them where their counters are present:
-        MFMA utilisation = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x CUs x 4)
-        and matrix-core work = SQ_INSTS_VALU_MFMA_MOPS_<type> x 512 flop.
+        MFMA utilisation = SQ_VALU_MFMA_BUSY_CYCLES (SIMD cycles with the
+        matrix core busy, summed over the SIMDs) / (GRBM_GUI_ACTIVE / XCDs x
+        CUs x 4 SIMDs); GRBM_GUI_ACTIVE reads as the sum over the XCDs' clocks
+        (per XCD it matches the kernel's duration), and matrix-core work =
+        SQ_INSTS_VALU_MFMA_MOPS_<type> x 512 flop.
 
 Prints markdown on stdout; the traffic mode also prints one line
 `bytes_per_launch=<float>` that bench.py reads back from the committed summary.
@@ -68,7 +73,7 @@ def _counter(d: str, counter: str):
     return out
 
 
-def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_mib: int) -> None:
+def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_mib: int, first: int = 0) -> None:
     expect_kib = copy_mib * 1024.0
     scales = {}
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
@@ -77,7 +82,9 @@ def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_mib: int) -> None:
         if not copies:
             sys.exit(f"{counter}: no calibration copy of {copy_mib} MiB in {d}")
         calib = max(copies)
-        mine = [v for n, v in rows.values() if short_name(n) == kernel]
+        mine = [v for _, (n, v) in sorted(rows.items()) if short_name(n) == kernel]
+        if first > 0:
+            mine = mine[:first]
         if not mine:
             sys.exit(f"{counter}: kernel {kernel} not found in {d}")
         scales[counter] = (expect_kib / calib, calib, sum(mine) / len(mine), len(mine))
@@ -93,7 +100,7 @@ def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_mib: int) -> None:
     print(f"bytes_per_launch={total:.6e}")
 
 
-def counters(d: str, top: int, cus: int) -> None:
+def counters(d: str, top: int, cus: int, xcds: int) -> None:
     per = {}  # kernel -> counter -> {dispatch: value}
     for r in csv.DictReader(open(_one(os.path.join(d, "*counter_collection.csv")))):
         k = short_name(r["Kernel_Name"])
@@ -111,7 +118,7 @@ def counters(d: str, top: int, cus: int) -> None:
         avg = {n: (sum(cs[n].values()) / len(cs[n]) if n in cs else float("nan")) for n in names}
         util = "—"
         if "SQ_VALU_MFMA_BUSY_CYCLES" in cs and "GRBM_GUI_ACTIVE" in cs and avg["GRBM_GUI_ACTIVE"] > 0:
-            util = f"{avg['SQ_VALU_MFMA_BUSY_CYCLES'] / (avg['GRBM_GUI_ACTIVE'] * cus * 4):.3f}"
+            util = f"{avg['SQ_VALU_MFMA_BUSY_CYCLES'] / (avg['GRBM_GUI_ACTIVE'] / xcds * cus * 4):.3f}"
         fl = []
         for t in ("F64", "BF16"):
             n = f"SQ_INSTS_VALU_MFMA_MOPS_{t}"
@@ -131,17 +138,19 @@ def main(argv=None) -> None:
     t.add_argument("write_dir")
     t.add_argument("--kernel", required=True)
     t.add_argument("--copy-mib", type=int, required=True)
+    t.add_argument("--first", type=int, default=0)
     c = sub.add_parser("counters")
     c.add_argument("dir")
     c.add_argument("--top", type=int, default=20)
     c.add_argument("--cus", type=int, default=256)
+    c.add_argument("--xcds", type=int, default=8)
     a = ap.parse_args(argv)
     if a.mode == "stats":
         stats(a.dir, a.top)
     elif a.mode == "counters":
-        counters(a.dir, a.top, a.cus)
+        counters(a.dir, a.top, a.cus, a.xcds)
     else:
-        traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_mib)
+        traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_mib, a.first)
 
 
 if __name__ == "__main__":
