@@ -362,7 +362,10 @@ def main():
     eng.profile(True)
     cS = eng.filter_matrix_sym_dev(dS, n, wl["filter"])
     cH = eng.filter_matrix_sym_dev(dH, n, wl["filter"])
-    filt_ms, filt_n = eng.profile_query("filter")
+    filt_ms, filt_n = eng.profile_query("filter")          # the call, count readback included
+    fk_ms, fk_n = eng.profile_query("filter_kernel")      # the packed pass alone
+    if fk_n == 0:
+        fk_ms, fk_n = filt_ms, filt_n
     eng.profile(False)
     log(f"filter: S positive {cS}, H positive {cH}, {filt_ms / filt_n:.3f} ms per matrix")
     dP = torch.zeros_like(dH)
@@ -484,9 +487,9 @@ def main():
             sites.append({"site": "dmat_scatter", "bound": "hbm", "algorithmic_gb": round(units / 1e9, 4),
                           "ms": round(ms, 3), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)})
-        gbs = 16.0 * n * (n + 1) / 2.0 / (filt_ms / max(filt_n, 1) * 1e-3) / 1e9
+        gbs = 16.0 * n * (n + 1) / 2.0 / (fk_ms / max(fk_n, 1) * 1e-3) / 1e9
         sites.append({"site": "filter", "bound": "hbm", "algorithmic_gb": round(16.0 * n * (n + 1) / 2.0 / 1e9, 4),
-                      "ms": round(filt_ms / max(filt_n, 1), 3), "achieved": round(gbs, 1),
+                      "ms": round(fk_ms / max(fk_n, 1), 3), "achieved": round(gbs, 1),
                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)})
     out = {
         "metric": METRIC,
@@ -509,6 +512,7 @@ def main():
                    "dc_bucket_min": eng.get_tuning("dc_bucket_min")},
         "fp64_builds_per_s": round(1000.0 / results["fp64"]["ms_per_step"], 4) if "fp64" in results else None,
         "filter_ms_per_matrix": round(filt_ms / max(filt_n, 1), 4),
+        "filter_kernel_ms_per_matrix": round(fk_ms / max(fk_n, 1), 4),
         "roofline": roof,
         "roofline_sites": sites,
     }

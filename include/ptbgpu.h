@@ -342,19 +342,26 @@ typedef struct ptbgpu_pur_report {
  * packed H, S -> packed P.  Where the reference ends in ERROR STOP (a NaN in
  * the iterative metric or in the purified matrix, :179-181,491-492,580-581,
  * 641-643) the call returns PTBGPU_E_NAN.  The direct metric and sign
- * diagonalisation use the engine's symmetric eigensolver (ndim <= 2048; larger
+ * diagonalisation use the engine's symmetric eigensolver (ndim <= 32768; larger
  * returns PTBGPU_E_ARG); the iterative metric with McWeeny, Pade or
- * Newton-Schulz purification has no size limit.  rep may be NULL. */
+ * Newton-Schulz purification is limited only by memory (ndim <= 46340).
+ * rep may be NULL. */
 int ptbgpu_purify(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *pur, int64_t ndim, const double *H,
                   const double *S, double *P, ptbgpu_pur_report *rep);
 int ptbgpu_purify_dev(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *pur, int64_t ndim,
                       const double *dH, const double *dS, double *dP, ptbgpu_pur_report *rep);
 
+/* Replaces adjust_thresholds_dp(ndim, Hmat) (source/metrics.f90:82-113): the
+ * size-dependent (high, normal, low) thresholds the purification and the
+ * density checks use, into vals[0..2] (the reference's default-REAL(4)
+ * literals).  Host only: no context, no device work. */
+int ptbgpu_thresholds(int64_t ndim, double *vals);
+
 /* ---- dense spectral routines ----------------------------------------------
  * matrix_sign(ndim,A,S) and matrix_root(ndim,A,sqrtAinv) (source/sm.f90:410-457):
  * S = V sgn(e) V^T (sgn 0 for e == 0) and V e^-1/2 V^T from the eigenpairs of
  * the symmetric A (dense ndim x ndim, column-major; upper triangle read, as
- * dsyevd('U')).  ndim <= 2048. */
+ * dsyevd('U')).  ndim <= 32768. */
 int ptbgpu_matrix_sign(ptbgpu_ctx *ctx, int64_t ndim, const double *A, double *S);
 int ptbgpu_matrix_root(ptbgpu_ctx *ctx, int64_t ndim, const double *A, double *sqrtAinv);
 
@@ -372,7 +379,7 @@ int ptbgpu_fermismear(int64_t norbs, int64_t nel, double t, const double *eig, d
  * (source/fermi.f); P = U diag(focc) U^T packed.  e[ndim] (ascending),
  * focc[ndim], U[ndim*ndim] (column-major, columns in e's order) and fail
  * (1 when the overlap is not positive definite: dsygvd INFO != 0) may be NULL.
- * ndim <= 2048. */
+ * ndim <= 32768. */
 int ptbgpu_solve2(ptbgpu_ctx *ctx, int64_t mode, int64_t ndim, int64_t nel, int64_t nopen,
                   double et, double shift0, double shift1, const double *H, const double *S,
                   double *P, double *e, double *focc, double *U, int64_t *fail);

@@ -7,8 +7,8 @@ Every function cites the reference lines it restates (paths under the
 reference checkout).  Dense LAPACK calls go to scipy (OpenBLAS ILP64 in this
 image): dsygvd via ``eigh(..., driver="gvd")``, dsyevd via ``driver="evd"``.
 
-Parity pins (what checks each function; tests/test_oracle.py and
-tests/test_oracle_configs.py):
+Parity pins (what checks each function; tests/test_oracle.py,
+tests/test_oracle_configs.py and tests/test_thresholds.py):
   filter_matrix_sym, analyse_matrix_sym   elementwise definition + compiled
                                           reference (oracle/_ref) on fixtures
   lin / blowsym / packsym                 closed-form index map, round trip
@@ -22,10 +22,16 @@ tests/test_oracle_configs.py):
                                           full_matrix mode == dense projector
   matrix_sign / matrix_root               textbook closed forms (V f(L) V^T)
   metrics (get_nel, band_structure,
-    idempotent, thresholds)               compiled reference + closed forms
+    idempotent)                           compiled reference + closed forms
+  adjust_thresholds_dp                    compiled reference on both sides of
+                                          every tier boundary (ndim 299/300,
+                                          1999/2000, 14999/15000)
   purification (metric, guess, McWeeny,
-    sign Pade/Newton, sign-diag, search)  compiled reference on small pencils;
-                                          converged P == dense projector
+    sign Pade/Newton, sign-diag, search)  compiled reference on the 2-water
+                                          pencil and, with the search, on the
+                                          8- and 18-water pencils (ndim 184 and
+                                          414, thresholds tier 2); converged
+                                          P == dense projector
   fermismear / occu / solve2              compiled reference FERMISMEAR/occu;
                                           zero-temperature limit
   mlpop2                                  compiled reference mlpop14 + mlpop2 on

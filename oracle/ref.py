@@ -111,6 +111,13 @@ def band_structure(ndim, P, H):
                                     _p(np.ascontiguousarray(H)))
 
 
+def thresholds(ndim):
+    """(high, normal, low) of adjust_thresholds_dp (source/metrics.f90:82-113)."""
+    out = np.zeros(3)
+    lib().ref_thresholds(I64(ndim), _p(out))
+    return tuple(out)
+
+
 def check_density(ndim, P, S, nel) -> str:
     log = []
     with quiet(log):

@@ -71,12 +71,31 @@ contains
       r = band_structure(ndim, P, H)
    end function
 
+   ! adjust_thresholds_dp (source/metrics.f90:82-113) for ndim: thrs is set
+   ! only when unallocated, so it is released first (each call takes its own
+   ! size's tier)
+   subroutine ref_thresholds(ndim, vals) bind(C, name="ref_thresholds")
+      use metrics, only: adjust_thresholds, thrs
+      integer(c_int64_t), value :: ndim
+      real(c_double), intent(out) :: vals(3)
+      real(c_double), allocatable :: dummy(:)
+      allocate (dummy(ndim*(ndim+1)/2))
+      dummy = 0.0_c_double
+      if (allocated(thrs)) deallocate (thrs)
+      call adjust_thresholds(ndim, dummy)
+      vals(1) = thrs%high
+      vals(2) = thrs%normal
+      vals(3) = thrs%low
+      deallocate (dummy)
+   end subroutine
+
    ! check_density_packed (source/metrics.f90:144-179); thresholds set by
-   ! adjust_thresholds_dp (:82-113) on first use. Prints the reference lines.
+   ! adjust_thresholds_dp (:82-113) for this ndim. Prints the reference lines.
    subroutine ref_check_density(ndim, P, S, nel) bind(C, name="ref_check_density")
-      use metrics, only: check_density, adjust_thresholds
+      use metrics, only: check_density, adjust_thresholds, thrs
       integer(c_int64_t), value :: ndim, nel
       real(c_double), intent(in) :: P(ndim*(ndim+1)/2), S(ndim*(ndim+1)/2)
+      if (allocated(thrs)) deallocate (thrs)
       call adjust_thresholds(ndim, P)
       call check_density(ndim, P, S, nel)
       flush (output_unit)
@@ -89,7 +108,7 @@ contains
          bind(C, name="ref_purification")
       use purification_, only: purification
       use purification_settings, only: tPurificationSet
-      use metrics, only: adjust_thresholds
+      use metrics, only: adjust_thresholds, thrs
       integer(c_int64_t), value :: ndim, nel, ptype, metric, iterative, cycles, &
          metric_cycles, chempot_cycles, prlvl
       real(c_double), value :: chempot_guess, chempot_increment
@@ -106,6 +125,7 @@ contains
       pur%chempot%guess = chempot_guess
       pur%chempot%increment = chempot_increment
       pur%prlvl = prlvl
+      if (allocated(thrs)) deallocate (thrs)
       call adjust_thresholds(ndim, H)
       call purification(pur, ndim, H, S, P)
       flush (output_unit)

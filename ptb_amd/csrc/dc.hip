@@ -207,6 +207,117 @@ __device__ __forceinline__ void copy_column(double *__restrict__ dst,
     }
 }
 
+// Secular root j of a merge with K poles dl (ascending), weights w and
+// w2 = rho w^2: the root's origin pole org[j] and offset tau[j].
+__device__ void dc_root(int j, int K, const double *dl, const double *w2, const double *w,
+                        double rho, int *org, double *tau) {
+    int o;
+    double lo, hi;
+    if (j < K - 1) {
+        const double mid = 0.5 * (dl[j + 1] - dl[j]);
+        double fs, fds;
+        secular_sums(dl, w2, 0, K, dl[j], mid, fs, fds);
+        const double f = 1.0 + fs;
+        if (f >= 0.0) {
+            o = j;
+            lo = 0.0;
+            hi = mid;
+        } else {
+            o = j + 1;
+            lo = -mid;
+            hi = 0.0;
+        }
+    } else {
+        o = j;
+        double s2 = 0.0;
+        for (int i = 0; i < K; ++i) s2 += w[i] * w[i];
+        lo = 0.0;
+        hi = rho * s2;
+    }
+    // f(t) = 1 + rho sum w_i^2 / (delta_i - t) (delta_i = d_i - d_origin) is
+    // increasing between its poles.  Each step fits the two poles that
+    // bracket the root exactly, a + b1/(D1 - eta) + b2/(D2 - eta) matching
+    // f and the derivatives of the parts left (psi) and right (phi) of the
+    // root ("middle way" rational model, as LAPACK dlaed4), and takes the
+    // model's root when it lies inside the bracket, else a Newton step or
+    // the midpoint.  Every evaluation also shrinks the bracket; stop when
+    // |f| is at rounding level or the bracket reaches adjacent doubles.
+    const double dorg = dl[o];
+    const bool last = (j == K - 1);
+    const double da = dl[j] - dorg, db = last ? 0.0 : dl[j + 1] - dorg;
+    double t = 0.5 * (lo + hi);
+    for (int it = 0; it < 200; ++it) {
+        double psi, dpsi, phi, dphi;
+        secular_sums(dl, w2, 0, j + 1, dorg, t, psi, dpsi);
+        secular_sums(dl, w2, j + 1, K, dorg, t, phi, dphi);
+        const double f = 1.0 + psi + phi;
+        if (f == 0.0 || fabs(f) <= 8.0 * kEps * (double)K * (1.0 + fabs(psi) + fabs(phi))) {
+            lo = hi = t;
+            break;
+        }
+        if (f < 0.0) lo = t;
+        else hi = t;
+        const double d1 = da - t;  // distance to the left pole
+        double tn;
+        if (last) {
+            // one-pole model a + b1/(d1 - eta): eta = d1 + b1/a
+            const double b1 = d1 * d1 * dpsi;
+            const double a = f - b1 / d1;
+            tn = (a != 0.0) ? t + d1 + b1 / a : 0.5 * (lo + hi);
+        } else {
+            const double d2 = db - t;  // distance to the right pole
+            const double b1 = d1 * d1 * dpsi, b2 = d2 * d2 * dphi;
+            const double a = f - b1 / d1 - b2 / d2;
+            // a eta^2 - B eta + C = 0
+            const double B = a * (d1 + d2) + b1 + b2;
+            const double C = a * d1 * d2 + b1 * d2 + b2 * d1;
+            // the root between the poles is the small one, 2C / (B + sgn(B) sqrt),
+            // formed without cancellation; the other one is tried if it is not
+            const double sq = sqrt(fmax(B * B - 4.0 * a * C, 0.0));
+            const double den = B >= 0.0 ? B + sq : B - sq;
+            tn = (den != 0.0) ? t + 2.0 * C / den : t;
+            if (!(tn > lo && tn < hi) && a != 0.0) tn = t + den / (2.0 * a);
+        }
+        if (!(tn > lo && tn < hi)) {
+            tn = t - f / (dpsi + dphi);
+            if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+        }
+        if (tn == t || tn == lo || tn == hi) break;
+        t = tn;
+    }
+    org[j] = o;
+    tau[j] = (lo == hi) ? lo : t;
+}
+
+// Loewner vector entry i: rho zhat_i^2 = (lam_i - d_i) prod_{j != i} (lam_j - d_i)/(d_j - d_i)
+__device__ double dc_loewner(int i, int K, const double *dl, const int *org, const double *tau,
+                             const double *w, double rho) {
+    const double di = dl[i];
+    double prod = -((di - dl[org[i]]) - tau[i]);
+    for (int j = 0; j < K; ++j) {
+        if (j == i) continue;
+        const double dlt = (di - dl[org[j]]) - tau[j];  // d_i - lam_j
+        prod *= (-dlt) * rcp64(dl[j] - di);
+    }
+    return copysign(sqrt(fmax(prod, 0.0) / rho), w[i]);
+}
+
+// Merged eigenvector column j by one wavefront: U(:, j) = zhat ./ (d - lam_j)
+// normalised, written to xc at the rows prow; the eigenvalue lam_j to Dn[j].
+__device__ void dc_vector(int j, int lane, int K, const double *dl, const int *org,
+                          const double *tau, const double *zh, const int *prow, double *xc,
+                          double *Dn) {
+    const double dorg = dl[org[j]], tj = tau[j];
+    double nrm = 0.0;
+    for (int i = lane; i < K; i += PTB_WAVE) {
+        const double u = zh[i] * rcp64((dl[i] - dorg) - tj);
+        nrm += u * u;
+    }
+    const double ci = 1.0 / sqrt(wave_sum(nrm));
+    for (int i = lane; i < K; i += PTB_WAVE) xc[prow[i]] = ci * zh[i] * rcp64((dl[i] - dorg) - tj);
+    if (lane == 0) Dn[j] = dorg + tj;
+}
+
 // exclusive prefix sum of one int per thread over the workgroup; *total gets
 // the sum of all (wsum: kMT / PTB_WAVE ints of LDS)
 __device__ int block_scan_excl(int v, int *wsum, int *total) {
@@ -254,7 +365,8 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
                                                   double *__restrict__ Za, double *__restrict__ Wa,
                                                   double *__restrict__ Xa,
                                                   double *__restrict__ rot_all,
-                                                  GemmDesc *__restrict__ gd, double *gscr) {
+                                                  GemmDesc *__restrict__ gd, double *gscr,
+                                                  int *gK) {
     extern __shared__ double shm_[];
     double *shm;
     if constexpr (kGlobal) shm = gscr + (size_t)blockIdx.x * 8 * (size_t)npow;
@@ -287,6 +399,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     constexpr int kWaves = kMT / PTB_WAVE;
 
     if (rho == 0.0) {  // decoupled: children are already eigenpairs of T
+        if (kGlobal && threadIdx.x == 0) gK[blockIdx.x] = 0;
         if (threadIdx.x < 2) {
             g[threadIdx.x].M = 0;
             g[threadIdx.x].N = 0;
@@ -502,114 +615,6 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     __syncthreads();
     double *w2 = zh;
     for (int i = threadIdx.x; i < K; i += kMT) w2[i] = rho * w[i] * w[i];
-    __syncthreads();
-    // secular roots, one thread per root
-    for (int j = threadIdx.x; j < K; j += kMT) {
-        int o;
-        double lo, hi;
-        if (j < K - 1) {
-            const double mid = 0.5 * (dl[j + 1] - dl[j]);
-            double fs, fds;
-            secular_sums(dl, w2, 0, K, dl[j], mid, fs, fds);
-            const double f = 1.0 + fs;
-            if (f >= 0.0) {
-                o = j;
-                lo = 0.0;
-                hi = mid;
-            } else {
-                o = j + 1;
-                lo = -mid;
-                hi = 0.0;
-            }
-        } else {
-            o = j;
-            double s2 = 0.0;
-            for (int i = 0; i < K; ++i) s2 += w[i] * w[i];
-            lo = 0.0;
-            hi = rho * s2;
-        }
-        // f(t) = 1 + rho sum w_i^2 / (delta_i - t) (delta_i = d_i - d_origin) is
-        // increasing between its poles.  Each step fits the two poles that
-        // bracket the root exactly, a + b1/(D1 - eta) + b2/(D2 - eta) matching
-        // f and the derivatives of the parts left (psi) and right (phi) of the
-        // root ("middle way" rational model, as LAPACK dlaed4), and takes the
-        // model's root when it lies inside the bracket, else a Newton step or
-        // the midpoint.  Every evaluation also shrinks the bracket; stop when
-        // |f| is at rounding level or the bracket reaches adjacent doubles.
-        const double dorg = dl[o];
-        const bool last = (j == K - 1);
-        const double da = dl[j] - dorg, db = last ? 0.0 : dl[j + 1] - dorg;
-        double t = 0.5 * (lo + hi);
-        for (int it = 0; it < 200; ++it) {
-            double psi, dpsi, phi, dphi;
-            secular_sums(dl, w2, 0, j + 1, dorg, t, psi, dpsi);
-            secular_sums(dl, w2, j + 1, K, dorg, t, phi, dphi);
-            const double f = 1.0 + psi + phi;
-            if (f == 0.0 || fabs(f) <= 8.0 * kEps * (double)K * (1.0 + fabs(psi) + fabs(phi))) {
-                lo = hi = t;
-                break;
-            }
-            if (f < 0.0) lo = t;
-            else hi = t;
-            const double d1 = da - t;  // distance to the left pole
-            double tn;
-            if (last) {
-                // one-pole model a + b1/(d1 - eta): eta = d1 + b1/a
-                const double b1 = d1 * d1 * dpsi;
-                const double a = f - b1 / d1;
-                tn = (a != 0.0) ? t + d1 + b1 / a : 0.5 * (lo + hi);
-            } else {
-                const double d2 = db - t;  // distance to the right pole
-                const double b1 = d1 * d1 * dpsi, b2 = d2 * d2 * dphi;
-                const double a = f - b1 / d1 - b2 / d2;
-                // a eta^2 - B eta + C = 0
-                const double B = a * (d1 + d2) + b1 + b2;
-                const double C = a * d1 * d2 + b1 * d2 + b2 * d1;
-                // the root between the poles is the small one, 2C / (B + sgn(B) sqrt),
-                // formed without cancellation; the other one is tried if it is not
-                const double sq = sqrt(fmax(B * B - 4.0 * a * C, 0.0));
-                const double den = B >= 0.0 ? B + sq : B - sq;
-                tn = (den != 0.0) ? t + 2.0 * C / den : t;
-                if (!(tn > lo && tn < hi) && a != 0.0) tn = t + den / (2.0 * a);
-            }
-            if (!(tn > lo && tn < hi)) {
-                tn = t - f / (dpsi + dphi);
-                if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
-            }
-            if (tn == t || tn == lo || tn == hi) break;
-            t = tn;
-        }
-        org[j] = o;
-        tau[j] = (lo == hi) ? lo : t;
-    }
-    __syncthreads();
-    // Loewner vector: rho zhat_i^2 = (lam_i - d_i) prod_{j != i} (lam_j - d_i)/(d_j - d_i)
-    for (int i = threadIdx.x; i < K; i += kMT) {
-        const double di = dl[i];
-        double prod = -((di - dl[org[i]]) - tau[i]);
-        for (int j = 0; j < K; ++j) {
-            if (j == i) continue;
-            const double dlt = (di - dl[org[j]]) - tau[j];  // d_i - lam_j
-            prod *= (-dlt) * rcp64(dl[j] - di);
-        }
-        zh[i] = copysign(sqrt(fmax(prod, 0.0) / rho), w[i]);
-    }
-    __syncthreads();
-    // U(:, j) = zhat ./ (d - lam_j), normalised; eigenvalues lam_j.  One
-    // column per wavefront: its norm, then the column (lanes on rows).
-    for (int j = wid; j < K; j += kWaves) {
-        const double dorg = dl[org[j]], tj = tau[j];
-        double nrm = 0.0;
-        for (int i = lane; i < K; i += PTB_WAVE) {
-            const double u = zh[i] * rcp64((dl[i] - dorg) - tj);
-            nrm += u * u;
-        }
-        const double ci = 1.0 / sqrt(wave_sum(nrm));
-        double *xc = X + (int64_t)(nd.lo + j) * ld + nd.lo;
-        for (int i = lane; i < K; i += PTB_WAVE) xc[prow[i]] = ci * zh[i] * rcp64((dl[i] - dorg) - tj);
-        if (lane == 0) D[nd.lo + j] = dorg + tj;
-    }
-    __syncthreads();
     if (threadIdx.x < 2) {
         // 0: rows above the split from slots [0, KL + KD); 1: rows below it
         // from slots [KL, K)
@@ -630,6 +635,88 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         gg.uplo = 0;
         gg.tri = 0;
     }
+    // a node from device scratch leaves the O(K^2) phases to three kernels
+    // over the whole chip (k_dc_roots_big, k_dc_loewner_big, k_dc_vectors_big)
+    if constexpr (kGlobal) {
+        if (threadIdx.x == 0) gK[blockIdx.x] = K;
+        return;
+    }
+    __syncthreads();
+    // secular roots, one thread per root
+    for (int j = threadIdx.x; j < K; j += kMT) dc_root(j, K, dl, w2, w, rho, org, tau);
+    __syncthreads();
+    // Loewner vector: rho zhat_i^2 = (lam_i - d_i) prod_{j != i} (lam_j - d_i)/(d_j - d_i)
+    for (int i = threadIdx.x; i < K; i += kMT) zh[i] = dc_loewner(i, K, dl, org, tau, w, rho);
+    __syncthreads();
+    // U(:, j) = zhat ./ (d - lam_j), normalised; eigenvalues lam_j.  One
+    // column per wavefront: its norm, then the column (lanes on rows).
+    for (int j = wid; j < K; j += kWaves)
+        dc_vector(j, lane, K, dl, org, tau, zh, prow, X + (int64_t)(nd.lo + j) * ld + nd.lo, D + nd.lo);
+    __syncthreads();
+}
+
+
+// The O(K^2) phases of the merges run from device scratch, over the whole
+// chip: grid (blocks, node of the run); the node's vectors at gscr + 8 npow
+// node, laid out as in k_dc_merge, and its K in gK.
+struct BigNode {
+    const double *dl, *w, *tau;
+    double *zh;
+    int *org, *prow;
+    double rho;
+    int K;
+};
+
+__device__ BigNode big_node(const SubDesc *desc, const DcNode *nodes, int npow,
+                            const double *e_all, double *gscr, const int *gK, int node) {
+    const DcNode nd = nodes[node];
+    const SubDesc ds = desc[nd.s];
+    double *b = gscr + (size_t)node * 8 * (size_t)npow;
+    BigNode g;
+    g.dl = b + 2 * (size_t)npow;
+    g.w = b + 3 * (size_t)npow;
+    g.tau = b + 4 * (size_t)npow;
+    g.zh = b + 5 * (size_t)npow;
+    int *ib = reinterpret_cast<int *>(b + 6 * (size_t)npow);
+    g.org = ib + 2 * (size_t)npow;
+    g.prow = ib + 3 * (size_t)npow;
+    g.rho = 2.0 * fabs(e_all[ds.mapoff + nd.sp - 1]);
+    g.K = gK[node];
+    return g;
+}
+
+__global__ __launch_bounds__(kMT) void k_dc_roots_big(const SubDesc *__restrict__ desc,
+                                                      const DcNode *__restrict__ nodes, int npow,
+                                                      const double *__restrict__ e_all,
+                                                      double *gscr, const int *__restrict__ gK) {
+    BigNode g = big_node(desc, nodes, npow, e_all, gscr, gK, blockIdx.y);
+    const int j = blockIdx.x * kMT + threadIdx.x;
+    if (j < g.K) dc_root(j, g.K, g.dl, g.zh, g.w, g.rho, g.org, const_cast<double *>(g.tau));
+}
+
+__global__ __launch_bounds__(kMT) void k_dc_loewner_big(const SubDesc *__restrict__ desc,
+                                                        const DcNode *__restrict__ nodes, int npow,
+                                                        const double *__restrict__ e_all,
+                                                        double *gscr, const int *__restrict__ gK) {
+    BigNode g = big_node(desc, nodes, npow, e_all, gscr, gK, blockIdx.y);
+    const int i = blockIdx.x * kMT + threadIdx.x;
+    if (i < g.K) g.zh[i] = dc_loewner(i, g.K, g.dl, g.org, g.tau, g.w, g.rho);
+}
+
+__global__ __launch_bounds__(kMT) void k_dc_vectors_big(const SubDesc *__restrict__ desc,
+                                                        const DcNode *__restrict__ nodes, int npow,
+                                                        const double *__restrict__ e_all,
+                                                        double *__restrict__ D_all,
+                                                        double *__restrict__ Xa, double *gscr,
+                                                        const int *__restrict__ gK) {
+    BigNode g = big_node(desc, nodes, npow, e_all, gscr, gK, blockIdx.y);
+    const DcNode nd = nodes[blockIdx.y];
+    const SubDesc ds = desc[nd.s];
+    const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
+    const int j = blockIdx.x * (kMT / PTB_WAVE) + wid;
+    if (j < g.K)
+        dc_vector(j, lane, g.K, g.dl, g.org, g.tau, g.zh, g.prow,
+                  Xa + ds.off + (int64_t)(nd.lo + j) * ds.ld + nd.lo, D_all + ds.mapoff + nd.lo);
 }
 
 }  // namespace
@@ -684,13 +771,22 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
                 const size_t shmem = (size_t)np * (6 * sizeof(double) + 4 * sizeof(int));
                 hipLaunchKernelGGL(k_dc_merge<false>, dim3(i1 - i0), dim3(kMT), shmem, ctx->stream,
                                    ddesc, dnodes + i0, np, e, D, Z, W, X, rot, dgemm + 2 * i0,
-                                   nullptr);
+                                   nullptr, nullptr);
             } else {
+                const int nn = i1 - i0;
                 auto *scr = static_cast<double *>(
-                    ws(ctx, "dc_gscr", sizeof(double) * 8 * (size_t)np * (size_t)(i1 - i0)));
-                if (!scr) return PTBGPU_E_NOMEM;
-                hipLaunchKernelGGL(k_dc_merge<true>, dim3(i1 - i0), dim3(kMT), 0, ctx->stream, ddesc,
-                                   dnodes + i0, np, e, D, Z, W, X, rot, dgemm + 2 * i0, scr);
+                    ws(ctx, "dc_gscr", sizeof(double) * 8 * (size_t)np * (size_t)nn));
+                auto *gK = static_cast<int *>(ws(ctx, "dc_gK", sizeof(int) * (size_t)nn));
+                if (!scr || !gK) return PTBGPU_E_NOMEM;
+                hipLaunchKernelGGL(k_dc_merge<true>, dim3(nn), dim3(kMT), 0, ctx->stream, ddesc,
+                                   dnodes + i0, np, e, D, Z, W, X, rot, dgemm + 2 * i0, scr, gK);
+                const unsigned nb = (unsigned)((np + kMT - 1) / kMT);
+                hipLaunchKernelGGL(k_dc_roots_big, dim3(nb, nn), dim3(kMT), 0, ctx->stream, ddesc,
+                                   dnodes + i0, np, e, scr, gK);
+                hipLaunchKernelGGL(k_dc_loewner_big, dim3(nb, nn), dim3(kMT), 0, ctx->stream, ddesc,
+                                   dnodes + i0, np, e, scr, gK);
+                hipLaunchKernelGGL(k_dc_vectors_big, dim3((unsigned)((np + 3) / 4), nn), dim3(kMT), 0,
+                                   ctx->stream, ddesc, dnodes + i0, np, e, D, X, scr, gK);
             }
             PTB_CHECK_LAUNCH(ctx);
         }

@@ -71,10 +71,20 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
         n0 = (w.t >> 16) * 64;
     } else {
         const int64_t lin = blockIdx.x;
-        const int64_t slot = lin >> 3;
-        gi = (int)((slot / tiles) * 8 + (lin & 7));
-        if (gi >= count) return;
-        const int tile = (int)(slot % tiles);
+        int tile;
+        if (tiles < 0) {
+            // fewer products than XCDs: consecutive workgroups (dealt
+            // round-robin to the XCDs) take consecutive tiles of a product,
+            // so that one large product runs on the whole chip
+            gi = (int)(lin / -tiles);
+            if (gi >= count) return;
+            tile = (int)(lin % -tiles);
+        } else {
+            const int64_t slot = lin >> 3;
+            gi = (int)((slot / tiles) * 8 + (lin & 7));
+            if (gi >= count) return;
+            tile = (int)(slot % tiles);
+        }
         const int tM = (descs[gi].M + 63) / 64, tN = (descs[gi].N + 63) / 64;
         if (tile >= tM * tN) return;
         m0 = (tile % max(tM, 1)) * 64;
@@ -301,9 +311,15 @@ __global__ __launch_bounds__(kT, 4) void k_gemm_nt1(const GemmDesc *__restrict__
 // the diagonal block in LDS, invert it, and form the panel below as
 // X = P L_kk^-T (rows independent).  The trailing update that follows is a
 // batched GEMM.  A non-positive pivot sets flags bit 0 (dsygvd: INFO > 0).
+//
+// mode 0: the whole panel (grid (submatrix, 1)).  For few submatrices the
+// panel is split: mode 1 forms the rows below the diagonal block, sliced over
+// grid.y (each workgroup factors the block in LDS for itself), then mode 2
+// writes the factored diagonal block and the flag (the rows read the block
+// unfactored, so they go first).
 __global__ __launch_bounds__(kT) void k_potrf_panel(const SubDesc *__restrict__ desc,
                                                     double *__restrict__ Aa,
-                                                    int *__restrict__ flags, int panel) {
+                                                    int *__restrict__ flags, int panel, int mode) {
     const SubDesc d = desc[blockIdx.x];
     const int m = d.m, ld = d.ld;
     const int k0 = panel * 32;
@@ -340,9 +356,14 @@ __global__ __launch_bounds__(kT) void k_potrf_panel(const SubDesc *__restrict__ 
         }
         __syncthreads();
     }
-    for (int l = threadIdx.x; l < 1024; l += kT) {
-        const int r = l & 31, c = l >> 5;
-        if (r < kb && c < kb && r >= c) a[(int64_t)(k0 + c) * ld + k0 + r] = Lk[r][c];
+    if (mode != 1)
+        for (int l = threadIdx.x; l < 1024; l += kT) {
+            const int r = l & 31, c = l >> 5;
+            if (r < kb && c < kb && r >= c) a[(int64_t)(k0 + c) * ld + k0 + r] = Lk[r][c];
+        }
+    if (mode == 2) {
+        if (threadIdx.x == 0 && bad) flags[blockIdx.x] |= 1;
+        return;
     }
     // Li = Lk^-1, column c by thread c (forward substitution)
     if (threadIdx.x < 32) {
@@ -355,7 +376,7 @@ __global__ __launch_bounds__(kT) void k_potrf_panel(const SubDesc *__restrict__ 
     }
     __syncthreads();
     // X(i, c) = sum_{t<=c} P(i, t) Li(c, t)
-    for (int i = k0 + kb + threadIdx.x; i < m; i += kT) {
+    for (int i = k0 + kb + blockIdx.y * kT + threadIdx.x; i < m; i += gridDim.y * kT) {
         double p[32];
 #pragma unroll
         for (int t = 0; t < 32; ++t) p[t] = t < kb ? a[(int64_t)(k0 + t) * ld + i] : 0.0;
@@ -366,7 +387,7 @@ __global__ __launch_bounds__(kT) void k_potrf_panel(const SubDesc *__restrict__ 
             a[(int64_t)(k0 + c) * ld + i] = sacc;
         }
     }
-    if (threadIdx.x == 0 && bad) flags[blockIdx.x] |= 1;
+    if (mode == 0 && threadIdx.x == 0 && bad) flags[blockIdx.x] |= 1;
 }
 
 // L = I (lower triangle) for the submatrices whose Cholesky met a
@@ -873,11 +894,15 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
     ProfScope sprof_(ctx, site ? (std::string("gemm_") + site).c_str() : "gemm_other");
     if (count <= 0 || max_m <= 0 || max_n <= 0) return PTBGPU_OK;
     const int tiles = ((max_m + 63) / 64) * ((max_n + 63) / 64);
+    // fewer products than XCDs: each product's tiles over the whole chip
+    // (a negative tile count selects that mapping in k_gemm)
+    const bool spread = count < 8;
     const int64_t groups8 = (count + 7) / 8;
-    const int64_t total = groups8 * 8 * tiles;
+    const int64_t total = spread ? (int64_t)count * tiles : groups8 * 8 * tiles;
     if (total > 0x7fffffff) return fail(ctx, PTBGPU_E_ARG, "GEMM batch too large (%lld workgroups)",
                                         (long long)total);
-    return launch_gemm(ctx, ta, tb, f32, dim3((unsigned)total), ddesc, count, tiles, nullptr);
+    return launch_gemm(ctx, ta, tb, f32, dim3((unsigned)total), ddesc, count, spread ? -tiles : tiles,
+                       nullptr);
 }
 
 // one: every product of the launch is a single k slice without triangular
@@ -912,7 +937,9 @@ int TileList::add(const GemmDesc *h, int count) {
 }
 
 // Every launch: products largest first (longest-processing-time order), each
-// dealt whole to the XCD with the fewest tiles so far; entry 8 k + x of the
+// dealt whole to the XCD with the fewest tiles so far (in a launch of fewer
+// products than XCDs, a product larger than an XCD's even share in runs of
+// tiles instead); entry 8 k + x of the
 // launch's block is XCD x's k-th tile (-1 where its queue has ended).  Tiles
 // of a product in column-major tile order, lower-triangle tiles only for uplo.
 void TileList::build(std::vector<GemmTile> &work) {
@@ -923,14 +950,13 @@ void TileList::build(std::vector<GemmTile> &work) {
         const int count = sig_[p++];
         const int *sh = sig_.data() + p;  // 5 ints per product
         p += 5 * (size_t)count;
-        std::vector<int> ord(count), xcd(count);
-        std::vector<int64_t> area(count), pos(count);
+        std::vector<int> ord(count);
+        std::vector<int64_t> area(count);
         for (int i = 0; i < count; ++i) {
             ord[i] = i;
             area[i] = (int64_t)std::max(sh[5 * i], 0) * std::max(sh[5 * i + 1], 0) * std::max(sh[5 * i + 2], 1);
         }
         std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return area[a] > area[b]; });
-        int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         bool one = true;
         auto tiles_of = [&](int i) -> int64_t {
             const int M = sh[5 * i], N = sh[5 * i + 1];
@@ -940,30 +966,38 @@ void TileList::build(std::vector<GemmTile> &work) {
             const int64_t c = std::min(tM, tN);  // columns tn < c hold tM - tn tiles
             return c * tM - c * (c - 1) / 2;
         };
+        int64_t total = 0;
+        for (int i = 0; i < count; ++i) total += tiles_of(i);
+        // With fewer products than XCDs, a product with more tiles than an
+        // XCD's even share of the launch is dealt in runs of consecutive tiles
+        // (column-major), each run to the XCD with the fewest tiles so far;
+        // every other product goes whole (its tiles share one L2).
+        const int64_t share = (total + 7) / 8;
+        std::vector<GemmTile> q[8];
         for (int i : ord) {
             const int64_t nt = tiles_of(i);
             if (nt == 0) continue;
             one = one && sh[5 * i + 2] <= kKS && sh[5 * i + 4] == 0;
+            const int tM = (sh[5 * i] + 63) / 64, tN = (sh[5 * i + 1] + 63) / 64;
+            const int64_t run = count < 8 && nt > share ? std::max<int64_t>(8, nt / 32) : nt;
+            int64_t k = 0;
             int x = 0;
-            for (int y = 1; y < 8; ++y)
-                if (load[y] < load[x]) x = y;
-            xcd[i] = x;
-            pos[i] = load[x];
-            load[x] += nt;
-        }
-        int64_t len = 0;
-        for (int x = 0; x < 8; ++x) len = std::max(len, load[x]);
-        const size_t base = work.size();
-        work.resize(base + 8 * (size_t)len, GemmTile{-1, 0});
-        for (int i : ord) {
-            const int M = sh[5 * i], N = sh[5 * i + 1];
-            if (tiles_of(i) == 0) continue;
-            const int tM = (M + 63) / 64, tN = (N + 63) / 64;
-            int64_t k = pos[i];
             for (int tn = 0; tn < tN; ++tn)
-                for (int tm = sh[5 * i + 3] ? tn : 0; tm < tM; ++tm, ++k)
-                    work[base + 8 * (size_t)k + xcd[i]] = GemmTile{i, tm | (tn << 16)};
+                for (int tm = sh[5 * i + 3] ? tn : 0; tm < tM; ++tm, ++k) {
+                    if (k % run == 0) {
+                        x = 0;
+                        for (int y = 1; y < 8; ++y)
+                            if (q[y].size() < q[x].size()) x = y;
+                    }
+                    q[x].push_back(GemmTile{i, tm | (tn << 16)});
+                }
         }
+        size_t len = 0;
+        for (int x = 0; x < 8; ++x) len = std::max(len, q[x].size());
+        const size_t base = work.size();
+        work.resize(base + 8 * len, GemmTile{-1, 0});
+        for (int x = 0; x < 8; ++x)
+            for (size_t k = 0; k < q[x].size(); ++k) work[base + 8 * k + x] = q[x][k];
         start_.push_back((int64_t)work.size());
         one_.push_back(one);
     }
@@ -1151,9 +1185,18 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
     for (int b = 0; b < nblk; ++b) tb.add(gb.data() + sb[b], sb[b + 1] - sb[b]);
     PTB_TRY(ts.upload(ctx, "potrf_ws"));
     PTB_TRY(tb.upload(ctx, "potrf_wb"));
+    // rows per panel over the device: one workgroup per submatrix when the
+    // batch fills the chip, else the rows split over several
+    const int ny = (int)std::min<int64_t>(std::max<int64_t>(1, ((int64_t)max_m + kT - 1) / kT),
+                                          std::max<int64_t>(1, (int64_t)ctx->num_cus / nsm));
     auto panel = [&](int p) -> int {
         ProfScope kprof_(ctx, "potrf_panel");
-        hipLaunchKernelGGL(k_potrf_panel, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, A, flags, p);
+        if (ny == 1) {
+            hipLaunchKernelGGL(k_potrf_panel, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, A, flags, p, 0);
+        } else {
+            hipLaunchKernelGGL(k_potrf_panel, dim3(nsm, ny), dim3(kT), 0, ctx->stream, ddesc, A, flags, p, 1);
+            hipLaunchKernelGGL(k_potrf_panel, dim3(nsm), dim3(kT), 0, ctx->stream, ddesc, A, flags, p, 2);
+        }
         PTB_CHECK_LAUNCH(ctx);
         return PTBGPU_OK;
     };

@@ -133,6 +133,7 @@ int filter_count(ptbgpu_ctx *ctx, double *dA, int64_t ndim, double filter, bool 
                                         (int64_t)std::max(ctx->num_cus, 1) * 16);
     if (blocks < 1) blocks = 1;
     const bool aligned = (reinterpret_cast<uintptr_t>(dA) % 16) == 0;
+    ProfScope kprof_(ctx, "filter_kernel");  // the packed pass alone
     if (aligned) {
         if (do_filter)
             hipLaunchKernelGGL(k_filter_pairs<true>, dim3(blocks), dim3(kThreads), 0, ctx->stream,

@@ -222,6 +222,12 @@ const char *kNaNPur = "Error: NaN is encountered during purification";
 
 extern "C" {
 
+int ptbgpu_thresholds(int64_t ndim, double *vals) {
+    if (ndim < 1 || !vals) return PTBGPU_E_ARG;
+    thresholds(ndim, vals[0], vals[1], vals[2]);
+    return PTBGPU_OK;
+}
+
 int ptbgpu_purify_dev(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *ps, int64_t n, const double *dHp,
                       const double *dSp, double *dPp, ptbgpu_pur_report *rep) {
     if (!ctx) return PTBGPU_E_ARG;

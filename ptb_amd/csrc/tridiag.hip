@@ -610,9 +610,21 @@ __global__ __launch_bounds__(kBR) void k_big_p(const T *__restrict__ a, int m, i
     double part = 0.0;
     if (i < m) {
         const int Ri = i / kBT;
+        // the partials in a fixed order, eight loads in flight at a time
         double s = 0.0;
-        for (int S = R0; S <= Ri; ++S) s += PR[(int64_t)S * m + i];
-        for (int R = Ri; R < R0 + nbr; ++R) s += PQ[(int64_t)R * m + i];
+        auto sum_rows = [&](const double *P, int b0, int b1) {
+            int b = b0;
+            for (; b + 8 <= b1; b += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = P[(int64_t)(b + u) * m + i];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += v[u];
+            }
+            for (; b < b1; ++b) s += P[(int64_t)b * m + i];
+        };
+        sum_rows(PR, R0, Ri + 1);
+        sum_rows(PQ, Ri, R0 + nbr);
         double p = s;
         for (int t = 0; t < jj; ++t)
             p -= a[(int64_t)(k0 + t) * ld + i] * q[t] + W[(int64_t)t * ld + i] * q[kNB + t];

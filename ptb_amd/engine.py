@@ -122,6 +122,8 @@ def _lib():
     for name in ("ptbgpu_matrix_sign", "ptbgpu_matrix_root"):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [VP, I64, VP, VP]
+    if hasattr(lib, "ptbgpu_thresholds"):
+        lib.ptbgpu_thresholds.argtypes = [I64, VP]
     if hasattr(lib, "ptbgpu_fermismear"):
         lib.ptbgpu_fermismear.argtypes = [I64, I64, F64, VP, VP, ctypes.POINTER(F64),
                                           ctypes.POINTER(F64), ctypes.POINTER(F64)]
@@ -203,6 +205,17 @@ def fermismear(norbs, nel, t, eig):
     if st != 0:
         raise PtbGpuError(st, lib.ptbgpu_strerror(st).decode())
     return occ, ef.value, fod.value, s.value
+
+
+def thresholds(ndim):
+    """adjust_thresholds_dp (source/metrics.f90:82-113) through the C ABI
+    (host only): (high, normal, low) for a problem of ndim basis functions."""
+    lib = _lib()
+    out = np.zeros(3)
+    st = lib.ptbgpu_thresholds(ndim, _ptr(out))
+    if st != 0:
+        raise PtbGpuError(st, lib.ptbgpu_strerror(st).decode())
+    return tuple(out)
 
 
 class Engine:

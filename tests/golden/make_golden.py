@@ -15,6 +15,11 @@ exist) on small deterministic inputs and stores inputs and outputs as .npz:
   pursearch_<case>  the same with a moving chemical-potential search (guess and
                     increment set, so bisection and the inner search engage)
   kmeans.npz        clustering_kmeans groups for nsm = 1..4 (flang RANDOM_NUMBER)
+  thresholds.npz    adjust_thresholds_dp (source/metrics.f90:82-113) on both
+                    sides of every tier boundary
+  purlarge_<case>   purification_wrapper with the chemical-potential search on
+                    the 8- and 18-water pencils (ndim 184 and 414, the second
+                    in the thresholds' second tier)
 
 Usage: python tests/golden/make_golden.py   (needs oracle/_ref/libptbref.so)
 """
@@ -97,6 +102,10 @@ def main():
             large_case(name)
     if want("mlpop"):
         mlpop_cases()
+    if want("thresholds"):
+        threshold_cases()
+    if want("purlarge"):
+        purification_large_cases()
 
 
 def metrics_and_fermi():
@@ -165,6 +174,44 @@ def purification_search_cases():
                             ndim=mol.ndim, ptype=ptype, metric=metric, iterative=it,
                             guess=guess, increment=inc, chempot_cycles=ccyc, P=P)
         print("pursearch", name, "nel", R.get_nel(mol.ndim, P, S))
+
+
+THRESHOLD_DIMS = [1, 46, 299, 300, 1999, 2000, 14999, 15000, 20000]
+
+
+def threshold_cases():
+    vals = np.array([R.thresholds(n) for n in THRESHOLD_DIMS])
+    np.savez_compressed(os.path.join(HERE, "thresholds.npz"), ndim=np.array(THRESHOLD_DIMS),
+                        vals=vals)
+    print("thresholds", dict(zip(THRESHOLD_DIMS, vals[:, 1])))
+
+
+# (type, metric, iterative): the combinations the GPU tests run on these pencils
+PUR_LARGE = [(1, 1, 0), (2, 2, 0), (3, 1, 1), (4, 2, 0), (4, 1, 0)]
+
+
+def purification_large_cases():
+    # each case in a child process: the compiled reference aborts on some of
+    # them (glibc heap check in sign diagonalisation at ndim 414), which must
+    # not end the others; such cases get no fixture
+    for size in ((2, 2, 2), (3, 3, 2)):
+        mol, H, S = system("water", size, 0, 1e-4)
+        for ptype, metric, it in PUR_LARGE:
+            name = f"purlarge_{ptype}{metric}{it}_{mol.ndim}"
+            pid = os.fork()
+            if pid == 0:
+                log = []
+                P = R.purification(mol.ndim, mol.nel, ptype, metric, it, 40, 50, 60, -1.5, 0.1,
+                                   H, S, log=log)
+                np.savez_compressed(os.path.join(HERE, name + ".npz"), H=H, S=S, nel=mol.nel,
+                                    ndim=mol.ndim, ptype=ptype, metric=metric, iterative=it,
+                                    guess=-1.5, increment=0.1, chempot_cycles=60, P=P,
+                                    log=log[0] if log else "")
+                print(name, "nel", R.get_nel(mol.ndim, P, S), flush=True)
+                os._exit(0)
+            _, status = os.waitpid(pid, 0)
+            if status != 0:
+                print(name, "the reference ended abnormally (status", status, "): no fixture")
 
 
 KMEANS_SM_CASES = {
