@@ -133,6 +133,9 @@ int ptbgpu_stream(ptbgpu_ctx *ctx, void **stream);
  * build reduces the submatrices below this dimension in FP64 (no float
  * storage, no refinement sweep) and the others on float storage with the FP64
  * refinement; 0 puts every submatrix on float storage (default 288).
+ * "gemm_tile": output tile edge of the FP64 products, 64 or 128 (identical
+ * results either way); 0 (default) chooses per launch: 128 only for launches
+ * of fewer than 8 large products.
  * "graph": 1 makes the construction and eigendecomposition phase of a build
  * (one chunk) replay a captured hipGraph once the same shapes and buffers
  * have been seen before (the first such build runs eagerly, the second

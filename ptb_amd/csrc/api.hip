@@ -172,6 +172,12 @@ int ptbgpu_set_tuning(ptbgpu_ctx *ctx, const char *key, int64_t value) {
         ctx->mixed_min_m = value;
         return PTBGPU_OK;
     }
+    if (k == "gemm_tile") {
+        if (value != 0 && value != 64 && value != 128)
+            return ptb::fail(ctx, PTBGPU_E_ARG, "gemm_tile must be 0, 64 or 128");
+        ctx->gemm_tile = value;
+        return PTBGPU_OK;
+    }
     if (k == "graph") {
         ctx->graph_mode = value != 0;
         if (!ctx->graph_mode) {
@@ -192,6 +198,10 @@ int ptbgpu_get_tuning(ptbgpu_ctx *ctx, const char *key, int64_t *value) {
     }
     if (k == "mixed_min_m") {
         *value = ctx->mixed_min_m;
+        return PTBGPU_OK;
+    }
+    if (k == "gemm_tile") {
+        *value = ctx->gemm_tile;
         return PTBGPU_OK;
     }
     if (k == "graph") {

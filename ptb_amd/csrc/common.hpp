@@ -48,6 +48,9 @@ struct ptbgpu_ctx {
     // less than the refinement costs, and small near-degenerate blocks leave
     // the refined density outside 1e-6 of FP64)
     int64_t mixed_min_m = 288;
+    // tile edge of the FP64 GEMM launches: 0 chooses per launch (128 x 128
+    // tiles for launches with many large products), 64 or 128 forces it
+    int64_t gemm_tile = 0;
     // "graph": the eigen phase of repeated one-chunk builds replays a captured
     // hipGraph; the cache of instantiated graphs (keyed by shapes and
     // buffers), the keys seen once, and whether a capture is in progress (no

@@ -37,14 +37,27 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 template <typename T>
 using gmem = __attribute__((address_space(1))) T;
 
-// 64x64 output tile per 256-thread workgroup; each wavefront owns a 32x32
-// quarter as 2x2 blocks of v_mfma_f64_16x16x4.  MFMA operand layout: lane l
-// supplies A(l%16, l/16) of a 16x4 slice and B(l/16, l%16) of a 4x16 slice;
-// result value r of lane l is D(4r + l/16, l%16).  K advances 32 at a time:
-// the next 64x32 / 32x64 slices are loaded into registers (coalesced along
-// the contiguous dimension of each operand) while the 32 MFMAs of the current
-// slices run from LDS.
-constexpr int kKS = 32;
+// TE x TE output tile per 256-thread workgroup (TE = 64 or 128); each
+// wavefront owns a TE/2 x TE/2 quarter as (TE/32)^2 blocks of
+// v_mfma_f64_16x16x4.  MFMA operand layout: lane l supplies A(l%16, l/16) of
+// a 16x4 slice and B(l/16, l%16) of a 4x16 slice; result value r of lane l is
+// D(4r + l/16, l%16).  K advances 2048 / TE at a time (8 elements of A and 8
+// of B per thread): the next slices are loaded into registers (coalesced
+// along the contiguous dimension of each operand) while the MFMAs of the
+// current slices run from LDS.  128 x 128 tiles read half the operand bytes
+// per flop of 64 x 64 ones (large products); both add every output element's
+// k terms in the same order (4 at a time, ascending, zeros outside the k
+// range), so the two tile sizes give identical results.
+constexpr int kKS = 32;  // k slice of the 64 x 64 tiles
+
+template <int TE>
+struct GemmTileShape {
+    static constexpr int KS = 2048 / TE;  // k slice
+    static constexpr int BW = TE / 32;    // 16 x 16 blocks along a wavefront's quarter
+    static constexpr int LD = TE + 4;     // row stride of the LDS slices
+    static constexpr int CLD = TE + 1;    // column stride of the epilogue's LDS columns
+    static constexpr int SMEM = 2 * KS * LD > 64 * CLD ? 2 * KS * LD : 64 * CLD;
+};
 
 // Workgroups are dealt to the 8 XCDs round-robin by linear id, so the 1-D grid
 // is read as (xcd, slot): XCD x runs the products x, x+8, x+16, ... tile after
@@ -59,16 +72,19 @@ constexpr int kKS = 32;
 //
 // With a tile list (work != nullptr) workgroup b computes the tile work[b],
 // already dealt to XCDs by the host; the grid is the list's length.
-template <bool TA, bool TB, typename T>
-__global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ descs, int count,
-                                               int tiles, const GemmTile *__restrict__ work) {
+template <bool TA, bool TB, typename T, int TE>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, kT), amdgpu_waves_per_eu(TE == 64 ? 4 : 2)))
+void k_gemm(const GemmDesc *__restrict__ descs, int count, int tiles,
+            const GemmTile *__restrict__ work) {
+    using S = GemmTileShape<TE>;
+    constexpr int KS = S::KS, BW = S::BW, LD = S::LD, CLD = S::CLD;
     int gi, m0, n0;
     if (work) {
         const GemmTile w = work[blockIdx.x];
         if (w.g < 0) return;
         gi = w.g;
-        m0 = (w.t & 0xffff) * 64;
-        n0 = (w.t >> 16) * 64;
+        m0 = (w.t & 0xffff) * TE;
+        n0 = (w.t >> 16) * TE;
     } else {
         const int64_t lin = blockIdx.x;
         int tile;
@@ -85,11 +101,11 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
             if (gi >= count) return;
             tile = (int)(slot % tiles);
         }
-        const int tM = (descs[gi].M + 63) / 64, tN = (descs[gi].N + 63) / 64;
+        const int tM = (descs[gi].M + TE - 1) / TE, tN = (descs[gi].N + TE - 1) / TE;
         if (tile >= tM * tN) return;
-        m0 = (tile % max(tM, 1)) * 64;
-        n0 = (tile / max(tM, 1)) * 64;
-        if (descs[gi].uplo && n0 > m0 + 63) return;  // tile strictly above the diagonal
+        m0 = (tile % max(tM, 1)) * TE;
+        n0 = (tile / max(tM, 1)) * TE;
+        if (descs[gi].uplo && n0 > m0 + TE - 1) return;  // tile strictly above the diagonal
     }
     const GemmDesc g = descs[gi];
     // (a list may hold tiles of a bound on a shape the device fills in)
@@ -101,33 +117,34 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
     // down to the slice depth: the other triangle of a triangular factor must
     // hold zeros (the few of its elements inside the range add nothing)
     int kbeg = 0, kend = g.K;
-    if (g.tri & kTriALower) kend = min(kend, m0 + 64);
+    if (g.tri & kTriALower) kend = min(kend, m0 + TE);
     if (g.tri & kTriAUpper) kbeg = max(kbeg, m0);
-    if (g.tri & kTriBUpper) kend = min(kend, n0 + 64);
+    if (g.tri & kTriBUpper) kend = min(kend, n0 + TE);
     if (g.tri & kTriBLower) kbeg = max(kbeg, n0);
-    kbeg &= ~(kKS - 1);
-    // A and B slices; after the k loop the same LDS holds the 64 x 64 result
-    // tile (column-major, leading dimension 65) for the epilogue
-    __shared__ double smem[2 * kKS * 68];
-    double (*As)[68] = reinterpret_cast<double (*)[68]>(smem);
-    double (*Bs)[68] = reinterpret_cast<double (*)[68]>(smem + kKS * 68);
+    kbeg &= ~(KS - 1);
+    // A and B slices; after the k loop the same LDS holds 64 columns of the
+    // result tile at a time (column-major, leading dimension TE + 1) for the
+    // epilogue
+    __shared__ double smem[S::SMEM];
+    double (*As)[LD] = reinterpret_cast<double (*)[LD]>(smem);
+    double (*Bs)[LD] = reinterpret_cast<double (*)[LD]>(smem + KS * LD);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int wm = (wave & 1) * (TE / 2), wn = (wave >> 1) * (TE / 2);
     const int lr = lane & 15, lk = lane >> 4;
-    d4_t acc[2][2];
+    d4_t acc[BW][BW];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < BW; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < BW; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
     T xa[8], xb[8];  // the next slices' raw elements (clamped addresses)
     // element q of this thread's share: A_op(mm, kk) and B_op(kk, nn)
     auto a_index = [&](int q, int &mm, int &kk) {
         const int e = threadIdx.x + q * kT;
-        if (!TA) { mm = e & 63; kk = e >> 6; } else { kk = e & 31; mm = e >> 5; }
+        if (!TA) { mm = e & (TE - 1); kk = e / TE; } else { kk = e & (KS - 1); mm = e / KS; }
     };
     auto b_index = [&](int q, int &kk, int &nn) {
         const int e = threadIdx.x + q * kT;
-        if (!TB) { kk = e & 31; nn = e >> 5; } else { nn = e & 63; kk = e >> 6; }
+        if (!TB) { kk = e & (KS - 1); nn = e / KS; } else { nn = e & (TE - 1); kk = e / TE; }
     };
     // element offsets are 32-bit (every operand block of the batch is below
     // 2^32 elements from its descriptor pointer), which keeps the 16 addresses
@@ -169,44 +186,61 @@ __global__ __launch_bounds__(kT, 4) void k_gemm(const GemmDesc *__restrict__ des
         store(kbeg);
         __syncthreads();
     }
-    for (int k0 = kbeg; k0 < kend; k0 += kKS) {
-        const bool more = k0 + kKS < kend;
-        if (more) load(k0 + kKS);
+    for (int k0 = kbeg; k0 < kend; k0 += KS) {
+        const bool more = k0 + KS < kend;
+        if (more) load(k0 + KS);
 #pragma unroll
-        for (int ks = 0; ks < kKS / 4; ++ks) {
+        for (int ks = 0; ks < KS / 4; ++ks) {
             const int kk = ks * 4 + lk;
-            const double a0 = As[kk][wm + lr], a1 = As[kk][wm + 16 + lr];
-            const double b0 = Bs[kk][wn + lr], b1 = Bs[kk][wn + 16 + lr];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            double a[BW], b[BW];
+#pragma unroll
+            for (int i = 0; i < BW; ++i) {
+                a[i] = As[kk][wm + 16 * i + lr];
+                b[i] = Bs[kk][wn + 16 * i + lr];
+            }
+#pragma unroll
+            for (int i = 0; i < BW; ++i)
+#pragma unroll
+                for (int j = 0; j < BW; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
         if (more) {
-            store(k0 + kKS);
+            store(k0 + KS);
             __syncthreads();
         }
     }
-    // epilogue: the tile goes through LDS so that C is read and written one
-    // column (64 consecutive rows) per wavefront instruction; the k loop ended
-    // with a barrier, so the slices' LDS is free
-    double *Ct = smem;  // Ct[col * 65 + row]
+    // epilogue, 64 columns of the tile at a time: they go through LDS so that
+    // C is read and written 64 consecutive rows per wavefront instruction; the
+    // k loop ended with a barrier, so the slices' LDS is free.  uplo = 2
+    // writes the lower 64 x 64 blocks whole at either tile size.
+    double *Ct = smem;  // Ct[col * CLD + row]
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int h = 0; h < TE / 64; ++h) {
+        if (h > 0) __syncthreads();  // the previous columns have been read
+        if (wn / 64 == h) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < BW; ++i)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                Ct[(wn + 16 * j + lr) * 65 + wm + 16 * i + 4 * r + lk] = acc[i][j][r];
-    __syncthreads();
-    const int gm = m0 + lane;
-    for (int cc = wave; cc < 64; cc += kT / 64) {
-        const int gn = n0 + cc;
-        if (gm >= g.M || gn >= g.N || (g.uplo == 1 && gm < gn)) continue;
-        auto *c = gC + (int64_t)gn * g.ldc + gm;
-        const double v = Ct[cc * 65 + lane];
-        *c = (T)(g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * (double)*c);
+                for (int j = 0; j < BW; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        Ct[(wn - 64 * h + 16 * j + lr) * CLD + wm + 16 * i + 4 * r + lk] = acc[i][j][r];
+        }
+        __syncthreads();
+        for (int cc = wave; cc < 64; cc += kT / 64) {
+            const int gn = n0 + 64 * h + cc;
+#pragma unroll
+            for (int rr = 0; rr < TE; rr += 64) {
+                const int gm = m0 + rr + lane;
+                if (gm >= g.M || gn >= g.N || (g.uplo == 1 && gm < gn) ||
+                    (g.uplo == 2 && (gm >> 6) < (gn >> 6)))
+                    continue;
+                auto *c = gC + (int64_t)gn * g.ldc + gm;
+                const double v = Ct[cc * CLD + rr + lane];
+                *c = (T)(g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * (double)*c);
+            }
+        }
     }
 }
 
@@ -865,8 +899,8 @@ int inv_lower_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
         auto *dg2 = static_cast<GemmDesc *>(upload_table(ctx, n2.c_str(), g2.data(), sizeof(GemmDesc) * g2.size()));
         if (!dg1 || !dg2) return PTBGPU_E_NOMEM;
         TileList t1, t2;
-        t1.add(g1.data(), (int)g1.size());
-        t2.add(g2.data(), (int)g2.size());
+        t1.add(g1.data(), (int)g1.size(), true);
+        t2.add(g2.data(), (int)g2.size(), true);
         PTB_TRY(t1.upload(ctx, ("linv_w1_" + std::to_string(level)).c_str()));
         PTB_TRY(t2.upload(ctx, ("linv_w2_" + std::to_string(level)).c_str()));
         PTB_TRY(t1.launch(ctx, 0, false, false, dg1, "linv"));
@@ -875,17 +909,70 @@ int inv_lower_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
     return PTBGPU_OK;
 }
 
+template <int TE>
+static void launch_gemm_te(ptbgpu_ctx *ctx, bool ta, bool tb, dim3 grid, const GemmDesc *ddesc,
+                           int count, int tiles, const GemmTile *work) {
+    if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false, double, TE>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
+    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false, double, TE>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
+    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true, double, TE>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
+    else hipLaunchKernelGGL((k_gemm<true, true, double, TE>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
+}
+
+// edge: 64 or 128 (double storage only)
 static int launch_gemm(ptbgpu_ctx *ctx, bool ta, bool tb, bool f32, dim3 grid, const GemmDesc *ddesc,
-                       int count, int tiles, const GemmTile *work) {
+                       int count, int tiles, const GemmTile *work, int edge) {
+    ProfScope eprof_(ctx, edge == 128 ? "gemm_tile128" : "gemm_tile64");
     if (f32) {  // float storage: only the tridiagonal update's form is used
-        if (ta || !tb) return fail(ctx, PTBGPU_E_ARG, "float-storage GEMM supports A B^T only");
-        hipLaunchKernelGGL((k_gemm<false, true, float>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
-    } else if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
-    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
-    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
-    else hipLaunchKernelGGL((k_gemm<true, true, double>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
+        if (ta || !tb || edge != 64)
+            return fail(ctx, PTBGPU_E_ARG, "float-storage GEMM supports A B^T on 64 x 64 tiles only");
+        hipLaunchKernelGGL((k_gemm<false, true, float, 64>), grid, dim3(kT), 0, ctx->stream, ddesc, count, tiles, work);
+    } else if (edge == 128) {
+        launch_gemm_te<128>(ctx, ta, tb, grid, ddesc, count, tiles, work);
+    } else {
+        launch_gemm_te<64>(ctx, ta, tb, grid, ddesc, count, tiles, work);
+    }
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
+}
+
+// Tile edge of an FP64 launch of `count` products from its 128 x 128 tile
+// count and the multiply-adds its tiles issue at both edges (padded output
+// times the k range of each tile): the gemm_tile knob forces 64 or 128; else
+// 128 x 128 tiles only for launches of fewer products than XCDs (single large
+// products, spread over the chip) with at least 4 such tiles per CU and at
+// most 3 % more work than 64 x 64 tiles.  On MI355X they reach 58 TFLOP/s on
+// one 4096^3 product against 50; on the batches of ragged, partly triangular
+// submatrix products (m about 100 to 1000) their coarser padding costs more
+// than the halved operand traffic saves, so those stay at 64.
+static int gemm_edge(int64_t mode, int num_cus, int count, int64_t tiles128, double work64,
+                     double work128) {
+    if (mode == 64 || mode == 128) return (int)mode;
+    return count < 8 && tiles128 >= 4 * (int64_t)std::max(num_cus, 1) && work128 <= 1.03 * work64
+               ? 128
+               : 64;
+}
+
+// Multiply-adds issued by the tiles of edge te of one product: per tile,
+// te^2 times its k range (the triangular factors' bounds, the start aligned
+// down to the slice depth 2048 / te), as k_gemm runs it.  K of a shape the
+// device fills in (0 on the host) counts as 1.
+static double tile_work(int M, int N, int K, int uplo, int tri, int te) {
+    if (M <= 0 || N <= 0) return 0.0;
+    const int ks = 2048 / te, tM = (M + te - 1) / te, tN = (N + te - 1) / te;
+    K = std::max(K, 1);
+    double w = 0.0;
+    for (int tn = 0; tn < tN; ++tn)
+        for (int tm = uplo ? tn : 0; tm < tM; ++tm) {
+            const int m0 = tm * te, n0 = tn * te;
+            int kbeg = 0, kend = K;
+            if (tri & kTriALower) kend = std::min(kend, m0 + te);
+            if (tri & kTriAUpper) kbeg = std::max(kbeg, m0);
+            if (tri & kTriBUpper) kend = std::min(kend, n0 + te);
+            if (tri & kTriBLower) kbeg = std::max(kbeg, n0);
+            kbeg &= ~(ks - 1);
+            if (kend > kbeg) w += (double)te * te * (((kend - kbeg) + ks - 1) / ks * ks);
+        }
+    return w;
 }
 
 int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int count, int max_m,
@@ -893,7 +980,12 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
     ProfScope prof_(ctx, "gemm");
     ProfScope sprof_(ctx, site ? (std::string("gemm_") + site).c_str() : "gemm_other");
     if (count <= 0 || max_m <= 0 || max_n <= 0) return PTBGPU_OK;
-    const int tiles = ((max_m + 63) / 64) * ((max_n + 63) / 64);
+    auto tiles_at = [&](int te) { return (int64_t)((max_m + te - 1) / te) * ((max_n + te - 1) / te); };
+    // (the k extent and triangles are on the device: padded outputs only)
+    const int edge = f32 ? 64
+                         : gemm_edge(ctx->gemm_tile, ctx->num_cus, count, count * tiles_at(128),
+                                     4096.0 * tiles_at(64), 16384.0 * tiles_at(128));
+    const int64_t tiles = tiles_at(edge);
     // fewer products than XCDs: each product's tiles over the whole chip
     // (a negative tile count selects that mapping in k_gemm)
     const bool spread = count < 8;
@@ -901,34 +993,37 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
     const int64_t total = spread ? (int64_t)count * tiles : groups8 * 8 * tiles;
     if (total > 0x7fffffff) return fail(ctx, PTBGPU_E_ARG, "GEMM batch too large (%lld workgroups)",
                                         (long long)total);
-    return launch_gemm(ctx, ta, tb, f32, dim3((unsigned)total), ddesc, count, spread ? -tiles : tiles,
-                       nullptr);
+    return launch_gemm(ctx, ta, tb, f32, dim3((unsigned)total), ddesc, count,
+                       (int)(spread ? -tiles : tiles), nullptr, edge);
 }
 
 // one: every product of the launch is a single k slice without triangular
-// factors (the A B^T form then runs k_gemm_nt1)
+// factors (the A B^T form then runs k_gemm_nt1); edge: the list's tile edge
 int gemm_tiles(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, const GemmTile *dwork,
-               int64_t nwork, const char *site, bool f32, bool one, int lowp) {
+               int64_t nwork, const char *site, bool f32, bool one, int lowp, int edge) {
     ProfScope prof_(ctx, "gemm");
     ProfScope sprof_(ctx, site ? (std::string("gemm_") + site).c_str() : "gemm_other");
     if (nwork <= 0) return PTBGPU_OK;
+    if (edge != 64 && (edge != 128 || f32 || lowp != kLowpNone))
+        return fail(ctx, PTBGPU_E_ARG, "GEMM tile edge %d is for FP64 products on double storage", edge);
     if (lowp != kLowpNone) {
         ProfScope lprof_(ctx, lowp == kLowpBF16 ? "gemm_bf16" : "gemm_bf16x3");
         return gemm_tiles_bf(ctx, ta, tb, ddesc, dwork, nwork, lowp, f32);
     }
     if (nwork > 0x7fffffff) return fail(ctx, PTBGPU_E_ARG, "GEMM tile list too long (%lld)", (long long)nwork);
     const dim3 grid((unsigned)nwork);
-    if (one && !ta && tb) {
+    if (one && !ta && tb && edge == 64) {
         if (f32) hipLaunchKernelGGL(k_gemm_nt1<float>, grid, dim3(kT), 0, ctx->stream, ddesc, dwork);
         else hipLaunchKernelGGL(k_gemm_nt1<double>, grid, dim3(kT), 0, ctx->stream, ddesc, dwork);
         PTB_CHECK_LAUNCH(ctx);
         return PTBGPU_OK;
     }
-    return launch_gemm(ctx, ta, tb, f32, grid, ddesc, 0, 1, dwork);
+    return launch_gemm(ctx, ta, tb, f32, grid, ddesc, 0, 1, dwork, edge);
 }
 
-int TileList::add(const GemmDesc *h, int count) {
+int TileList::add(const GemmDesc *h, int count, bool fp64) {
     sig_.push_back(count);
+    sig_.push_back(fp64 ? 1 : 0);
     for (int i = 0; i < count; ++i) {
         const GemmDesc &g = h[i];
         sig_.insert(sig_.end(), {g.M, g.N, g.K, g.uplo, g.tri});
@@ -936,20 +1031,48 @@ int TileList::add(const GemmDesc *h, int count) {
     return nlaunch_++;
 }
 
-// Every launch: products largest first (longest-processing-time order), each
+// Every launch: the tile edge (128 for an FP64 launch where gemm_edge says
+// so, else 64); products largest first (longest-processing-time order), each
 // dealt whole to the XCD with the fewest tiles so far (in a launch of fewer
 // products than XCDs, a product larger than an XCD's even share in runs of
-// tiles instead); entry 8 k + x of the
-// launch's block is XCD x's k-th tile (-1 where its queue has ended).  Tiles
-// of a product in column-major tile order, lower-triangle tiles only for uplo.
-void TileList::build(std::vector<GemmTile> &work) {
+// tiles instead); entry 8 k + x of the launch's block is XCD x's k-th tile
+// (-1 where its queue has ended).  Tiles of a product in column-major tile
+// order, lower-triangle tiles only for uplo.
+void TileList::build(std::vector<GemmTile> &work, int64_t tile_mode, int num_cus) {
     start_.assign(1, 0);
-    one_.clear();
+    flags_.clear();
     size_t p = 0;
     for (int l = 0; l < nlaunch_; ++l) {
         const int count = sig_[p++];
+        const bool fp64 = sig_[p++] != 0;
         const int *sh = sig_.data() + p;  // 5 ints per product
         p += 5 * (size_t)count;
+        auto tiles_of = [&](int i, int te) -> int64_t {
+            const int M = sh[5 * i], N = sh[5 * i + 1];
+            if (M <= 0 || N <= 0) return 0;
+            const int64_t tM = (M + te - 1) / te, tN = (N + te - 1) / te;
+            if (!sh[5 * i + 3]) return tM * tN;
+            const int64_t c = std::min(tM, tN);  // columns tn < c hold tM - tn tiles
+            return c * tM - c * (c - 1) / 2;
+        };
+        bool one = true;
+        int64_t t128 = 0;
+        for (int i = 0; i < count; ++i) {
+            if (tiles_of(i, 64) == 0) continue;
+            one = one && sh[5 * i + 2] <= kKS && sh[5 * i + 4] == 0;
+            t128 += tiles_of(i, 128);
+        }
+        int te = 64;
+        if (fp64 && !one && t128 > 0) {
+            double w64 = 0.0, w128 = 0.0;
+            if (tile_mode == 0 && count < 8)  // (gemm_edge needs the work only then)
+                for (int i = 0; i < count; ++i) {
+                    const int *d = sh + 5 * i;
+                    w64 += tile_work(d[0], d[1], d[2], d[3], d[4], 64);
+                    w128 += tile_work(d[0], d[1], d[2], d[3], d[4], 128);
+                }
+            te = gemm_edge(tile_mode, num_cus, count, t128, w64, w128);
+        }
         std::vector<int> ord(count);
         std::vector<int64_t> area(count);
         for (int i = 0; i < count; ++i) {
@@ -957,17 +1080,8 @@ void TileList::build(std::vector<GemmTile> &work) {
             area[i] = (int64_t)std::max(sh[5 * i], 0) * std::max(sh[5 * i + 1], 0) * std::max(sh[5 * i + 2], 1);
         }
         std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return area[a] > area[b]; });
-        bool one = true;
-        auto tiles_of = [&](int i) -> int64_t {
-            const int M = sh[5 * i], N = sh[5 * i + 1];
-            if (M <= 0 || N <= 0) return 0;
-            const int64_t tM = (M + 63) / 64, tN = (N + 63) / 64;
-            if (!sh[5 * i + 3]) return tM * tN;
-            const int64_t c = std::min(tM, tN);  // columns tn < c hold tM - tn tiles
-            return c * tM - c * (c - 1) / 2;
-        };
         int64_t total = 0;
-        for (int i = 0; i < count; ++i) total += tiles_of(i);
+        for (int i = 0; i < count; ++i) total += tiles_of(i, te);
         // With fewer products than XCDs, a product with more tiles than an
         // XCD's even share of the launch is dealt in runs of consecutive tiles
         // (column-major), each run to the XCD with the fewest tiles so far;
@@ -975,10 +1089,9 @@ void TileList::build(std::vector<GemmTile> &work) {
         const int64_t share = (total + 7) / 8;
         std::vector<GemmTile> q[8];
         for (int i : ord) {
-            const int64_t nt = tiles_of(i);
+            const int64_t nt = tiles_of(i, te);
             if (nt == 0) continue;
-            one = one && sh[5 * i + 2] <= kKS && sh[5 * i + 4] == 0;
-            const int tM = (sh[5 * i] + 63) / 64, tN = (sh[5 * i + 1] + 63) / 64;
+            const int tM = (sh[5 * i] + te - 1) / te, tN = (sh[5 * i + 1] + te - 1) / te;
             const int64_t run = count < 8 && nt > share ? std::max<int64_t>(8, nt / 32) : nt;
             int64_t k = 0;
             int x = 0;
@@ -999,7 +1112,7 @@ void TileList::build(std::vector<GemmTile> &work) {
         for (int x = 0; x < 8; ++x)
             for (size_t k = 0; k < q[x].size(); ++k) work[base + 8 * k + x] = q[x][k];
         start_.push_back((int64_t)work.size());
-        one_.push_back(one);
+        flags_.push_back((char)((one ? kOne : 0) | (te == 128 ? kEdge128 : 0)));
     }
 }
 
@@ -1007,11 +1120,14 @@ int TileList::upload(ptbgpu_ctx *ctx, const char *name_in) {
     const std::string full = ctx->table_prefix + name_in;  // see upload_table
     const char *name = full.c_str();
     // the shadow entry of `name` holds the shapes of the table in its buffer
-    // (ws drops it when the buffer is reallocated); "<name>#meta" holds the
-    // launch starts and flags built from them
+    // and the tile-edge knob they were dealt under (ws drops it when the
+    // buffer is reallocated); "<name>#meta" holds the launch starts and flags
+    // built from them
     const std::string mkey = std::string(name) + "#meta";
-    const char *sb = reinterpret_cast<const char *>(sig_.data());
-    const size_t sn = sizeof(int) * sig_.size();
+    std::vector<int> sig(sig_);
+    sig.push_back((int)ctx->gemm_tile);
+    const char *sb = reinterpret_cast<const char *>(sig.data());
+    const size_t sn = sizeof(int) * sig.size();
     auto sh = ctx->shadow.find(name);
     auto mh = ctx->shadow.find(mkey);
     const size_t nl = (size_t)nlaunch_;
@@ -1019,18 +1135,18 @@ int TileList::upload(ptbgpu_ctx *ctx, const char *name_in) {
         memcmp(sh->second.data(), sb, sn) == 0 &&
         mh->second.size() == sizeof(int64_t) * (nl + 1) + nl) {
         start_.resize(nl + 1);
-        one_.resize(nl);
+        flags_.resize(nl);
         memcpy(start_.data(), mh->second.data(), sizeof(int64_t) * (nl + 1));
-        memcpy(one_.data(), mh->second.data() + sizeof(int64_t) * (nl + 1), nl);
+        memcpy(flags_.data(), mh->second.data() + sizeof(int64_t) * (nl + 1), nl);
         if (start_.back() == 0) return PTBGPU_OK;
         dwork_ = static_cast<const GemmTile *>(ws(ctx, name, sizeof(GemmTile) * start_.back()));
         return dwork_ ? PTBGPU_OK : PTBGPU_E_NOMEM;
     }
     std::vector<GemmTile> work;
-    build(work);
+    build(work, ctx->gemm_tile, ctx->num_cus);
     std::vector<char> meta(sizeof(int64_t) * (nl + 1) + nl);
     memcpy(meta.data(), start_.data(), sizeof(int64_t) * (nl + 1));
-    memcpy(meta.data() + sizeof(int64_t) * (nl + 1), one_.data(), nl);
+    memcpy(meta.data() + sizeof(int64_t) * (nl + 1), flags_.data(), nl);
     if (work.empty()) {
         ctx->shadow[name].assign(sb, sb + sn);
         ctx->shadow[mkey] = meta;
@@ -1057,7 +1173,8 @@ int TileList::launch(ptbgpu_ctx *ctx, int i, bool ta, bool tb, const GemmDesc *d
     const int64_t n = start_[i + 1] - start_[i];
     if (n == 0) return PTBGPU_OK;
     if (!dwork_) return fail(ctx, PTBGPU_E_ARG, "GEMM tile list launched before upload");
-    return gemm_tiles(ctx, ta, tb, ddesc, dwork_ + start_[i], n, site, f32, one_[i] != 0, lowp);
+    return gemm_tiles(ctx, ta, tb, ddesc, dwork_ + start_[i], n, site, f32, (flags_[i] & kOne) != 0,
+                      lowp, (flags_[i] & kEdge128) ? 128 : 64);
 }
 
 // X = L^-1 B (trans=false) or L^-T B (trans=true), all m columns, blocked by
@@ -1181,8 +1298,8 @@ int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc
     auto *dgb = static_cast<GemmDesc *>(upload_table(ctx, "potrf_gb", gb.data(), sizeof(GemmDesc) * gb.size()));
     if (!dgs || !dgb) return PTBGPU_E_NOMEM;
     TileList ts, tb;
-    for (int l = 0; l < nblk * (kPB - 1); ++l) ts.add(gs.data() + ss[l], ss[l + 1] - ss[l]);
-    for (int b = 0; b < nblk; ++b) tb.add(gb.data() + sb[b], sb[b + 1] - sb[b]);
+    for (int l = 0; l < nblk * (kPB - 1); ++l) ts.add(gs.data() + ss[l], ss[l + 1] - ss[l], true);
+    for (int b = 0; b < nblk; ++b) tb.add(gb.data() + sb[b], sb[b + 1] - sb[b], true);
     PTB_TRY(ts.upload(ctx, "potrf_ws"));
     PTB_TRY(tb.upload(ctx, "potrf_wb"));
     // rows per panel over the device: one workgroup per submatrix when the
@@ -1354,9 +1471,9 @@ int larft_all(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *
     auto *dgt = static_cast<GemmDesc *>(upload_table(ctx, "q_t128_gt", gt.data(), gbytes));
     if (!dga || !dgm || !dgt) return PTBGPU_E_NOMEM;
     TileList ta, tm, tt;
-    ta.add(ga.data(), (int)ga.size());
-    tm.add(gm.data(), (int)gm.size());
-    tt.add(gt.data(), (int)gt.size());
+    ta.add(ga.data(), (int)ga.size(), true);
+    tm.add(gm.data(), (int)gm.size(), true);
+    tt.add(gt.data(), (int)gt.size(), true);
     PTB_TRY(ta.upload(ctx, "q_t128_wa"));
     PTB_TRY(tm.upload(ctx, "q_t128_wm"));
     PTB_TRY(tt.upload(ctx, "q_t128_wt"));
@@ -1427,9 +1544,9 @@ int apply_q(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const QFactor &qf
     TileList t1, t2, t3;
     for (int q = 0; q < qf.maxblk128; ++q) {
         const int c = start[q + 1] - start[q];
-        t1.add(g1.data() + start[q], c);
-        t2.add(g2.data() + start[q], c);
-        t3.add(g3.data() + start[q], c);
+        t1.add(g1.data() + start[q], c, true);
+        t2.add(g2.data() + start[q], c, true);
+        t3.add(g3.data() + start[q], c, true);
     }
     PTB_TRY(t1.upload(ctx, ("q_w1_" + t).c_str()));
     PTB_TRY(t2.upload(ctx, ("q_w2_" + t).c_str()));

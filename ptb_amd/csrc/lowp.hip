@@ -234,7 +234,7 @@ extern "C" int ptbgpu_gemm_batch_dev(ptbgpu_ctx *ctx, int64_t precision, int64_t
                                                           sizeof(GemmDesc) * g.size()));
     if (!dg) return PTBGPU_E_NOMEM;
     ptb::TileList tl;
-    tl.add(g.data(), (int)count);
+    tl.add(g.data(), (int)count, precision == kLowpNone);
     PTB_TRY(tl.upload(ctx, "gemm_api_tiles"));
     PTB_TRY(tl.launch(ctx, 0, ta, tb, dg, "api", false, (int)precision));
     PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));

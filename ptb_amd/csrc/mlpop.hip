@@ -98,7 +98,7 @@ int mlpop2_device(ptbgpu_ctx *ctx, int64_t n, int64_t nao, const double *dP, con
     auto *dg = static_cast<GemmDesc *>(ptb::upload_table(ctx, "mlpop_g", &g, sizeof(g)));
     if (!dg) return PTBGPU_E_NOMEM;
     ptb::TileList tl;
-    tl.add(&g, 1);
+    tl.add(&g, 1, true);
     PTB_TRY(tl.upload(ctx, "mlpop_w"));
     PTB_TRY(tl.launch(ctx, 0, false, false, dg, "mlpop"));
     hipLaunchKernelGGL(k_row_col_dot, dim3((unsigned)((nao + 63) / 64)), dim3(kT), 0, ctx->stream,

@@ -328,7 +328,7 @@ int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
         GemmDesc *dg;
         PTB_TRY(upload(ctx, name, g, &dg));
         ptb::TileList tl;
-        tl.add(g.data(), nsm);
+        tl.add(g.data(), nsm, prec == kLowpNone);
         PTB_TRY(tl.upload(ctx, (std::string(name) + "_w").c_str()));
         return tl.launch(ctx, 0, ta, false, dg, "refine", false, prec);
     };
@@ -818,8 +818,8 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             PTB_TRY(upload(ctx, "sm_gk1", g1, &dg1));
             PTB_TRY(upload(ctx, "sm_gk2", g2, &dg2));
             ptb::TileList t1, t2;
-            t1.add(g1.data(), n);
-            t2.add(g2.data(), n);
+            t1.add(g1.data(), n, true);
+            t2.add(g2.data(), n, true);
             PTB_TRY(t1.upload(ctx, "sm_wk1"));
             PTB_TRY(t2.upload(ctx, "sm_wk2"));
             PTB_TRY(t1.launch(ctx, 0, false, false, dg1, "kform"));
@@ -844,7 +844,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             }
             PTB_TRY(upload(ctx, "sm_gf", gf, &c.dgf));
             c.tf = ptb::TileList{};
-            c.tf.add(gf.data(), n);
+            c.tf.add(gf.data(), n, true);
             PTB_TRY(c.tf.upload(ctx, "sm_wf"));
             PTB_TRY(c.tf.launch(ctx, 0, true, false, c.dgf, "fe"));
         }
@@ -867,7 +867,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             // the addresses of every shared buffer key the graph itself
             mix(c.desc.data(), sizeof(SubDesc) * c.desc.size());
             const int64_t knobs[] = {(int64_t)lowp, (int64_t)mixed, ctx->dc_bucket_min,
-                                     ctx->mixed_min_m};
+                                     ctx->mixed_min_m, ctx->gemm_tile};
             mix(knobs, sizeof(knobs));
             char pfx[32];
             snprintf(pfx, sizeof(pfx), "g%016llx:", (unsigned long long)key);
@@ -948,7 +948,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             GemmDesc *dgu;
             PTB_TRY(upload(ctx, "sm_gu", gu, &dgu));
             ptb::TileList tu;
-            tu.add(gu.data(), n);
+            tu.add(gu.data(), n, true);
             PTB_TRY(tu.upload(ctx, "sm_wu"));
             PTB_TRY(tu.launch(ctx, 0, true, false, dgu, "u"));
         }
@@ -1177,7 +1177,7 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
             shape[2 * i].N = shape[2 * i + 1].N = n;
         }
         TileList tl;
-        if (nbig > 0) tl.add(shape.data(), 2 * nbig);
+        if (nbig > 0) tl.add(shape.data(), 2 * nbig, true);  // launched in FP64
         if (nbig < (int)nodes.size()) tl.add(shape.data() + 2 * nbig, 2 * ((int)nodes.size() - nbig));
         PTB_TRY(tl.upload(ctx, ("sm_lwork" + std::to_string(lv)).c_str()));
         std::vector<int> sizes(nodes.size());

@@ -143,30 +143,35 @@ int gemm_vbatch(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, int co
                 int max_n, const char *site = nullptr, bool f32 = false);
 int gemm_tiles(ptbgpu_ctx *ctx, bool ta, bool tb, const GemmDesc *ddesc, const GemmTile *dwork,
                int64_t nwork, const char *site = nullptr, bool f32 = false, bool one = false,
-               int lowp = kLowpNone);
+               int lowp = kLowpNone, int edge = 64);
 
 // Tile lists of batched GEMMs whose shapes the host knows.  Each launch gets
 // exactly the tiles that hold output (lower-triangle tiles for uplo), dealt to
 // the 8 XCDs so that a product's tiles share one XCD's L2 and the XCDs get
 // equal tile counts.  All launches' lists go to the device as one table.
-// The lists are built on the host only when the shapes differ from those of
-// the table already in the device buffer of that name (repeated builds of the
-// same system reuse it).
+// The lists are built on the host only when the shapes (or the tile-edge
+// knob) differ from those of the table already in the device buffer of that
+// name (repeated builds of the same system reuse it).
 class TileList {
   public:
-    // records the shapes of one launch over products h[0 .. count); returns its index
-    int add(const GemmDesc *h, int count);
+    // records the shapes of one launch over products h[0 .. count); returns
+    // its index.  fp64: the launch runs in FP64 arithmetic on double storage
+    // (never the low-precision or float-storage forms), so it may take
+    // 128 x 128 tiles (same results as 64 x 64 ones)
+    int add(const GemmDesc *h, int count, bool fp64 = false);
     // builds (if the shapes changed) and uploads the lists of every launch
     int upload(ptbgpu_ctx *ctx, const char *name);
     int launch(ptbgpu_ctx *ctx, int i, bool ta, bool tb, const GemmDesc *ddesc,
                const char *site = nullptr, bool f32 = false, int lowp = kLowpNone) const;
 
   private:
-    void build(std::vector<GemmTile> &work);
-    std::vector<int> sig_;        // per launch: count, then M, N, K, uplo, tri per product
+    void build(std::vector<GemmTile> &work, int64_t tile_mode, int num_cus);
+    static constexpr char kOne = 1;       // every product is one k slice (K <= 32, no tri)
+    static constexpr char kEdge128 = 2;   // 128 x 128 tiles
+    std::vector<int> sig_;        // per launch: count, fp64, then M, N, K, uplo, tri per product
     int nlaunch_ = 0;
     std::vector<int64_t> start_;  // per launch: first entry in the table (+ the total)
-    std::vector<char> one_;       // per launch: every product is one k slice (K <= 32, no tri)
+    std::vector<char> flags_;     // per launch: kOne, kEdge128
     const GemmTile *dwork_ = nullptr;
 };
 int potrf_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
