@@ -568,7 +568,12 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     PTB_HIP(ctx, hipMemcpyAsync(mcount.data(), dcount, sizeof(int) * nsm, hipMemcpyDeviceToHost, st));
     PTB_HIP(ctx, hipStreamSynchronize(st));
     std::vector<SubDesc> desc(nsm);
-    const bool mixed = opts.precision != PTBGPU_FP64;
+    // a mixed build reduces the submatrices below mixed_min_m in FP64; when
+    // that is all of them it is the FP64 build (same kernels, same results,
+    // none of the float copies)
+    bool mixed = false;
+    if (opts.precision != PTBGPU_FP64)
+        for (int s = 0; s < nsm && !mixed; ++s) mixed = mcount[s] >= ctx->mixed_min_m;
     std::vector<int64_t> rbeg(nmol + 1, 0);
     std::vector<int> mol_max_m(nmol, 0);
     int64_t arena = 0, rows = 0;
@@ -583,7 +588,6 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             d.mapoff = (int)rows;
             d.gbeg = gptr[s];
             d.gend = gptr[s + 1];
-            // a mixed build reduces the submatrices below mixed_min_m in FP64
             d.flags = mixed && d.m < ctx->mixed_min_m ? kSubExact : 0;
             d.mol = i;
             arena += (int64_t)d.ld * d.m;
@@ -720,11 +724,8 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     auto bisect_all = [&]() -> int {
         return ptb::bisect_batch(ctx, dvec, fe, drbeg, dnelref, nmol, max_rows, bis);
     };
-    // the refinement sweeps correct the float-reduced submatrices (none when
-    // every submatrix of a mixed build is below mixed_min_m)
-    bool any_float = false;
-    for (const auto &d : desc) any_float = any_float || !(d.flags & kSubExact);
-    const int sweeps = !mixed || !any_float || opts.refine_steps < 0
+    // the refinement sweeps correct the float-reduced submatrices
+    const int sweeps = !mixed || opts.refine_steps < 0
                            ? 0
                            : (int)std::max<int64_t>(1, opts.refine_steps);
 
