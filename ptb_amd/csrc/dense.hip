@@ -2,9 +2,9 @@
 //
 // * k_gemm:     C = alpha op(A) op(B) + beta C for a variable-size batch whose
 //               shapes and pointers come from device descriptors (so a kernel
-//               earlier in the stream can size the next product).  64x64 output
-//               tile per workgroup, 32-deep k slices staged in LDS with the
-//               next slice prefetched into registers.
+//               earlier in the stream can size the next product).  64x64 (or
+//               128x128) output tile per workgroup, 16-deep k slices staged in
+//               LDS with the next slice prefetched into registers.
 // * k_potrf_panel: one 32-column panel of the lower Cholesky S = L L^T: the
 //               diagonal block factored and inverted in LDS, the panel below as
 //               X = P L_kk^-T; the trailing update is a batched GEMM.  A
@@ -41,22 +41,30 @@ using gmem = __attribute__((address_space(1))) T;
 // wavefront owns a TE/2 x TE/2 quarter as (TE/32)^2 blocks of
 // v_mfma_f64_16x16x4.  MFMA operand layout: lane l supplies A(l%16, l/16) of
 // a 16x4 slice and B(l/16, l%16) of a 4x16 slice; result value r of lane l is
-// D(4r + l/16, l%16).  K advances 2048 / TE at a time (8 elements of A and 8
-// of B per thread): the next slices are loaded into registers (coalesced
-// along the contiguous dimension of each operand) while the MFMAs of the
-// current slices run from LDS.  128 x 128 tiles read half the operand bytes
-// per flop of 64 x 64 ones (large products); both add every output element's
-// k terms in the same order (4 at a time, ascending, zeros outside the k
-// range), so the two tile sizes give identical results.
-constexpr int kKS = 32;  // k slice of the 64 x 64 tiles
+// D(4r + l/16, l%16).  K advances 16 at a time (TE/16 elements of A and of B
+// per thread): the next slices are loaded into registers (coalesced along the
+// contiguous dimension of each operand) while the MFMAs of the current slices
+// run from LDS.  128 x 128 tiles read half the operand bytes per flop of
+// 64 x 64 ones (large products); both add every output element's k terms in
+// the same order (4 at a time, ascending, zeros outside the k range), so the
+// two tile sizes give identical results.
+//
+// The 64 x 64 instance is latency-bound: its speed follows the wavefronts
+// resident per SIMD.  16-deep slices and 32-column epilogue passes (17 KB of
+// LDS, 88 VGPRs) hold five per SIMD; on the 512-water build that is 4 % less
+// GEMM time than 32-deep slices at four per SIMD, and 64-deep slices at two
+// per SIMD cost 20 % more.
+constexpr int kKS = 32;  // k depth up to which a product counts as one slice (k_gemm_nt1)
 
 template <int TE>
 struct GemmTileShape {
-    static constexpr int KS = 2048 / TE;  // k slice
+    static constexpr int KS = 16;         // k slice
+    static constexpr int NL = TE * KS / 256;  // elements of A (and of B) per thread per slice
     static constexpr int BW = TE / 32;    // 16 x 16 blocks along a wavefront's quarter
     static constexpr int LD = TE + 4;     // row stride of the LDS slices
     static constexpr int CLD = TE + 1;    // column stride of the epilogue's LDS columns
-    static constexpr int SMEM = 2 * KS * LD > 64 * CLD ? 2 * KS * LD : 64 * CLD;
+    static constexpr int CW = 32;         // columns of the tile per epilogue pass
+    static constexpr int SMEM = 2 * KS * LD > CW * CLD ? 2 * KS * LD : CW * CLD;
 };
 
 // Workgroups are dealt to the 8 XCDs round-robin by linear id, so the 1-D grid
@@ -73,11 +81,11 @@ struct GemmTileShape {
 // With a tile list (work != nullptr) workgroup b computes the tile work[b],
 // already dealt to XCDs by the host; the grid is the list's length.
 template <bool TA, bool TB, typename T, int TE>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, kT), amdgpu_waves_per_eu(TE == 64 ? 4 : 2)))
+__global__ __attribute__((amdgpu_flat_work_group_size(1, kT), amdgpu_waves_per_eu(TE == 64 ? 5 : 2)))
 void k_gemm(const GemmDesc *__restrict__ descs, int count, int tiles,
             const GemmTile *__restrict__ work) {
     using S = GemmTileShape<TE>;
-    constexpr int KS = S::KS, BW = S::BW, LD = S::LD, CLD = S::CLD;
+    constexpr int KS = S::KS, BW = S::BW, LD = S::LD, CLD = S::CLD, NL = S::NL, CW = S::CW;
     int gi, m0, n0;
     if (work) {
         const GemmTile w = work[blockIdx.x];
@@ -136,7 +144,7 @@ void k_gemm(const GemmDesc *__restrict__ descs, int count, int tiles,
     for (int i = 0; i < BW; ++i)
 #pragma unroll
         for (int j = 0; j < BW; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-    T xa[8], xb[8];  // the next slices' raw elements (clamped addresses)
+    T xa[NL], xb[NL];  // the next slices' raw elements (clamped addresses)
     // element q of this thread's share: A_op(mm, kk) and B_op(kk, nn)
     auto a_index = [&](int q, int &mm, int &kk) {
         const int e = threadIdx.x + q * kT;
@@ -158,7 +166,7 @@ void k_gemm(const GemmDesc *__restrict__ descs, int count, int tiles,
     const unsigned ulda = (unsigned)g.lda, uldb = (unsigned)g.ldb;
     auto load = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < NL; ++q) {
             int mm, kk;
             a_index(q, mm, kk);
             const unsigned cm = (unsigned)min(m0 + mm, g.M - 1), ck = (unsigned)min(k0 + kk, kend - 1);
@@ -171,9 +179,9 @@ void k_gemm(const GemmDesc *__restrict__ descs, int count, int tiles,
     };
     auto store = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) asm volatile("" : "+v"(xa[q]), "+v"(xb[q]));
+        for (int q = 0; q < NL; ++q) asm volatile("" : "+v"(xa[q]), "+v"(xb[q]));
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < NL; ++q) {
             int mm, kk, kb, nn;
             a_index(q, mm, kk);
             b_index(q, kb, nn);
@@ -210,26 +218,26 @@ void k_gemm(const GemmDesc *__restrict__ descs, int count, int tiles,
             __syncthreads();
         }
     }
-    // epilogue, 64 columns of the tile at a time: they go through LDS so that
+    // epilogue, CW columns of the tile at a time: they go through LDS so that
     // C is read and written 64 consecutive rows per wavefront instruction; the
     // k loop ended with a barrier, so the slices' LDS is free.  uplo = 2
     // writes the lower 64 x 64 blocks whole at either tile size.
     double *Ct = smem;  // Ct[col * CLD + row]
 #pragma unroll
-    for (int h = 0; h < TE / 64; ++h) {
+    for (int h = 0; h < TE / CW; ++h) {
         if (h > 0) __syncthreads();  // the previous columns have been read
-        if (wn / 64 == h) {
+#pragma unroll
+        for (int j = 0; j < BW; ++j) {
+            if ((wn + 16 * j) / CW != h) continue;  // (uniform over the wavefront)
 #pragma unroll
             for (int i = 0; i < BW; ++i)
 #pragma unroll
-                for (int j = 0; j < BW; ++j)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        Ct[(wn - 64 * h + 16 * j + lr) * CLD + wm + 16 * i + 4 * r + lk] = acc[i][j][r];
+                for (int r = 0; r < 4; ++r)
+                    Ct[(wn - CW * h + 16 * j + lr) * CLD + wm + 16 * i + 4 * r + lk] = acc[i][j][r];
         }
         __syncthreads();
-        for (int cc = wave; cc < 64; cc += kT / 64) {
-            const int gn = n0 + 64 * h + cc;
+        for (int cc = wave; cc < CW; cc += kT / 64) {
+            const int gn = n0 + CW * h + cc;
 #pragma unroll
             for (int rr = 0; rr < TE; rr += 64) {
                 const int gm = m0 + rr + lane;
@@ -249,9 +257,25 @@ void k_gemm(const GemmDesc *__restrict__ descs, int count, int tiles,
 // read into registers together with the A and B slices, so that a tile waits
 // on memory once before its MFMAs instead of a second time in the epilogue.
 // Tiles come from a tile list; layouts as in k_gemm.
+//
+// The slices stay in LDS in the storage type (float rows padded to 80
+// elements: the four k rows a wavefront reads at once fall on disjoint
+// banks), and the result goes out in two passes of 32 columns: 20 KB of LDS
+// with float storage, so that up to seven workgroups share a CU (these
+// updates are bound by the trailing matrix's traffic).
 template <typename T>
-__global__ __launch_bounds__(kT, 4) void k_gemm_nt1(const GemmDesc *__restrict__ descs,
-                                                    const GemmTile *__restrict__ work) {
+struct Nt1Shape {
+    static constexpr int LD = sizeof(T) == 4 ? 80 : 68;  // slice row stride (elements of T)
+    static constexpr int SLICE = 2 * kKS * LD * (int)sizeof(T);
+    static constexpr int EPI = 32 * 65 * (int)sizeof(double);
+    static constexpr int BYTES = SLICE > EPI ? SLICE : EPI;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kT, sizeof(T) == 4 ? 6 : 4) void k_gemm_nt1(
+    const GemmDesc *__restrict__ descs, const GemmTile *__restrict__ work) {
+    using S = Nt1Shape<T>;
+    constexpr int LD = S::LD;
     const GemmTile wt = work[blockIdx.x];
     if (wt.g < 0) return;
     const GemmDesc g = descs[wt.g];
@@ -260,9 +284,9 @@ __global__ __launch_bounds__(kT, 4) void k_gemm_nt1(const GemmDesc *__restrict__
     const gmem<T> *__restrict__ gA = (const gmem<T> *)(const void *)g.A;
     const gmem<T> *__restrict__ gB = (const gmem<T> *)(const void *)g.B;
     gmem<T> *gC = (gmem<T> *)(void *)g.C;
-    __shared__ double smem[2 * kKS * 68];
-    double (*As)[68] = reinterpret_cast<double (*)[68]>(smem);
-    double (*Bs)[68] = reinterpret_cast<double (*)[68]>(smem + kKS * 68);
+    __shared__ double smem[S::BYTES / sizeof(double)];
+    T (*As)[LD] = reinterpret_cast<T (*)[LD]>(smem);
+    T (*Bs)[LD] = reinterpret_cast<T (*)[LD]>(reinterpret_cast<T *>(smem) + kKS * LD);
     // wave-uniform wavefront index: the C column offsets stay scalar
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
@@ -303,8 +327,8 @@ __global__ __launch_bounds__(kT, 4) void k_gemm_nt1(const GemmDesc *__restrict__
     for (int q = 0; q < 8; ++q) {
         const int e = threadIdx.x + q * kT, mm = e & 63, kk = e >> 6;
         const bool kin = kk < g.K;
-        As[kk][mm] = (kin && m0 + mm < g.M) ? (double)xa[q] : 0.0;
-        Bs[kk][mm] = (kin && n0 + mm < g.N) ? (double)xb[q] : 0.0;
+        As[kk][mm] = (kin && m0 + mm < g.M) ? xa[q] : T(0);
+        Bs[kk][mm] = (kin && n0 + mm < g.N) ? xb[q] : T(0);
     }
     __syncthreads();
     d4_t acc[2][2];
@@ -322,22 +346,28 @@ __global__ __launch_bounds__(kT, 4) void k_gemm_nt1(const GemmDesc *__restrict__
         acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
         acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
-    __syncthreads();
-    double *Ct = smem;  // Ct[col * 65 + row]
+    // columns 32 h .. 32 h + 31 per pass, from the two wavefronts that own them
+    double *Ct = smem;  // Ct[(col - 32 h) * 65 + row]
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int h = 0; h < 2; ++h) {
+        __syncthreads();  // the slices (h = 0) or the previous columns (h = 1) are read
+        if (wn == 32 * h) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                Ct[(wn + 16 * j + lr) * 65 + wm + 16 * i + 4 * r + lk] = acc[i][j][r];
-    __syncthreads();
+                for (int j = 0; j < 2; ++j)
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        const int cc = wave + 4 * t, gn = n0 + cc;
-        if (gm >= g.M || gn >= g.N || (g.uplo == 1 && gm < gn)) continue;
-        const double v = g.alpha * Ct[cc * 65 + lane];
-        gC[(unsigned)gn * uldc + gm] = (T)(g.beta == 0.0 ? v : v + g.beta * (double)craw[t]);
+                    for (int r = 0; r < 4; ++r)
+                        Ct[(16 * j + lr) * 65 + wm + 16 * i + 4 * r + lk] = acc[i][j][r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 8 * h; t < 8 * h + 8; ++t) {
+            const int cc = wave + 4 * t, gn = n0 + cc;
+            if (gm >= g.M || gn >= g.N || (g.uplo == 1 && gm < gn)) continue;
+            const double v = g.alpha * Ct[(cc - 32 * h) * 65 + lane];
+            gC[(unsigned)gn * uldc + gm] = (T)(g.beta == 0.0 ? v : v + g.beta * (double)craw[t]);
+        }
     }
 }
 
