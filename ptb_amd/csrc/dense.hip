@@ -261,12 +261,17 @@ void k_gemm(const GemmDesc *__restrict__ descs, int count, int tiles,
 // The slices stay in LDS in the storage type (float rows padded to 80
 // elements: the four k rows a wavefront reads at once fall on disjoint
 // banks), and the result goes out in two passes of 32 columns: 20 KB of LDS
-// with float storage, so that up to seven workgroups share a CU (these
-// updates are bound by the trailing matrix's traffic).
+// with float storage, so that up to eight workgroups share a CU (these
+// updates are bound by the trailing matrix's traffic).  KH < 32 stages the
+// slices in parts of KH k rows (less LDS); with FP64 storage that keeps the
+// later parts' operands live during the first part's products, which spills
+// registers at five wavefronts per SIMD, so FP64 slices go whole (four per
+// SIMD, LDS-bound).
 template <typename T>
 struct Nt1Shape {
     static constexpr int LD = sizeof(T) == 4 ? 80 : 68;  // slice row stride (elements of T)
-    static constexpr int SLICE = 2 * kKS * LD * (int)sizeof(T);
+    static constexpr int KH = kKS;  // k rows staged at a time
+    static constexpr int SLICE = 2 * KH * LD * (int)sizeof(T);
     static constexpr int EPI = 32 * 65 * (int)sizeof(double);
     static constexpr int BYTES = SLICE > EPI ? SLICE : EPI;
 };
@@ -275,7 +280,8 @@ template <typename T>
 __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 6 : 4) void k_gemm_nt1(
     const GemmDesc *__restrict__ descs, const GemmTile *__restrict__ work) {
     using S = Nt1Shape<T>;
-    constexpr int LD = S::LD;
+    constexpr int LD = S::LD, KH = S::KH;
+    constexpr int NQ = 8 * KH / kKS;  // elements per thread of each staged part
     const GemmTile wt = work[blockIdx.x];
     if (wt.g < 0) return;
     const GemmDesc g = descs[wt.g];
@@ -286,7 +292,7 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 6 : 4) void k_gemm_nt1(
     gmem<T> *gC = (gmem<T> *)(void *)g.C;
     __shared__ double smem[S::BYTES / sizeof(double)];
     T (*As)[LD] = reinterpret_cast<T (*)[LD]>(smem);
-    T (*Bs)[LD] = reinterpret_cast<T (*)[LD]>(reinterpret_cast<T *>(smem) + kKS * LD);
+    T (*Bs)[LD] = reinterpret_cast<T (*)[LD]>(reinterpret_cast<T *>(smem) + KH * LD);
     // wave-uniform wavefront index: the C column offsets stay scalar
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
@@ -323,28 +329,34 @@ __global__ __launch_bounds__(kT, sizeof(T) == 4 ? 6 : 4) void k_gemm_nt1(
     for (int q = 0; q < 8; ++q) asm volatile("" : "+v"(xa[q]), "+v"(xb[q]));
 #pragma unroll
     for (int t = 0; t < 16; ++t) asm volatile("" : "+v"(craw[t]));
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int e = threadIdx.x + q * kT, mm = e & 63, kk = e >> 6;
-        const bool kin = kk < g.K;
-        As[kk][mm] = (kin && m0 + mm < g.M) ? xa[q] : T(0);
-        Bs[kk][mm] = (kin && n0 + mm < g.N) ? xb[q] : T(0);
-    }
-    __syncthreads();
     d4_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    // k rows [KH h, KH h + KH) in part h (elements q of this thread have
+    // kk = thread / 64 + 4 q, so part h holds q in [NQ h, NQ h + NQ))
 #pragma unroll
-    for (int ks = 0; ks < kKS / 4; ++ks) {
-        const int kk = ks * 4 + lk;
-        const double a0 = As[kk][wm + lr], a1 = As[kk][wm + 16 + lr];
-        const double b0 = Bs[kk][wn + lr], b1 = Bs[kk][wn + 16 + lr];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    for (int h = 0; h < kKS / KH; ++h) {
+        if (h > 0) __syncthreads();  // the previous part is read
+#pragma unroll
+        for (int q = NQ * h; q < NQ * h + NQ; ++q) {
+            const int e = threadIdx.x + q * kT, mm = e & 63, kk = e >> 6;
+            const bool kin = kk < g.K;
+            As[kk - KH * h][mm] = (kin && m0 + mm < g.M) ? xa[q] : T(0);
+            Bs[kk - KH * h][mm] = (kin && n0 + mm < g.N) ? xb[q] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KH / 4; ++ks) {
+            const int kk = ks * 4 + lk;
+            const double a0 = As[kk][wm + lr], a1 = As[kk][wm + 16 + lr];
+            const double b0 = Bs[kk][wn + lr], b1 = Bs[kk][wn + 16 + lr];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
     }
     // columns 32 h .. 32 h + 31 per pass, from the two wavefronts that own them
     double *Ct = smem;  // Ct[(col - 32 h) * 65 + row]
