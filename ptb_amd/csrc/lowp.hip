@@ -71,15 +71,24 @@ __global__ __launch_bounds__(kT, 2) void k_gemm_bf(const GemmDesc *__restrict__ 
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
     T xa[8], xb[8];
-    // element q of this thread: op(A)(mm, kk), op(B)(kk, nn); the thread index
-    // runs along the operand's contiguous dimension in global memory
-    auto a_index = [&](int q, int &mm, int &kk) {
+    // Staging of each operand: where it is contiguous along m (n) in global
+    // memory, a wavefront instruction reads 64 consecutive rows and each
+    // thread takes eight consecutive k of its row (row = thread % 64, k =
+    // 8 (thread / 64) + q), so the eight split values go to LDS as one
+    // 16-byte store per half; where it is contiguous along k, an instruction
+    // reads 32 consecutive k of two rows (k = e % 32, row = e / 32, e =
+    // thread + 256 q) and each value is stored on its own (eight consecutive
+    // k per thread there would spread one instruction over 16 rows).
+    constexpr bool kPackA = !TA, kPackB = TB;
+    auto a_at = [&](int q, int &mm, int &kk) {
         const int e = threadIdx.x + q * kT;
-        if (!TA) { mm = e & 63; kk = e >> 6; } else { kk = e & 31; mm = e >> 5; }
+        if (kPackA) { mm = threadIdx.x & 63; kk = (threadIdx.x >> 6) * 8 + q; }
+        else { kk = e & 31; mm = e >> 5; }
     };
-    auto b_index = [&](int q, int &kk, int &nn) {
+    auto b_at = [&](int q, int &kk, int &nn) {
         const int e = threadIdx.x + q * kT;
-        if (!TB) { kk = e & 31; nn = e >> 5; } else { nn = e & 63; kk = e >> 6; }
+        if (kPackB) { nn = threadIdx.x & 63; kk = (threadIdx.x >> 6) * 8 + q; }
+        else { kk = e & 31; nn = e >> 5; }
     };
     const unsigned ulda = (unsigned)g.lda, uldb = (unsigned)g.ldb;
     // loads from clamped addresses, masked when staged (no load under a branch)
@@ -87,11 +96,11 @@ __global__ __launch_bounds__(kT, 2) void k_gemm_bf(const GemmDesc *__restrict__ 
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             int mm, kk;
-            a_index(q, mm, kk);
+            a_at(q, mm, kk);
             const unsigned cm = (unsigned)min(m0 + mm, g.M - 1), ck = (unsigned)min(k0 + kk, kend - 1);
             xa[q] = gA[TA ? cm * ulda + ck : ck * ulda + cm];
             int kb, nn;
-            b_index(q, kb, nn);
+            b_at(q, kb, nn);
             const unsigned cn = (unsigned)min(n0 + nn, g.N - 1), ckb = (unsigned)min(k0 + kb, kend - 1);
             xb[q] = gB[TB ? ckb * uldb + cn : cn * uldb + ckb];
         }
@@ -99,22 +108,42 @@ __global__ __launch_bounds__(kT, 2) void k_gemm_bf(const GemmDesc *__restrict__ 
     auto store = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) asm volatile("" : "+v"(xa[q]), "+v"(xb[q]));
+        bf16x8 pah, pal, pbh, pbl;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             int mm, kk, kb, nn;
-            a_index(q, mm, kk);
-            b_index(q, kb, nn);
+            a_at(q, mm, kk);
+            b_at(q, kb, nn);
             // split in double first: x - hi is exact there, and its rounding
             // to float before the second bf16 rounding loses nothing
             const double va = (m0 + mm < g.M && k0 + kk < kend) ? (double)xa[q] : 0.0;
             const double vb = (n0 + nn < g.N && k0 + kb < kend) ? (double)xb[q] : 0.0;
             const __bf16 ah = (__bf16)(float)va, bh = (__bf16)(float)vb;
-            Ah[mm * kLd + kk] = ah;
-            Bh[nn * kLd + kb] = bh;
-            if (NT > 1) {
-                Al[mm * kLd + kk] = (__bf16)(float)(va - (double)(float)ah);
-                Bl[nn * kLd + kb] = (__bf16)(float)(vb - (double)(float)bh);
+            const __bf16 al = (__bf16)(float)(va - (double)(float)ah);
+            const __bf16 bl = (__bf16)(float)(vb - (double)(float)bh);
+            if (kPackA) {
+                pah[q] = ah;
+                pal[q] = al;
+            } else {
+                Ah[mm * kLd + kk] = ah;
+                if (NT > 1) Al[mm * kLd + kk] = al;
             }
+            if (kPackB) {
+                pbh[q] = bh;
+                pbl[q] = bl;
+            } else {
+                Bh[nn * kLd + kb] = bh;
+                if (NT > 1) Bl[nn * kLd + kb] = bl;
+            }
+        }
+        const int prow = threadIdx.x & 63, pkb = (threadIdx.x >> 6) * 8;
+        if (kPackA) {
+            *reinterpret_cast<bf16x8 *>(Ah + prow * kLd + pkb) = pah;
+            if (NT > 1) *reinterpret_cast<bf16x8 *>(Al + prow * kLd + pkb) = pal;
+        }
+        if (kPackB) {
+            *reinterpret_cast<bf16x8 *>(Bh + prow * kLd + pkb) = pbh;
+            if (NT > 1) *reinterpret_cast<bf16x8 *>(Bl + prow * kLd + pkb) = pbl;
         }
     };
     if (kbeg < kend) {
