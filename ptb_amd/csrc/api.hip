@@ -1,6 +1,9 @@
 // C ABI entry points: context management, workspace, filter/analyse.
 #include <cstdarg>
+#include <algorithm>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "common.hpp"
 
@@ -96,6 +99,101 @@ void *hws(ptbgpu_ctx *ctx, const char *name, size_t bytes) {
     return b.ptr;
 }
 
+namespace {
+
+constexpr size_t kStageDirect = (size_t)8 << 20;  // smaller copies: one runtime copy
+constexpr size_t kStageChunk = (size_t)32 << 20;
+
+// dst[0, n) = src[0, n) on up to eight host threads (at least 4 MB each)
+void par_copy(char *dst, const char *src, size_t n) {
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(8, hw), std::max<size_t>(1, n >> 22));
+    if (nt <= 1) {
+        memcpy(dst, src, n);
+        return;
+    }
+    const size_t part = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt && t * part < n; ++t) {
+        const size_t b = t * part, len = std::min(part, n - b);
+        th.emplace_back([=] { memcpy(dst + b, src + b, len); });
+    }
+    memcpy(dst, src, std::min(part, n));
+    for (auto &x : th) x.join();
+}
+
+// the staging buffers, with their events created
+int stage_buffers(ptbgpu_ctx *ctx, char *(&stage)[2]) {
+    for (auto &e : ctx->stage_ev)
+        if (!e) PTB_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    stage[0] = static_cast<char *>(hws(ctx, "stage0", kStageChunk));
+    stage[1] = static_cast<char *>(hws(ctx, "stage1", kStageChunk));
+    return stage[0] && stage[1] ? PTBGPU_OK : PTBGPU_E_NOMEM;
+}
+
+// waits until staging buffer b is no longer read or written by a device copy
+int stage_wait(ptbgpu_ctx *ctx, int b) {
+    if (ctx->stage_busy[b]) PTB_HIP(ctx, hipEventSynchronize(ctx->stage_ev[b]));
+    ctx->stage_busy[b] = false;
+    return PTBGPU_OK;
+}
+
+}  // namespace
+
+int h2d(ptbgpu_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes < kStageDirect) {
+        PTB_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return PTBGPU_OK;
+    }
+    char *stage[2];
+    PTB_TRY(stage_buffers(ctx, stage));
+    size_t i = 0;
+    for (size_t off = 0; off < bytes; off += kStageChunk, ++i) {
+        const size_t n = std::min(kStageChunk, bytes - off);
+        const int b = (int)(i & 1);
+        PTB_TRY(stage_wait(ctx, b));
+        par_copy(stage[b], static_cast<const char *>(src) + off, n);
+        PTB_HIP(ctx, hipMemcpyAsync(static_cast<char *>(dst) + off, stage[b], n, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        PTB_HIP(ctx, hipEventRecord(ctx->stage_ev[b], ctx->stream));
+        ctx->stage_busy[b] = true;
+    }
+    return PTBGPU_OK;
+}
+
+int d2h(ptbgpu_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes < kStageDirect) {
+        PTB_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return PTBGPU_OK;
+    }
+    char *stage[2];
+    PTB_TRY(stage_buffers(ctx, stage));
+    // chunk held by each buffer, still to be copied out
+    size_t pend_off[2] = {0, 0}, pend_n[2] = {0, 0};
+    auto drain = [&](int b) -> int {
+        PTB_TRY(stage_wait(ctx, b));
+        if (pend_n[b]) par_copy(static_cast<char *>(dst) + pend_off[b], stage[b], pend_n[b]);
+        pend_n[b] = 0;
+        return PTBGPU_OK;
+    };
+    size_t i = 0;
+    for (size_t off = 0; off < bytes; off += kStageChunk, ++i) {
+        const size_t n = std::min(kStageChunk, bytes - off);
+        const int b = (int)(i & 1);
+        PTB_TRY(drain(b));  // the chunk two back (its copy-out overlaps the next DMA)
+        PTB_HIP(ctx, hipMemcpyAsync(stage[b], static_cast<const char *>(src) + off, n,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        PTB_HIP(ctx, hipEventRecord(ctx->stage_ev[b], ctx->stream));
+        ctx->stage_busy[b] = true;
+        pend_off[b] = off;
+        pend_n[b] = n;
+    }
+    PTB_TRY(drain((int)(i & 1)));  // the older of the last two chunks first
+    PTB_TRY(drain((int)((i + 1) & 1)));
+    return PTBGPU_OK;
+}
+
 }  // namespace ptb
 
 static void prof_clear(ptbgpu_ctx *ctx);
@@ -154,6 +252,8 @@ int ptbgpu_finalize(ptbgpu_ctx *ctx) {
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     for (auto &kv : ctx->host_bufs)
         if (kv.second.ptr) (void)hipHostFree(kv.second.ptr);
+    for (auto &e : ctx->stage_ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PTBGPU_OK;
@@ -304,11 +404,9 @@ int ptbgpu_filter_sym(ptbgpu_ctx *ctx, int64_t ndim, double *A, double filter,
     const size_t bytes = sizeof(double) * (size_t)(ndim * (ndim + 1) / 2);
     auto *d = static_cast<double *>(ptb::ws(ctx, "io_a", bytes));
     if (!d) return PTBGPU_E_NOMEM;
-    PTB_HIP(ctx, hipMemcpyAsync(d, A, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PTB_TRY(ptb::h2d(ctx, d, A, bytes));
     PTB_TRY(ptb::filter_count(ctx, d, ndim, filter, true, counts));
-    PTB_HIP(ctx, hipMemcpyAsync(A, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return PTBGPU_OK;
+    return ptb::d2h(ctx, A, d, bytes);
 }
 
 int ptbgpu_analyse_sym(ptbgpu_ctx *ctx, int64_t ndim, const double *A, int64_t *npos) {
@@ -322,7 +420,7 @@ int ptbgpu_analyse_sym(ptbgpu_ctx *ctx, int64_t ndim, const double *A, int64_t *
     const size_t bytes = sizeof(double) * (size_t)(ndim * (ndim + 1) / 2);
     auto *d = static_cast<double *>(ptb::ws(ctx, "io_a", bytes));
     if (!d) return PTBGPU_E_NOMEM;
-    PTB_HIP(ctx, hipMemcpyAsync(d, A, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PTB_TRY(ptb::h2d(ctx, d, A, bytes));
     int64_t c[2];
     PTB_TRY(ptb::filter_count(ctx, d, ndim, 0.0, false, c));
     if (npos) *npos = c[0];

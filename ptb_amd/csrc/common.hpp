@@ -66,6 +66,10 @@ struct ptbgpu_ctx {
     // key) and its chunk bounds
     std::vector<int64_t> chunk_key;
     std::vector<int> chunk_cbeg;
+    // the pinned staging buffers of large host copies: the event recorded
+    // after each buffer's last device copy, and whether one is pending
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_busy[2] = {false, false};
 };
 
 namespace ptb {
@@ -84,6 +88,14 @@ void ws_trim(ptbgpu_ctx *ctx, const char *name, size_t bytes);
 void *upload_table(ptbgpu_ctx *ctx, const char *name, const void *src, size_t bytes);
 // Named pinned host buffer of at least `bytes`.
 void *hws(ptbgpu_ctx *ctx, const char *name, size_t bytes);
+// Copies between pageable host memory and the device, on the context's
+// stream.  Large copies go in chunks through two pinned staging buffers: the
+// host side of each chunk is copied by several threads while the previous
+// chunk's DMA runs (the runtime stages pageable memory on one thread).  h2d
+// returns once the host memory may be reused (device copies may still be in
+// flight); d2h returns with the host memory written.
+int h2d(ptbgpu_ctx *ctx, void *dst, const void *src, size_t bytes);
+int d2h(ptbgpu_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 // Brackets the launches issued during its lifetime with events on the
 // context's stream when profiling is enabled (ptbgpu_profile).

@@ -255,9 +255,9 @@ int spectral_host(ptbgpu_ctx *ctx, int64_t n, const double *A, double *out, int 
     double *dA = static_cast<double *>(ptb::ws(ctx, "ms_in", mb));
     double *dO = static_cast<double *>(ptb::ws(ctx, "ms_out", mb));
     if (!dA || !dO) return PTBGPU_E_NOMEM;
-    PTB_HIP(ctx, hipMemcpy(dA, A, mb, hipMemcpyHostToDevice));
+    PTB_TRY(ptb::h2d(ctx, dA, A, mb));
     PTB_TRY(spectral_function(ctx, n, dA, dO, which));
-    PTB_HIP(ctx, hipMemcpy(out, dO, mb, hipMemcpyDeviceToHost));
+    PTB_TRY(ptb::d2h(ctx, out, dO, mb));
     return PTBGPU_OK;
 }
 
@@ -302,8 +302,8 @@ int ptbgpu_solve2(ptbgpu_ctx *ctx, int64_t mode, int64_t ndim, int64_t nel, int6
     int *dperm = static_cast<int *>(ptb::ws(ctx, "s2_perm", sizeof(int) * n));
     GemmDesc *dg = static_cast<GemmDesc *>(ptb::ws(ctx, "s2_g", sizeof(GemmDesc)));
     if (!dHp || !dSp || !dPp || !Uo || !T || !Pd || !fw || !dperm || !dg) return PTBGPU_E_NOMEM;
-    PTB_HIP(ctx, hipMemcpyAsync(dHp, H, pb, hipMemcpyHostToDevice, ctx->stream));
-    PTB_HIP(ctx, hipMemcpyAsync(dSp, S, pb, hipMemcpyHostToDevice, ctx->stream));
+    PTB_TRY(ptb::h2d(ctx, dHp, H, pb));
+    PTB_TRY(ptb::h2d(ctx, dSp, S, pb));
     hipLaunchKernelGGL(k_unpack_ld, dim3(grid_of(n)), dim3(kT), 0, ctx->stream, dHp, n, a.A,
                        (int64_t)a.ld);
     hipLaunchKernelGGL(k_unpack_ld, dim3(grid_of(n)), dim3(kT), 0, ctx->stream, dSp, n, a.B,
@@ -360,8 +360,8 @@ int ptbgpu_solve2(ptbgpu_ctx *ctx, int64_t mode, int64_t ndim, int64_t nel, int6
     PTB_TRY(ptb::gemm_vbatch(ctx, false, true, dg, 1, (int)n, (int)n, "dense"));
     hipLaunchKernelGGL(k_pack, dim3(grid_of(n)), dim3(kT), 0, ctx->stream, Pd, n, dPp);
     PTB_CHECK_LAUNCH(ctx);
-    PTB_HIP(ctx, hipMemcpyAsync(P, dPp, pb, hipMemcpyDeviceToHost, ctx->stream));
-    if (U) PTB_HIP(ctx, hipMemcpyAsync(U, Uo, mb, hipMemcpyDeviceToHost, ctx->stream));
+    PTB_TRY(ptb::d2h(ctx, P, dPp, pb));
+    if (U) PTB_TRY(ptb::d2h(ctx, U, Uo, mb));
     PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (e) std::copy(ev.begin(), ev.end(), e);
     if (focc) std::copy(oc.begin(), oc.end(), focc);

@@ -480,11 +480,10 @@ int ptbgpu_purify(ptbgpu_ctx *ctx, const ptbgpu_pur_settings *ps, int64_t n, con
     auto *dS = static_cast<double *>(ptb::ws(ctx, "io_s", bytes));
     auto *dP = static_cast<double *>(ptb::ws(ctx, "io_p", bytes));
     if (!dH || !dS || !dP) return PTBGPU_E_NOMEM;
-    PTB_HIP(ctx, hipMemcpyAsync(dH, H, bytes, hipMemcpyHostToDevice, ctx->stream));
-    PTB_HIP(ctx, hipMemcpyAsync(dS, S, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PTB_TRY(ptb::h2d(ctx, dH, H, bytes));
+    PTB_TRY(ptb::h2d(ctx, dS, S, bytes));
     PTB_TRY(ptbgpu_purify_dev(ctx, ps, n, dH, dS, dP, rep));
-    PTB_HIP(ctx, hipMemcpyAsync(P, dP, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    PTB_TRY(ptb::d2h(ctx, P, dP, bytes));
     return PTBGPU_OK;
 }
 

@@ -1280,8 +1280,8 @@ int ptbgpu_sm_density_batch(ptbgpu_ctx *ctx, int64_t nmol, const int64_t *ndim,
     std::vector<double *> pp(nmol);
     for (int64_t i = 0; i < nmol; ++i) {
         const size_t b = sizeof(double) * (off[i + 1] - off[i]);
-        PTB_HIP(ctx, hipMemcpyAsync(dH + off[i], H[i], b, hipMemcpyHostToDevice, ctx->stream));
-        PTB_HIP(ctx, hipMemcpyAsync(dS + off[i], S[i], b, hipMemcpyHostToDevice, ctx->stream));
+        PTB_TRY(ptb::h2d(ctx, dH + off[i], H[i], b));
+        PTB_TRY(ptb::h2d(ctx, dS + off[i], S[i], b));
         ph[i] = dH + off[i];
         ps[i] = dS + off[i];
         pp[i] = dP + off[i];
@@ -1295,8 +1295,7 @@ int ptbgpu_sm_density_batch(ptbgpu_ctx *ctx, int64_t nmol, const int64_t *ndim,
     if (st == PTBGPU_OK) {
         auto t1 = std::chrono::steady_clock::now();
         for (int64_t i = 0; i < nmol; ++i)
-            PTB_HIP(ctx, hipMemcpyAsync(P[i], dP + off[i], sizeof(double) * (off[i + 1] - off[i]),
-                                        hipMemcpyDeviceToHost, ctx->stream));
+            PTB_TRY(ptb::d2h(ctx, P[i], dP + off[i], sizeof(double) * (off[i + 1] - off[i])));
         PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
         const double t_down = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
         const double tot = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1356,8 +1355,8 @@ int ptbgpu_sm_density(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
     auto *dS = static_cast<double *>(ptb::ws(ctx, "io_s", bytes));
     auto *dP = static_cast<double *>(ptb::ws(ctx, "io_p", bytes));
     if (!dH || !dS || !dP) return PTBGPU_E_NOMEM;
-    PTB_HIP(ctx, hipMemcpyAsync(dH, H, bytes, hipMemcpyHostToDevice, ctx->stream));
-    PTB_HIP(ctx, hipMemcpyAsync(dS, S, bytes, hipMemcpyHostToDevice, ctx->stream));
+    PTB_TRY(ptb::h2d(ctx, dH, H, bytes));
+    PTB_TRY(ptb::h2d(ctx, dS, S, bytes));
     PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double t_up = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     ptbgpu_report r;
@@ -1365,8 +1364,7 @@ int ptbgpu_sm_density(ptbgpu_ctx *ctx, int64_t ndim, int64_t nelref, const doubl
                                    submatrix_mode, opts, &r);
     if (st == PTBGPU_OK) {
         auto t1 = std::chrono::steady_clock::now();
-        PTB_HIP(ctx, hipMemcpyAsync(P, dP, bytes, hipMemcpyDeviceToHost, ctx->stream));
-        PTB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        PTB_TRY(ptb::d2h(ctx, P, dP, bytes));
         r.t_phase[PTBGPU_T_DEALLOC] =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
     }
