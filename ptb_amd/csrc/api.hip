@@ -114,11 +114,18 @@ void par_copy(char *dst, const char *src, size_t n) {
     }
     const size_t part = (n + nt - 1) / nt;
     std::vector<std::thread> th;
-    for (size_t t = 1; t < nt && t * part < n; ++t) {
-        const size_t b = t * part, len = std::min(part, n - b);
-        th.emplace_back([=] { memcpy(dst + b, src + b, len); });
+    size_t done = part;  // the calling thread copies [0, part)
+    try {
+        for (size_t t = 1; t < nt && t * part < n; ++t) {
+            const size_t b = t * part, len = std::min(part, n - b);
+            th.emplace_back([=] { memcpy(dst + b, src + b, len); });
+            done = b + len;
+        }
+    } catch (...) {
+        // no thread could be started: what is not handed out is copied here
     }
     memcpy(dst, src, std::min(part, n));
+    if (done < n) memcpy(dst + done, src + done, n - done);
     for (auto &x : th) x.join();
 }
 
