@@ -625,29 +625,28 @@ __global__ __launch_bounds__(kT) void k_larft_all(const SubDesc *__restrict__ de
     __shared__ double Vs[64][33];
     __shared__ double G[32][33];
     __shared__ double Ts[32][33];
-    // pairs (i, j), j < i < kb, p = i (i - 1) / 2 + j; at most 496 < 2 * kT
-    const int npair = kb * (kb - 1) / 2;
-    int pi[2], pj[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int p = threadIdx.x + h * kT;
-        int i = 1;
-        while (p < npair && (i + 1) * i / 2 <= p) ++i;
-        pi[h] = i;
-        pj[h] = p < npair ? p - i * (i - 1) / 2 : 0;
-    }
-    double acc[2] = {0.0, 0.0};
-    // column c is zero above its unit at row b0 + c + 1, so every pair can run
-    // over rows b0 + 1 .. m - 1
-    for (int r0 = b0 + 1; r0 < d.m; r0 += 64) {
-        // the chunk's 8 elements per thread: every load issued from a clamped
-        // address, then masked (the empty asm keeps loads out of branches)
-        double x[64 * 32 / kT];
+    // G = V^T V on the matrix cores: wavefront w owns the 16 x 16 tile
+    // (rows 16 (w & 1), columns 16 (w >> 1)) as v_mfma_f64_16x16x4 over the
+    // block's rows, 64 at a time through LDS (lane l supplies V(k0 + l/16,
+    // i0 + l%16) and V(k0 + l/16, j0 + l%16); result value r of lane l is
+    // G(i0 + 4r + l/16, j0 + l%16)).  Column c is zero above its unit at row
+    // b0 + c + 1, so every element runs over rows b0 + 1 .. m - 1.  The next
+    // 64 rows are loaded into registers while the current ones run.
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i0 = (wave & 1) * 16, j0 = (wave >> 1) * 16;
+    const int lr = lane & 15, lk = lane >> 4;
+    d4_t acc = d4_t{0.0, 0.0, 0.0, 0.0};
+    double x[64 * 32 / kT];
+    // every load issued from a clamped address, then masked when staged (the
+    // empty asm keeps the loads out of the range tests)
+    auto load = [&](int r0) {
 #pragma unroll
         for (int u = 0; u < 64 * 32 / kT; ++u) {
             const int l = threadIdx.x + u * kT, rr = l & 63, cc = l >> 6;
             x[u] = V[(int64_t)min(cc, kb - 1) * d.ld + min(r0 + rr, d.m - 1)];
         }
+    };
+    auto stage = [&](int r0) {
 #pragma unroll
         for (int u = 0; u < 64 * 32 / kT; ++u) asm volatile("" : "+v"(x[u]));
 #pragma unroll
@@ -655,26 +654,29 @@ __global__ __launch_bounds__(kT) void k_larft_all(const SubDesc *__restrict__ de
             const int l = threadIdx.x + u * kT, rr = l & 63, cc = l >> 6;
             Vs[rr][cc] = (r0 + rr < d.m && cc < kb) ? x[u] : 0.0;
         }
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            if (threadIdx.x + h * kT < npair) {
-                double sacc = 0.0;
-#pragma unroll 8
-                for (int rr = 0; rr < 64; ++rr) sacc += Vs[rr][pi[h]] * Vs[rr][pj[h]];
-                acc[h] += sacc;
-            }
-        }
-        __syncthreads();
-    }
-    for (int l = threadIdx.x; l < 1024; l += kT) {
-        G[l & 31][l >> 5] = 0.0;
-        Ts[l & 31][l >> 5] = 0.0;
+    };
+    if (b0 + 1 < d.m) {
+        load(b0 + 1);
+        stage(b0 + 1);
     }
     __syncthreads();
+    for (int r0 = b0 + 1; r0 < d.m; r0 += 64) {
+        const bool more = r0 + 64 < d.m;
+        if (more) load(r0 + 64);
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
-        if (threadIdx.x + h * kT < npair) G[pi[h]][pj[h]] = acc[h];
+        for (int ks = 0; ks < 16; ++ks) {
+            const int kk = 4 * ks + lk;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[kk][i0 + lr], Vs[kk][j0 + lr], acc, 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            stage(r0 + 64);
+            __syncthreads();
+        }
+    }
+    for (int l = threadIdx.x; l < 1024; l += kT) Ts[l & 31][l >> 5] = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) G[i0 + 4 * r + lk][j0 + lr] = acc[r];
     __syncthreads();
     for (int i = 0; i < kb; ++i) {
         const int r = threadIdx.x;
@@ -719,17 +721,21 @@ __global__ __launch_bounds__(kT) void k_larft_merge(const SubDesc *__restrict__ 
         T1[r][c] = T1g[c * 32 + r];
         T2[r][c] = two ? Ta[toff[blockIdx.x] + (int64_t)(b + 1) * 1024 + c * 32 + r] : 0.0;
     }
-    // G = V1^T V2: thread owns entries (i, j) = (l & 31, l >> 5), l = tid + 256 h
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    // G = V1^T V2 on the matrix cores, 64 rows at a time through LDS with the
+    // next rows' loads in flight (tiles and lane layout as in k_larft_all)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i0 = (wave & 1) * 16, j0 = (wave >> 1) * 16;
+    const int lr = lane & 15, lk = lane >> 4;
+    d4_t acc = d4_t{0.0, 0.0, 0.0, 0.0};
     if (two) {
         // the second half holds kb2 reflectors (its columns past them are not
         // read: they may lie past the submatrix)
         const int kb2 = min(32, nr - (b + 1) * 32);
         const double *V1 = Va + d.off + (int64_t)(b * 32) * d.ld;
         const double *V2 = Va + d.off + (int64_t)((b + 1) * 32) * d.ld;
-        for (int r0 = (b + 1) * 32 + 1; r0 < d.m; r0 += 64) {
-            // 2 x 8 elements per thread, loads issued together (see k_larft_all)
-            double x1[64 * 32 / kT], x2[64 * 32 / kT];
+        // 2 x 8 elements per thread, loads issued together (see k_larft_all)
+        double x1[64 * 32 / kT], x2[64 * 32 / kT];
+        auto load = [&](int r0) {
 #pragma unroll
             for (int u = 0; u < 64 * 32 / kT; ++u) {
                 const int l = threadIdx.x + u * kT, rr = l & 63, cc = l >> 6;
@@ -737,9 +743,10 @@ __global__ __launch_bounds__(kT) void k_larft_merge(const SubDesc *__restrict__ 
                 x1[u] = V1[(int64_t)cc * d.ld + r];
                 x2[u] = V2[(int64_t)min(cc, kb2 - 1) * d.ld + r];
             }
+        };
+        auto stage = [&](int r0) {
 #pragma unroll
             for (int u = 0; u < 64 * 32 / kT; ++u) asm volatile("" : "+v"(x1[u]), "+v"(x2[u]));
-            __syncthreads();
 #pragma unroll
             for (int u = 0; u < 64 * 32 / kT; ++u) {
                 const int l = threadIdx.x + u * kT, rr = l & 63, cc = l >> 6;
@@ -747,23 +754,31 @@ __global__ __launch_bounds__(kT) void k_larft_merge(const SubDesc *__restrict__ 
                 V1s[rr][cc] = in ? x1[u] : 0.0;
                 V2s[rr][cc] = (in && cc < kb2) ? x2[u] : 0.0;
             }
-            __syncthreads();
+        };
+        const int rb = (b + 1) * 32 + 1;
+        if (rb < d.m) {
+            load(rb);
+            stage(rb);
+        }
+        __syncthreads();
+        for (int r0 = rb; r0 < d.m; r0 += 64) {
+            const bool more = r0 + 64 < d.m;
+            if (more) load(r0 + 64);
 #pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                const int l = threadIdx.x + h * kT, i = l & 31, j = l >> 5;
-                double sacc = 0.0;
-#pragma unroll 8
-                for (int rr = 0; rr < 64; ++rr) sacc += V1s[rr][i] * V2s[rr][j];
-                acc[h] += sacc;
+            for (int ks = 0; ks < 16; ++ks) {
+                const int kk = 4 * ks + lk;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(V1s[kk][i0 + lr], V2s[kk][j0 + lr], acc, 0, 0, 0);
+            }
+            __syncthreads();
+            if (more) {
+                stage(r0 + 64);
+                __syncthreads();
             }
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
-        const int l = threadIdx.x + h * kT;
-        G[l & 31][l >> 5] = acc[h];
-    }
+    for (int r = 0; r < 4; ++r) G[i0 + 4 * r + lk][j0 + lr] = acc[r];
     __syncthreads();
     // M = G T2 into V1s (32 x 32), then T12 = -T1 M
     double *M = &V1s[0][0];  // M(i, j) at M[i * 33 + j]
