@@ -135,7 +135,11 @@ int ptbgpu_stream(ptbgpu_ctx *ctx, void **stream);
  * refinement; 0 puts every submatrix on float storage (default 288).
  * "gemm_tile": output tile edge of the FP64 products, 64 or 128 (identical
  * results either way); 0 (default) chooses per launch: 128 only for launches
- * of fewer than 8 large products.
+ * of fewer than 8 large products.  "dc_global_min": merge size classes of at
+ * least this many rows run their secular roots and vectors over the whole
+ * chip from device scratch even when they fit in LDS (identical results;
+ * slower on MI355X, so 0, the default, keeps that path for the classes that
+ * do not fit).
  * "graph": 1 makes the construction and eigendecomposition phase of a build
  * (one chunk) replay a captured hipGraph once the same shapes and buffers
  * have been seen before (the first such build runs eagerly, the second

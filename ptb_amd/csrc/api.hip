@@ -279,6 +279,11 @@ int ptbgpu_set_tuning(ptbgpu_ctx *ctx, const char *key, int64_t value) {
         ctx->mixed_min_m = value;
         return PTBGPU_OK;
     }
+    if (k == "dc_global_min") {
+        if (value < 0 || value > (1 << 24)) return ptb::fail(ctx, PTBGPU_E_ARG, "dc_global_min out of range");
+        ctx->dc_global_min = value;
+        return PTBGPU_OK;
+    }
     if (k == "gemm_tile") {
         if (value != 0 && value != 64 && value != 128)
             return ptb::fail(ctx, PTBGPU_E_ARG, "gemm_tile must be 0, 64 or 128");
@@ -305,6 +310,10 @@ int ptbgpu_get_tuning(ptbgpu_ctx *ctx, const char *key, int64_t *value) {
     }
     if (k == "mixed_min_m") {
         *value = ctx->mixed_min_m;
+        return PTBGPU_OK;
+    }
+    if (k == "dc_global_min") {
+        *value = ctx->dc_global_min;
         return PTBGPU_OK;
     }
     if (k == "gemm_tile") {

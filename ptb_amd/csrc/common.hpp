@@ -48,6 +48,10 @@ struct ptbgpu_ctx {
     // less than the refinement costs, and small near-degenerate blocks leave
     // the refined density outside 1e-6 of FP64)
     int64_t mixed_min_m = 288;
+    // merge size classes (power-of-two node sizes) at or above this run their
+    // secular roots, Loewner vectors and eigenvectors over the whole chip from
+    // device scratch even when they fit in LDS (0: only those that do not)
+    int64_t dc_global_min = 0;
     // tile edge of the FP64 GEMM launches: 0 chooses per launch (128 x 128
     // tiles for launches with many large products), 64 or 128 forces it
     int64_t gemm_tile = 0;

@@ -767,7 +767,8 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
             const int i0 = runs[r].first;
             const int i1 = r + 1 < runs.size() ? runs[r + 1].first : count;
             const int np = runs[r].second;
-            if (in_lds(np)) {
+            const bool global = !in_lds(np) || (ctx->dc_global_min > 0 && np >= ctx->dc_global_min);
+            if (!global) {
                 const size_t shmem = (size_t)np * (6 * sizeof(double) + 4 * sizeof(int));
                 hipLaunchKernelGGL(k_dc_merge<false>, dim3(i1 - i0), dim3(kMT), shmem, ctx->stream,
                                    ddesc, dnodes + i0, np, e, D, Z, W, X, rot, dgemm + 2 * i0,

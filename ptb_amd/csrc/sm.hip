@@ -868,7 +868,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             // the addresses of every shared buffer key the graph itself
             mix(c.desc.data(), sizeof(SubDesc) * c.desc.size());
             const int64_t knobs[] = {(int64_t)lowp, (int64_t)mixed, ctx->dc_bucket_min,
-                                     ctx->mixed_min_m, ctx->gemm_tile};
+                                     ctx->mixed_min_m, ctx->gemm_tile, ctx->dc_global_min};
             mix(knobs, sizeof(knobs));
             char pfx[32];
             snprintf(pfx, sizeof(pfx), "g%016llx:", (unsigned long long)key);

@@ -1,4 +1,6 @@
-"""The FP64 product kernel at both tile edges (64 x 64 and 128 x 128 output
+"""Knobs that change how work is laid out but not its arithmetic.
+
+The FP64 product kernel at both tile edges (64 x 64 and 128 x 128 output
 tiles per workgroup).  Both add every output element's k terms in the same
 order, 4 at a time ascending, so forcing either edge (tuning knob
 "gemm_tile") must give bit-identical products, density matrices, dense
@@ -162,3 +164,26 @@ def test_dense_solve_and_purification_identical_at_both_edges(engine):
             engine.purification_wrapper(pur, mol.ndim, H.copy(), S.copy(), P)
         Ps[edge] = P
     assert np.array_equal(Ps[64], Ps[128])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_chip_wide_merges_identical(engine, precision):
+    """Tuning knob dc_global_min = 64: every divide-and-conquer merge class of
+    64 rows and up runs its roots, Loewner vectors and eigenvectors over the
+    whole chip from device scratch (the path of nodes too large for LDS)
+    instead of one workgroup per node in LDS.  Same device functions, same
+    order: the density is the same bit for bit."""
+    mol, H, S = small_system("water", (4, 4, 4), 3)
+    n = mol.ndim
+    Ps = {}
+    saved = engine.get_tuning("dc_global_min")
+    try:
+        for v in (0, 64):
+            engine.set_tuning("dc_global_min", v)
+            P = np.zeros_like(H)
+            engine.sm_stupid_simple(n, mol.nel, H, S, P, mol.n, mol.xyz, mol.aoat, 2, 2,
+                                    precision=precision)
+            Ps[v] = P
+    finally:
+        engine.set_tuning("dc_global_min", saved)
+    assert np.array_equal(Ps[0], Ps[64])
