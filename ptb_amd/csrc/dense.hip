@@ -876,8 +876,12 @@ __global__ __launch_bounds__(64) void k_diag_inv(const SubDesc *__restrict__ des
     for (int r = 0; r < 64; ++r) out[r] = x[r];
 }
 
-// Li = the block-diagonal of 64x64 inverses (from k_diag_inv), zero
-// elsewhere: the first level of the batched triangular inverse.
+// The first level of the batched triangular inverse: the 64x64 diagonal
+// blocks of Li are the inverses from k_diag_inv (zero above their diagonal),
+// and the 64x64 block above each diagonal block is zero.  Nothing else is
+// written: the level products write every block below the diagonal, and the
+// readers of Li take it as a triangular factor (kTri* k ranges), which reach
+// at most one 64-column block past the diagonal block with 128 x 128 tiles.
 // grid: (submatrix, 64-column block)
 __global__ __launch_bounds__(kT) void k_linv_init(const SubDesc *__restrict__ desc,
                                                   const int64_t *__restrict__ doff,
@@ -888,8 +892,9 @@ __global__ __launch_bounds__(kT) void k_linv_init(const SubDesc *__restrict__ de
     if (c0 >= d.m) return;
     const double *blk = dinv + doff[blockIdx.x] + (int64_t)b * 4096;
     const int nc = min(64, d.m - c0);
-    for (int l = threadIdx.x; l < nc * d.m; l += kT) {
-        const int c = l / d.m, r = l % d.m;
+    const int r0 = max(c0 - 64, 0), nr = min(c0 + 64, d.m) - r0;  // rows written
+    for (int l = threadIdx.x; l < nc * nr; l += kT) {
+        const int c = l / nr, r = r0 + l % nr;
         const int rr = r - c0;
         const double v = (rr >= c && rr < 64) ? blk[c * 64 + rr] : 0.0;
         Li[d.off + (int64_t)(c0 + c) * d.ld + r] = v;

@@ -47,9 +47,11 @@ __global__ __launch_bounds__(kT) void k_xall_build(const SubDesc *__restrict__ d
     const double *li = Lia + d.off + (int64_t)row * d.ld;
     double *xa = XA + xoff[blockIdx.x] + (int64_t)blockIdx.y * d.ld;
     double *xb = XA + xoff[blockIdx.x] + (int64_t)(c + blockIdx.y) * d.ld;
+    // (L^-1 is lower triangular: its column `row` is zero above the
+    // diagonal, where the inverse's arena is not written)
     for (int t = threadIdx.x; t < d.m; t += kT) {
         xa[t] = t <= row ? L[(int64_t)t * d.ld + row] : 0.0;
-        xb[t] = li[t];
+        xb[t] = t >= row ? li[t] : 0.0;
     }
 }
 
