@@ -1,6 +1,7 @@
 // C ABI entry points: context management, workspace, filter/analyse.
 #include <cstdarg>
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -244,6 +245,7 @@ int ptbgpu_init(ptbgpu_ctx **out, int64_t device, int64_t flags) {
     hipDeviceProp_t prop;
     PTB_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
     ctx->num_cus = prop.multiProcessorCount;
+    if (const char *pz = std::getenv("PTBGPU_POISON_ARENAS")) ctx->poison_arenas = pz[0] == '1';
     PTB_TRY(ptb::noalloc_scratch_init(ctx));
     *out = ctx;
     return PTBGPU_OK;
@@ -326,6 +328,10 @@ int ptbgpu_get_tuning(ptbgpu_ctx *ctx, const char *key, int64_t *value) {
     }
     if (k == "graphs") {  // instantiated graphs held by the context
         *value = (int64_t)ctx->graphs.size();
+        return PTBGPU_OK;
+    }
+    if (k == "poison_arenas") {  // PTBGPU_POISON_ARENAS at ptbgpu_init (read only)
+        *value = ctx->poison_arenas ? 1 : 0;
         return PTBGPU_OK;
     }
     if (k == "graph_failures") {  // captures or instantiations that did not succeed

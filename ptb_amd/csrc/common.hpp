@@ -74,6 +74,10 @@ struct ptbgpu_ctx {
     // after each buffer's last device copy, and whether one is pending
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     bool stage_busy[2] = {false, false};
+    // PTBGPU_POISON_ARENAS=1 at ptbgpu_init: the floating-point arenas of a
+    // density build are filled with NaN before each (eager) use, so that a
+    // read of an element the build did not write shows in the result (tests)
+    bool poison_arenas = false;
 };
 
 namespace ptb {

@@ -166,7 +166,9 @@ __global__ __launch_bounds__(kThreads) void k_gather(const MolDev *__restrict__ 
             const int a = a0 + lr, b = b0 + lc;
             if (a < d.m && b < d.m) {
                 Hd[d.off + (int64_t)b * d.ld + a] = h;
-                Sd[d.off + (int64_t)b * d.ld + a] = s;
+                // S: the lower triangle only (and the diagonal tiles whole):
+                // it is factored in place and read as a lower-triangular L
+                if (ta == tb) Sd[d.off + (int64_t)b * d.ld + a] = s;
             }
         }
         if (ta != tb) {

@@ -787,6 +787,18 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             c.R = static_cast<double *>(ptb::ws(ctx, "sm_R", abytes));
             if (!c.K32 || !c.V || !c.R) return PTBGPU_E_NOMEM;
         }
+        if (ctx->poison_arenas && !ctx->capturing) {  // all bytes 0xff: NaN as double and float
+            const size_t xbytes = sizeof(double) * (size_t)std::max<int64_t>(xtot, 1);
+            for (double *p : {c.A, c.B, c.Z, c.W, c.X, c.Li})
+                PTB_HIP(ctx, hipMemsetAsync(p, 0xff, abytes, st));
+            PTB_HIP(ctx, hipMemsetAsync(c.XA, 0xff, xbytes, st));
+            PTB_HIP(ctx, hipMemsetAsync(c.F, 0xff, xbytes, st));
+            if (mixed) {
+                PTB_HIP(ctx, hipMemsetAsync(c.K32, 0xff, sizeof(float) * (size_t)std::max<int64_t>(off, 1), st));
+                PTB_HIP(ctx, hipMemsetAsync(c.V, 0xff, abytes, st));
+                PTB_HIP(ctx, hipMemsetAsync(c.R, 0xff, abytes, st));
+            }
+        }
         return PTBGPU_OK;
     };
     // construction, eigendecomposition and fe of a chunk: eigenvalues into
