@@ -301,7 +301,9 @@ def run_batch(args):
            "config": {"workload": "batch256 (128 water clusters of 4-40 H2O at filter 1e-4, 128 "
                                   "CnH2n+2 n=3-40 at filter 1e-5), atoms_columns, submatrix_sygv",
                       "precision": main_prec, "mixed_min_m": eng.get_tuning("mixed_min_m"),
-                      "dc_bucket_min": eng.get_tuning("dc_bucket_min")},
+                      "dc_bucket_min": eng.get_tuning("dc_bucket_min"),
+                      "gemm_tile": eng.get_tuning("gemm_tile"),
+                      "dc_global_min": eng.get_tuning("dc_global_min")},
            "per_precision": res}
     if "fp64" in Ps and "mixed" in Ps:
         dp, de = 0.0, 0.0
@@ -509,7 +511,9 @@ def main():
                    "nsm": int(r["rep"]["nsm"]), "max_m": int(r["rep"]["max_m"]),
                    "sum_m3": float(r["rep"]["sum_m3"]), "precision": main_prec,
                    "mixed_min_m": eng.get_tuning("mixed_min_m"),
-                   "dc_bucket_min": eng.get_tuning("dc_bucket_min")},
+                   "dc_bucket_min": eng.get_tuning("dc_bucket_min"),
+                   "gemm_tile": eng.get_tuning("gemm_tile"),
+                   "dc_global_min": eng.get_tuning("dc_global_min")},
         "fp64_builds_per_s": round(1000.0 / results["fp64"]["ms_per_step"], 4) if "fp64" in results else None,
         "filter_ms_per_matrix": round(filt_ms / max(filt_n, 1), 4),
         "filter_kernel_ms_per_matrix": round(fk_ms / max(fk_n, 1), 4),
