@@ -54,13 +54,14 @@ enum ptbgpu_status {
 /* arithmetic of the per-submatrix solve */
 enum ptbgpu_precision {
     PTBGPU_FP64 = 0,       /* plain FP64 drop-in */
-    PTBGPU_MIXED = 1,      /* the tridiagonalisation on float storage, the divide-
-                              and-conquer merge products and the refinement's
-                              correction products on the BF16 matrix cores as
-                              split-BF16x3 with FP32 accumulation, then one FP64
-                              refinement of the occupied eigenvectors (Cholesky,
-                              K, the secular equations, fe, mu and the density
-                              stay FP64) */
+    PTBGPU_MIXED = 1,      /* the tridiagonalisation on float storage, then one
+                              FP64 refinement of the occupied eigenvectors whose
+                              correction products run on the BF16 matrix cores
+                              as split-BF16x3 with FP32 accumulation (the
+                              divide-and-conquer merge products too, for the
+                              submatrices up to the "lowp_merge_max_m" knob,
+                              0 by default; Cholesky, K, the secular
+                              equations, fe, mu and the density stay FP64) */
     PTBGPU_MIXED_BF16 = 2  /* the same with plain BF16 products (8-bit operands);
                               for accuracy studies of the precision ladder */
 };
@@ -139,7 +140,10 @@ int ptbgpu_stream(ptbgpu_ctx *ctx, void **stream);
  * least this many rows run their secular roots and vectors over the whole
  * chip from device scratch even when they fit in LDS (identical results;
  * slower on MI355X, so 0, the default, keeps that path for the classes that
- * do not fit).
+ * do not fit).  "lowp_merge_max_m": a mixed build's float-reduced
+ * submatrices of up to this many rows take their merge products in the
+ * low-precision arithmetic (then two Loewdin steps in the refinement instead
+ * of one); 0 (default) keeps FP64 merge products, as fast on MI355X.
  * "graph": 1 makes the construction and eigendecomposition phase of a build
  * (one chunk) replay a captured hipGraph once the same shapes and buffers
  * have been seen before (the first such build runs eagerly, the second

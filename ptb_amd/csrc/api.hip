@@ -281,6 +281,11 @@ int ptbgpu_set_tuning(ptbgpu_ctx *ctx, const char *key, int64_t value) {
         ctx->mixed_min_m = value;
         return PTBGPU_OK;
     }
+    if (k == "lowp_merge_max_m") {
+        if (value < 0 || value > 4096) return ptb::fail(ctx, PTBGPU_E_ARG, "lowp_merge_max_m out of range");
+        ctx->lowp_merge_max_m = value;
+        return PTBGPU_OK;
+    }
     if (k == "dc_global_min") {
         if (value < 0 || value > (1 << 24)) return ptb::fail(ctx, PTBGPU_E_ARG, "dc_global_min out of range");
         ctx->dc_global_min = value;
@@ -312,6 +317,10 @@ int ptbgpu_get_tuning(ptbgpu_ctx *ctx, const char *key, int64_t *value) {
     }
     if (k == "mixed_min_m") {
         *value = ctx->mixed_min_m;
+        return PTBGPU_OK;
+    }
+    if (k == "lowp_merge_max_m") {
+        *value = ctx->lowp_merge_max_m;
         return PTBGPU_OK;
     }
     if (k == "dc_global_min") {

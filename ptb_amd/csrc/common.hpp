@@ -48,6 +48,10 @@ struct ptbgpu_ctx {
     // less than the refinement costs, and small near-degenerate blocks leave
     // the refined density outside 1e-6 of FP64)
     int64_t mixed_min_m = 288;
+    // a mixed build's float-reduced submatrices of up to this many rows take
+    // their divide-and-conquer merge products in the low-precision
+    // arithmetic (split BF16x3, or BF16); 0: FP64 merge products everywhere
+    int64_t lowp_merge_max_m = 0;
     // merge size classes (power-of-two node sizes) at or above this run their
     // secular roots, Loewner vectors and eigenvectors over the whole chip from
     // device scratch even when they fit in LDS (0: only those that do not)

@@ -296,7 +296,7 @@ int kmeans_groups(ptbgpu_ctx *ctx, int64_t ndim, int64_t n, const double *xyz,
 int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                     const QFactor &qf, const double *K, double *Z, double *W, double *X, double *R,
                     double *eig, const double *bis, double *wocc, int *nocc, int *oidx,
-                    double *thn, int lowp) {
+                    double *thn, int lowp, int lowdin) {
     hipStream_t st = ctx->stream;
     const int nsm = (int)desc.size();
     int max_m = 0;
@@ -347,10 +347,12 @@ int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
     PTB_TRY(run("rf_g3", false, lowp, [&](const SubDesc &d, int n) {
         return GemmDesc{Z + d.off, X + d.off, W + d.off, d.m, n, d.m, d.ld, d.ld, d.ld, 1.0, 1.0, 0, 0};
     }));
-    // Loewdin orthonormalisation by two Newton-Schulz steps of the inverse
-    // square root, R = (3 I - W^T W) / 2, X = W R, then the same on X (each
-    // step squares the departure from orthonormality: the low-precision
-    // merge products leave the columns of Z orthonormal only to about 1e-5)
+    // Loewdin orthonormalisation by Newton-Schulz steps of the inverse
+    // square root, R = (3 I - W^T W) / 2, X = W R (then the same on X).  Each
+    // step squares the departure from orthonormality: one step after FP64
+    // merge products (W departs by the correction's square), two after
+    // low-precision ones (they leave the columns of Z orthonormal only to
+    // about 1e-5).
     PTB_TRY(run("rf_g4", true, kLowpNone, [&](const SubDesc &d, int n) {
         return GemmDesc{W + d.off, W + d.off, R + d.off, n, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
@@ -358,6 +360,7 @@ int refine_occupied(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
     PTB_TRY(run("rf_g5", false, kLowpNone, [&](const SubDesc &d, int n) {
         return GemmDesc{W + d.off, R + d.off, X + d.off, d.m, n, n, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
+    if (lowdin < 2) return ptb::refine_scatter(ctx, ddesc, nsm, max_n, nocc, oidx, X, thn, Z, eig);
     PTB_TRY(run("rf_g6", true, kLowpNone, [&](const SubDesc &d, int n) {
         return GemmDesc{X + d.off, X + d.off, R + d.off, n, n, d.m, d.ld, d.ld, d.ld, 1.0, 0.0, 0, 0};
     }));
@@ -724,6 +727,11 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     auto bisect_all = [&]() -> int {
         return ptb::bisect_batch(ctx, dvec, fe, drbeg, dnelref, nmol, max_rows, bis);
     };
+    // Loewdin steps of a refinement sweep: two when some float-reduced
+    // submatrix took low-precision merge products, else one
+    int lowdin = 1;
+    for (const auto &d : desc)
+        if (mixed && !(d.flags & kSubExact) && d.m <= ctx->lowp_merge_max_m) lowdin = 2;
     // the refinement sweeps correct the float-reduced submatrices
     const int sweeps = !mixed || opts.refine_steps < 0
                            ? 0
@@ -880,7 +888,8 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
             // the addresses of every shared buffer key the graph itself
             mix(c.desc.data(), sizeof(SubDesc) * c.desc.size());
             const int64_t knobs[] = {(int64_t)lowp, (int64_t)mixed, ctx->dc_bucket_min,
-                                     ctx->mixed_min_m, ctx->gemm_tile, ctx->dc_global_min};
+                                     ctx->mixed_min_m, ctx->gemm_tile, ctx->dc_global_min,
+                                     ctx->lowp_merge_max_m};
             mix(knobs, sizeof(knobs));
             char pfx[32];
             snprintf(pfx, sizeof(pfx), "g%016llx:", (unsigned long long)key);
@@ -931,7 +940,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
     auto refine_chunk = [&](Chunk &c, const double *mu_ref) -> int {
         for (int sweep = 0; sweep < sweeps; ++sweep)
             PTB_TRY(refine_occupied(ctx, c.desc, c.ddesc, c.qf, c.A, c.Z, c.W, c.X, c.R, dvec, mu_ref,
-                                    wocc, nocc + c.c0, oidx, thn, lowp));
+                                    wocc, nocc + c.c0, oidx, thn, lowp, lowdin));
         PTB_TRY(c.tf.launch(ctx, 0, true, false, c.dgf, "fe"));
         return ptb::fe_proj(ctx, c.ddesc, c.n, c.max_m, c.dxoff, c.F, fe);
     };
@@ -1153,18 +1162,21 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
     PTB_TRY(upload(ctx, "sm_leaves", leaves, &dleaves));
     PTB_TRY(dc_leaves(ctx, ddesc, dleaves, (int)leaves.size(), dvec, evec, Z));
     for (size_t lv = 0; lv < levels.size(); ++lv) {
-        // The low-precision merge products carry an error of about
-        // sqrt(n) 2^-17 per vector element at a node of n rows, and the tree's
-        // levels add up.  Submatrices of up to 1024 rows keep the error within
-        // what one FP64 refinement sweep removes; larger ones (the 64-water
-        // box as one block, the largest of the 2048-water box) keep FP64
-        // merge products, and so do the FP64-reduced ones (kSubExact), which
-        // no refinement follows.  Per level: the nodes of large submatrices first,
+        // Merge products in the low-precision arithmetic only for the
+        // float-reduced submatrices of up to lowp_merge_max_m rows (0 by
+        // default: on MI355X the FP64 products are as fast, and they leave
+        // the eigenvectors orthonormal to FP64 rounding).  The low-precision
+        // products carry an error of about sqrt(n) 2^-17 per vector element
+        // at a node of n rows, and the tree's levels add up; up to 1024 rows
+        // that stays within what one FP64 refinement sweep removes.  The
+        // FP64-reduced submatrices (kSubExact), which no refinement follows,
+        // always keep FP64 products.  Per level: the nodes of large submatrices first,
         // then the others, each group largest first (so that the level's last
         // workgroups are small); the two groups' products are two launches.
         auto &nodes = levels[lv];
         auto big = [&](const DcNode &nd) {
-            return lowp != kLowpNone && (desc[nd.s].m > 1024 || (desc[nd.s].flags & kSubExact));
+            return lowp != kLowpNone &&
+                   (desc[nd.s].m > ctx->lowp_merge_max_m || (desc[nd.s].flags & kSubExact));
         };
         std::stable_sort(nodes.begin(), nodes.end(), [&](const DcNode &a, const DcNode &b) {
             if (big(a) != big(b)) return big(a);

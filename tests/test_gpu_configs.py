@@ -138,9 +138,10 @@ def test_h2o64_full_density_matches_oracle(engine):
 
 @pytest.mark.parametrize("name", ["c2_c400h802", "c1_h2o512"])
 def test_precision_ladder(engine, name):
-    """The mixed build's accuracy per matrix-core precision, against the FP64
-    build: split-BF16x3 products (the default mixed build) with one FP64
-    refinement sweep meet max |dP| <= 1e-6 and |dE_band| <= 1e-6 Eh.  Plain
+    """The mixed build's accuracy per matrix-core precision of its
+    divide-and-conquer merge products, against the FP64 build: split-BF16x3
+    merge products with one FP64 refinement sweep meet max |dP| <= 1e-6 and
+    |dE_band| <= 1e-6 Eh.  Plain
     BF16 products (8-bit operands) in the divide-and-conquer merges leave the
     eigenvectors too far from orthogonal for a first-order refinement to
     recover: reported with one and three sweeps, and required to be far worse
@@ -152,10 +153,13 @@ def test_precision_ladder(engine, name):
         P64, _ = _build(engine, mol, H, S)
     e64 = engine.band_structure(n, P64, H)
     rows = {}
-    # every submatrix on float storage (mixed_min_m 0): the precision of the
-    # low-precision path itself, not of the size policy
+    # every submatrix on float storage (mixed_min_m 0) and its merge products
+    # in the low-precision arithmetic (lowp_merge_max_m 1024): the precision
+    # of the low-precision path itself, not of the size policy
     saved = engine.get_tuning("mixed_min_m")
+    saved_merge = engine.get_tuning("lowp_merge_max_m")
     engine.set_tuning("mixed_min_m", 0)
+    engine.set_tuning("lowp_merge_max_m", 1024)
     try:
         for label, prec, sweeps in (("bf16x3", "mixed", 1), ("bf16", "mixed_bf16", 1),
                                     ("bf16+3", "mixed_bf16", 3), ("none", "mixed", -1)):
@@ -165,6 +169,7 @@ def test_precision_ladder(engine, name):
                   f"|dE_band| {rows[label][1]:.3e} mu {rep['mu']}")
     finally:
         engine.set_tuning("mixed_min_m", saved)
+        engine.set_tuning("lowp_merge_max_m", saved_merge)
     assert rows["bf16x3"][0] <= 1e-6 and rows["bf16x3"][1] <= 1e-6
     assert rows["bf16"][0] > 100 * rows["bf16x3"][0]
     assert rows["none"][0] > rows["bf16x3"][0]    # the refinement does the work
