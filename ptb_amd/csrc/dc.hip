@@ -404,6 +404,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
             g[threadIdx.x].M = 0;
             g[threadIdx.x].N = 0;
             g[threadIdx.x].K = 0;
+            g[threadIdx.x].C32 = nullptr;
         }
         // the node block is diag(Z1, Z2): zero its off-diagonal blocks
         for (int c = wid; c < n; c += kWaves) {
@@ -634,6 +635,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         gg.beta = 0.0;
         gg.uplo = 0;
         gg.tri = 0;
+        gg.C32 = nullptr;
     }
     // a node from device scratch leaves the O(K^2) phases to three kernels
     // over the whole chip (k_dc_roots_big, k_dc_loewner_big, k_dc_vectors_big)

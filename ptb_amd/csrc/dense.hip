@@ -246,7 +246,9 @@ void k_gemm(const GemmDesc *__restrict__ descs, int count, int tiles,
                     continue;
                 auto *c = gC + (int64_t)gn * g.ldc + gm;
                 const double v = Ct[cc * CLD + rr + lane];
-                *c = (T)(g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * (double)*c);
+                const T r = (T)(g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * (double)*c);
+                *c = r;
+                if (g.C32) ((gmem<float> *)(void *)g.C32)[(int64_t)gn * g.ldc + gm] = (float)r;
             }
         }
     }

@@ -832,8 +832,11 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
                 const SubDesc &d = cd[s];
                 g1[s] = {c.Li + d.off, c.A + d.off, c.Z + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0,
                          0.0, 2, kTriALower};
+                // (a mixed build's float-reduced submatrices: the float copy
+                // of K's lower triangle in the same epilogue)
+                float *k32 = mixed && !(d.flags & kSubExact) ? c.K32 + d.off : nullptr;
                 g2[s] = {c.Z + d.off, c.Li + d.off, c.A + d.off, d.m, d.m, d.m, d.ld, d.ld, d.ld, 1.0,
-                         0.0, 1, kTriBUpper};
+                         0.0, 1, kTriBUpper, k32};
             }
             GemmDesc *dg1, *dg2;
             PTB_TRY(upload(ctx, "sm_gk1", g1, &dg1));
@@ -851,7 +854,7 @@ int sm_density_batch_device(ptbgpu_ctx *ctx, const std::vector<MolIn> &mols, int
         // reflectors in V, and K (FP64) kept in A for the refinement
         c.qf = QFactor{};
         PTB_TRY(ptb::eig_tri_batch(ctx, cd, c.ddesc, c.A, c.Z, c.W, c.X, dvec, evec, tau, rot, c.qf,
-                                   c.K32, c.V, lowp));
+                                   c.K32, c.V, lowp, true));
         if (timed) tm.mark(4, st);
         // ---- fe from the owned rows: F = Z^T Q^T [L(O,:)^T, L^-1 e_O]
         PTB_TRY(ptb::xall_build(ctx, c.ddesc, n, c.max_own, doloc, c.B, c.Li, c.dxoff, c.XA));
@@ -1103,7 +1106,8 @@ namespace ptb {
 // vectors and qf the blocked form of Q (apply_q multiplies by Q or Q^T).
 int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
-                  double *tau, double *rot, QFactor &qf, float *K32, double *V, int lowp) {
+                  double *tau, double *rot, QFactor &qf, float *K32, double *V, int lowp,
+                  bool k32_ready) {
     hipStream_t st = ctx->stream;
     const int nsm = (int)desc.size();
     int max_m = 0;
@@ -1135,7 +1139,7 @@ int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDe
         }
         if (!lo.empty()) {
             const int n = (int)lo.size();
-            PTB_TRY(to_f32_batch(ctx, dlo, n, lo_m, A, K32));
+            if (!k32_ready) PTB_TRY(to_f32_batch(ctx, dlo, n, lo_m, A, K32));
             PTB_TRY(sytrd_batch_f32(ctx, lo, dlo, K32, dvec, evec, tau));
             PTB_TRY(promote_reflectors(ctx, dlo, n, lo_m, K32, V, tau));
         }

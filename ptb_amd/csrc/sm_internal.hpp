@@ -59,6 +59,9 @@ struct GemmDesc {
     double alpha, beta;
     int uplo;
     int tri;
+    // optional (k_gemm only): every element written to C also goes, rounded
+    // to float, to C32 at the same offset (the mixed build's float copy of K)
+    float *C32;
 };
 
 constexpr int kTriALower = 1;  // op(A)(i, k) = 0 for k > i
@@ -215,10 +218,12 @@ int sytrd_batch_f32(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const Sub
 // With K32 and V given, the tridiagonalisation runs on float storage (K32) and
 // the FP64 reflectors go to V; A (the FP64 matrix) is left intact.
 // lowp: arithmetic of the divide-and-conquer merge products (kLowp*)
+// k32_ready: the float copy of K's lower triangle is already in K32 (written
+// by the product that formed K)
 int eig_tri_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
                   double *tau, double *rot, QFactor &qf, float *K32 = nullptr,
-                  double *V = nullptr, int lowp = 0);
+                  double *V = nullptr, int lowp = 0, bool k32_ready = false);
 int eig_sym_batch(ptbgpu_ctx *ctx, const std::vector<SubDesc> &desc, const SubDesc *ddesc,
                   double *A, double *Z, double *W, double *X, double *dvec, double *evec,
                   double *tau, double *rot);
