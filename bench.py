@@ -130,31 +130,42 @@ def cpu_info():
     return {"cpu_model": model, "cores_available": avail, "blas": blas}
 
 
-def gemm_site_flops(site, ms, cs, nos, mixed):
+def gemm_site_flops(site, ms, cs, nos, mixed, thr=0, lowp_max=0):
     """Algorithmic flops of one build's products at a GEMM site, from the
-    submatrix dimensions m, owned columns c and occupied states n_o."""
+    submatrix dimensions m, owned columns c and occupied states n_o, as a pair
+    (FP64 flops, split-BF16x3 flops).  In the mixed build the submatrices at
+    or above `thr` (mixed_min_m) are refined; of those, the ones up to
+    `lowp_max` rows (lowp_merge_max_m) take split-BF16x3 merge products and a
+    second Loewdin step.  The refinement's correction products (Z^T R and
+    Z E) are split-BF16x3, its residual and Loewdin products FP64."""
     m = np.asarray(ms, dtype=np.float64)
     c = np.asarray(cs, dtype=np.float64)
     no = np.asarray(nos, dtype=np.float64)
+    refined = (m >= thr) if mixed else np.zeros(m.shape, dtype=bool)
+    lowp = refined & (m <= lowp_max)
     if site == "gemm_potrf":
-        return float(np.sum(m ** 3) / 3.0)
+        return float(np.sum(m ** 3) / 3.0), 0.0
     if site == "gemm_linv":
-        return float(np.sum(m ** 3) / 3.0)
+        return float(np.sum(m ** 3) / 3.0), 0.0
     if site == "gemm_kform":
-        return float(np.sum(m ** 3))
+        return float(np.sum(m ** 3)), 0.0
     if site == "gemm_sytrd":
-        return float(np.sum(2.0 * m ** 3 / 3.0))
+        return float(np.sum(2.0 * m ** 3 / 3.0)), 0.0
     if site == "gemm_dc":
-        return float(np.sum(2.0 * m ** 3 / 3.0))
+        f = 2.0 * m ** 3 / 3.0
+        return float(np.sum(f[~lowp])), float(np.sum(f[lowp]))
     if site == "gemm_u":
-        return float(np.sum(m * m * no))
+        return float(np.sum(m * m * no)), 0.0
     if site == "gemm_fe":
-        return float(np.sum(2.0 * m * m * 2.0 * c)) * (2.0 if mixed else 1.0)
+        return float(np.sum(2.0 * m * m * 2.0 * c)) * (2.0 if mixed else 1.0), 0.0
     if site == "gemm_q":
         f = np.sum(4.0 * m * m * (2.0 * c + no))
-        return float(f + (np.sum(8.0 * m * m * no) if mixed else 0.0))
+        return float(f + np.sum((8.0 * m * m * no)[refined])), 0.0
     if site == "gemm_refine":
-        return float(np.sum(3 * 2.0 * m * m * no + 2 * 2.0 * m * no * no))
+        steps = 2.0 if lowp.any() else 1.0
+        fp = (2.0 * m * m * no + steps * 2 * 2.0 * m * no * no)[refined]
+        bf = (2 * 2.0 * m * m * no)[refined]
+        return float(np.sum(fp)), float(np.sum(bf))
     return None
 
 
@@ -445,30 +456,35 @@ def main():
                 "ms_per_launch": round(per_launch_ms, 3), "share_of_build": round(ms / r["ms_per_step"], 3)}
         break
     # per-site rooflines of the main build: every GEMM site against the peak of
-    # its arithmetic (FP64 matrix cores, or the BF16 matrix cores for the
-    # split-BF16x3 sites of the mixed build), the HBM-bound kernels in GB/s
+    # its mix of arithmetic (the FP64 matrix cores for FP64 products, a third
+    # of the BF16 peak for split-BF16x3 ones), the HBM-bound kernels in GB/s
     starts = np.r_[0, np.flatnonzero(np.diff(mol.aoat)) + 1, n]
     cs = np.diff(starts) if wl["columns"] == 2 else None
     sites = []
     if cs is not None and len(cs) == len(r["m"]):
         _, eigs, _ = eng.sm_spectra(r["rep"]["nsm"], r["rep"]["sum_m"])
         nos = np.array([int(np.sum(e < r["rep"]["mu"])) for e in eigs])
-        lowp_sites = {"gemm_dc"} if main_prec == "mixed" else set()
+        bf_peak = BF16_PEAK_TFS / 3.0
         for site, (ms, cnt) in sorted(r["kernels"].items(), key=lambda kv: -kv[1][0]):
-            fl = gemm_site_flops(site, r["m"], cs, nos, main_prec == "mixed")
+            fl = gemm_site_flops(site, r["m"], cs, nos, main_prec == "mixed", thr,
+                                 eng.get_tuning("lowp_merge_max_m"))
             if fl is None or ms <= 0:
                 continue
-            tf = fl / (ms * 1e-3) / 1e12
-            if site in lowp_sites:
-                sites.append({"site": site, "bound": "mfma", "arith": "bf16x3 (3 bf16 products)",
-                              "algorithmic_tflop": round(fl / 1e12, 4), "ms": round(ms, 3),
-                              "achieved": round(tf, 2), "peak": round(BF16_PEAK_TFS / 3.0, 1),
-                              "unit": "TFLOP/s", "frac": round(tf / (BF16_PEAK_TFS / 3.0), 4)})
+            fp, bf = fl
+            tf = (fp + bf) / (ms * 1e-3) / 1e12
+            # the site's peak for its mix of arithmetic: its flops over the
+            # least time the two kinds of matrix cores need for them
+            peak = (fp + bf) / (fp / FP64_PEAK_TFS + bf / bf_peak)
+            if bf == 0:
+                arith = "fp64"
+            elif fp == 0:
+                arith = "bf16x3 (3 bf16 products)"
             else:
-                sites.append({"site": site, "bound": "mfma", "arith": "fp64",
-                              "algorithmic_tflop": round(fl / 1e12, 4), "ms": round(ms, 3),
-                              "achieved": round(tf, 2), "peak": FP64_PEAK_TFS,
-                              "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFS, 4)})
+                arith = f"fp64 + bf16x3 ({100.0 * bf / (fp + bf):.0f} % of the flops)"
+            sites.append({"site": site, "bound": "mfma", "arith": arith,
+                          "algorithmic_tflop": round((fp + bf) / 1e12, 4), "ms": round(ms, 3),
+                          "achieved": round(tf, 2), "peak": round(peak, 1),
+                          "unit": "TFLOP/s", "frac": round(tf / peak, 4)})
         for name in ("sytrd_panel_f32", "sytrd_panel", "gather", "pattern"):
             if name in r["kernels"]:
                 ms, cnt = r["kernels"][name]
