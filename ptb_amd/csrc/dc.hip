@@ -108,12 +108,14 @@ __global__ __launch_bounds__(64) void k_dc_leaf(const SubDesc *__restrict__ desc
         if (lane < n) Z[(int64_t)(nd.lo + c) * ds.ld + nd.lo + lane] = z[lane][c];
 }
 
-// bitonic sort of (key, idx) pairs in LDS, ascending, ties by idx
+// bitonic sort of (key, idx) pairs in LDS, ascending, ties by idx, by a
+// workgroup of NT threads
+template <int NT>
 __device__ void bitonic_sort(double *key, int *idx, int npow) {
     for (int size = 2; size <= npow; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
             __syncthreads();
-            for (int t = threadIdx.x; t < npow / 2; t += kMT) {
+            for (int t = threadIdx.x; t < npow / 2; t += NT) {
                 const int lo = 2 * stride * (t / stride) + (t % stride);
                 const int hi = lo + stride;
                 const bool up = ((lo & size) == 0);
@@ -132,6 +134,7 @@ __device__ void bitonic_sort(double *key, int *idx, int npow) {
     __syncthreads();
 }
 
+template <int NT>
 __device__ __forceinline__ double block_max(double v, double *red) {
     v = wave_max(v);
     const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
@@ -139,7 +142,7 @@ __device__ __forceinline__ double block_max(double v, double *red) {
     if (lane == 0) red[wid] = v;
     __syncthreads();
     double t = red[0];
-    for (int w = 1; w < kMT / PTB_WAVE; ++w) t = fmax(t, red[w]);
+    for (int w = 1; w < NT / PTB_WAVE; ++w) t = fmax(t, red[w]);
     __syncthreads();
     return t;
 }
@@ -318,8 +321,9 @@ __device__ void dc_vector(int j, int lane, int K, const double *dl, const int *o
     if (lane == 0) Dn[j] = dorg + tj;
 }
 
-// exclusive prefix sum of one int per thread over the workgroup; *total gets
-// the sum of all (wsum: kMT / PTB_WAVE ints of LDS)
+// exclusive prefix sum of one int per thread over the workgroup of NT
+// threads; *total gets the sum of all (wsum: NT / PTB_WAVE ints of LDS)
+template <int NT>
 __device__ int block_scan_excl(int v, int *wsum, int *total) {
     const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
     int inc = v;
@@ -331,7 +335,7 @@ __device__ int block_scan_excl(int v, int *wsum, int *total) {
     if (lane == PTB_WAVE - 1) wsum[wid] = inc;
     __syncthreads();
     int base = 0, tot = 0;
-    for (int w = 0; w < kMT / PTB_WAVE; ++w) {
+    for (int w = 0; w < NT / PTB_WAVE; ++w) {
         if (w < wid) base += wsum[w];
         tot += wsum[w];
     }
@@ -357,8 +361,8 @@ __device__ int block_scan_excl(int v, int *wsum, int *total) {
 // two npow) live in LDS up to npow = 2048; larger nodes (kGlobal) keep them
 // in a device scratch of 8 npow doubles per node at gscr, which only this
 // workgroup touches (its barriers order the accesses as they do in LDS).
-template <bool kGlobal>
-__global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ desc,
+template <bool kGlobal, int NT>
+__global__ __launch_bounds__(NT) void k_dc_merge(const SubDesc *__restrict__ desc,
                                                   const DcNode *__restrict__ nodes, int npow,
                                                   const double *__restrict__ e_all,
                                                   double *__restrict__ D_all,
@@ -384,8 +388,8 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     int *slot = ix + npow;          // n output slot of sorted position
     int *org = slot + npow;         // K sorted position of each pole, then origin pole of each root
     int *prow = org + npow;         // K W slot (and U row) of each pole
-    __shared__ double red[kMT / PTB_WAVE];
-    __shared__ int wsum[kMT / PTB_WAVE];
+    __shared__ double red[NT / PTB_WAVE];
+    __shared__ int wsum[NT / PTB_WAVE];
     __shared__ int sK, sNr, sClose, sKL, sKD;
 
     const double beta = e_all[ds.mapoff + nd.sp - 1];
@@ -396,7 +400,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     double *rot = rot_all + 4 * (int64_t)(ds.mapoff + nd.lo);
     GemmDesc *g = gd + 2 * blockIdx.x;
     const int lane = threadIdx.x % PTB_WAVE, wid = threadIdx.x / PTB_WAVE;
-    constexpr int kWaves = kMT / PTB_WAVE;
+    constexpr int kWaves = NT / PTB_WAVE;
 
     if (rho == 0.0) {  // decoupled: children are already eigenpairs of T
         if (kGlobal && threadIdx.x == 0) gK[blockIdx.x] = 0;
@@ -420,7 +424,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     // so the n real entries always sort ahead of the padding and the sorted
     // positions 0..n-1 name only columns of this node, whatever the data (a
     // NaN key would otherwise leave the order undefined).
-    for (int i = threadIdx.x; i < npow; i += kMT) {
+    for (int i = threadIdx.x; i < npow; i += NT) {
         if (i < n) {
             const double dv = D[nd.lo + i];
             Dk[i] = dv == dv ? dv : HUGE_VAL;
@@ -434,12 +438,12 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
             ix[i] = i;
         }
     }
-    bitonic_sort(Dk, ix, npow);
-    for (int j = threadIdx.x; j < n; j += kMT) z[j] = zraw[ix[j]];
+    bitonic_sort<NT>(Dk, ix, npow);
+    for (int j = threadIdx.x; j < n; j += NT) z[j] = zraw[ix[j]];
     __syncthreads();
     double mx = 0.0;
-    for (int j = threadIdx.x; j < n; j += kMT) mx = fmax(mx, fmax(fabs(Dk[j]), fabs(z[j])));
-    const double tol = 8.0 * kEps * block_max(mx, red);
+    for (int j = threadIdx.x; j < n; j += NT) mx = fmax(mx, fmax(fabs(Dk[j]), fabs(z[j])));
+    const double tol = 8.0 * kEps * block_max<NT>(mx, red);
 
     // Deflation.  Type 1: rho |z_j| <= tol.  Type 2: neighbouring surviving
     // poles so close that a Givens rotation zeroes one weight (|t c s| <= tol);
@@ -450,12 +454,12 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     // survivor j gets slot = survivors before j, deflated j gets
     // slot = #survivors + deflated before j (the scan's order).
     {
-        const int chunk = (n + kMT - 1) / kMT;
+        const int chunk = (n + NT - 1) / NT;
         const int jb = min(n, (int)threadIdx.x * chunk), je = min(n, jb + chunk);
         int cnt = 0;
         for (int j = jb; j < je; ++j) cnt += (rho * fabs(z[j]) > tol);
         int nc;
-        int pos = block_scan_excl(cnt, wsum, &nc);
+        int pos = block_scan_excl<NT>(cnt, wsum, &nc);
         if (threadIdx.x == 0) sClose = 0;
         for (int j = jb; j < je; ++j) {
             if (rho * fabs(z[j]) > tol) {
@@ -466,7 +470,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
             }
         }
         __syncthreads();
-        for (int q = threadIdx.x + 1; q < nc; q += kMT) {
+        for (int q = threadIdx.x + 1; q < nc; q += NT) {
             const int pj = org[q - 1], j = org[q];
             double s = z[pj], c = z[j];
             const double tv = hypot(c, s);
@@ -477,7 +481,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
         }
         __syncthreads();
         if (!sClose) {
-            for (int q = threadIdx.x; q < nc; q += kMT) {
+            for (int q = threadIdx.x; q < nc; q += NT) {
                 dl[q] = Dk[org[q]];
                 w[q] = z[org[q]];
             }
@@ -543,7 +547,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     // child of origin and the rotations in order; bit 2 = touched by a
     // rotation (zh is free until the roots).
     int *cmask = reinterpret_cast<int *>(zh);
-    for (int j = threadIdx.x; j < n; j += kMT) cmask[j] = ix[j] < n1 ? 1 : 2;
+    for (int j = threadIdx.x; j < n; j += NT) cmask[j] = ix[j] < n1 ? 1 : 2;
     __syncthreads();
     if (nr > 0 && threadIdx.x == 0)
         for (int q = 0; q < nr; ++q) {
@@ -555,7 +559,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     __syncthreads();
     // slots [left-only | dense | right-only] by prefix counts over the poles
     {
-        const int chunk = (K + kMT - 1) / kMT;
+        const int chunk = (K + NT - 1) / NT;
         const int ib = min(K, (int)threadIdx.x * chunk), ie = min(K, ib + chunk);
         int cl = 0, cd = 0;
         for (int i = ib; i < ie; ++i) {
@@ -564,8 +568,8 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
             cd += t == 3;
         }
         int KL, KD;
-        int pl = block_scan_excl(cl, wsum, &KL);
-        int pd = block_scan_excl(cd, wsum, &KD);
+        int pl = block_scan_excl<NT>(cl, wsum, &KL);
+        int pd = block_scan_excl<NT>(cd, wsum, &KD);
         int pr = ib - pl - pd;  // right-only poles before ib
         for (int i = ib; i < ie; ++i) {
             const int t = cmask[org[i]] & 3;
@@ -576,7 +580,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
             sKD = KD;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < K; i += kMT) slot[org[i]] = prow[i];
+        for (int i = threadIdx.x; i < K; i += NT) slot[org[i]] = prow[i];
         __syncthreads();
     }
     const int KL = sKL, KD = sKD;
@@ -596,7 +600,7 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     }
     __syncthreads();
     // deflation rotations, in order, row-parallel
-    for (int r = threadIdx.x; r < n; r += kMT) {
+    for (int r = threadIdx.x; r < n; r += NT) {
         for (int q = 0; q < nr; ++q) {
             const int cp = nd.lo + slot[(int)rot[4 * q]], cj = nd.lo + slot[(int)rot[4 * q + 1]];
             const double c = rot[4 * q + 2], s = rot[4 * q + 3];
@@ -611,11 +615,11 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     for (int c = K + wid; c < n; c += kWaves)
         copy_column(Z + (int64_t)(nd.lo + c) * ld + nd.lo, W + (int64_t)(nd.lo + c) * ld + nd.lo,
                     0, n, lane);
-    for (int j = threadIdx.x; j < n; j += kMT)
+    for (int j = threadIdx.x; j < n; j += NT)
         if (slot[j] >= K) D[nd.lo + slot[j]] = Dk[j];
     __syncthreads();
     double *w2 = zh;
-    for (int i = threadIdx.x; i < K; i += kMT) w2[i] = rho * w[i] * w[i];
+    for (int i = threadIdx.x; i < K; i += NT) w2[i] = rho * w[i] * w[i];
     if (threadIdx.x < 2) {
         // 0: rows above the split from slots [0, KL + KD); 1: rows below it
         // from slots [KL, K)
@@ -645,10 +649,10 @@ __global__ __launch_bounds__(kMT) void k_dc_merge(const SubDesc *__restrict__ de
     }
     __syncthreads();
     // secular roots, one thread per root
-    for (int j = threadIdx.x; j < K; j += kMT) dc_root(j, K, dl, w2, w, rho, org, tau);
+    for (int j = threadIdx.x; j < K; j += NT) dc_root(j, K, dl, w2, w, rho, org, tau);
     __syncthreads();
     // Loewner vector: rho zhat_i^2 = (lam_i - d_i) prod_{j != i} (lam_j - d_i)/(d_j - d_i)
-    for (int i = threadIdx.x; i < K; i += kMT) zh[i] = dc_loewner(i, K, dl, org, tau, w, rho);
+    for (int i = threadIdx.x; i < K; i += NT) zh[i] = dc_loewner(i, K, dl, org, tau, w, rho);
     __syncthreads();
     // U(:, j) = zhat ./ (d - lam_j), normalised; eigenvalues lam_j.  One
     // column per wavefront: its norm, then the column (lanes on rows).
@@ -771,17 +775,22 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
             const int np = runs[r].second;
             const bool global = !in_lds(np) || (ctx->dc_global_min > 0 && np >= ctx->dc_global_min);
             if (!global) {
+                // a workgroup of min(max(np, 64), 256) threads per node: the
+                // small nodes of the low levels leave no thread of a larger
+                // group idle at every barrier, and more of them fit per CU
                 const size_t shmem = (size_t)np * (6 * sizeof(double) + 4 * sizeof(int));
-                hipLaunchKernelGGL(k_dc_merge<false>, dim3(i1 - i0), dim3(kMT), shmem, ctx->stream,
-                                   ddesc, dnodes + i0, np, e, D, Z, W, X, rot, dgemm + 2 * i0,
-                                   nullptr, nullptr);
+                auto k = np <= 64 ? k_dc_merge<false, 64>
+                                  : (np <= 128 ? k_dc_merge<false, 128> : k_dc_merge<false, kMT>);
+                const int nt = np <= 64 ? 64 : (np <= 128 ? 128 : kMT);
+                hipLaunchKernelGGL(k, dim3(i1 - i0), dim3(nt), shmem, ctx->stream, ddesc, dnodes + i0,
+                                   np, e, D, Z, W, X, rot, dgemm + 2 * i0, nullptr, nullptr);
             } else {
                 const int nn = i1 - i0;
                 auto *scr = static_cast<double *>(
                     ws(ctx, "dc_gscr", sizeof(double) * 8 * (size_t)np * (size_t)nn));
                 auto *gK = static_cast<int *>(ws(ctx, "dc_gK", sizeof(int) * (size_t)nn));
                 if (!scr || !gK) return PTBGPU_E_NOMEM;
-                hipLaunchKernelGGL(k_dc_merge<true>, dim3(nn), dim3(kMT), 0, ctx->stream, ddesc,
+                hipLaunchKernelGGL((k_dc_merge<true, kMT>), dim3(nn), dim3(kMT), 0, ctx->stream, ddesc,
                                    dnodes + i0, np, e, D, Z, W, X, rot, dgemm + 2 * i0, scr, gK);
                 const unsigned nb = (unsigned)((np + kMT - 1) / kMT);
                 hipLaunchKernelGGL(k_dc_roots_big, dim3(nb, nn), dim3(kMT), 0, ctx->stream, ddesc,
