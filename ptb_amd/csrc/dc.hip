@@ -775,13 +775,15 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
             const int np = runs[r].second;
             const bool global = !in_lds(np) || (ctx->dc_global_min > 0 && np >= ctx->dc_global_min);
             if (!global) {
-                // a workgroup of min(max(np, 64), 256) threads per node: the
-                // small nodes of the low levels leave no thread of a larger
-                // group idle at every barrier, and more of them fit per CU
+                // a workgroup of min(max(np / 2, 64), 256) threads per node:
+                // the small nodes of the low levels leave fewer threads of a
+                // larger group idle at every barrier, and more of them fit
+                // per CU (np / 2 measured faster than np and np / 4)
                 const size_t shmem = (size_t)np * (6 * sizeof(double) + 4 * sizeof(int));
-                auto k = np <= 64 ? k_dc_merge<false, 64>
-                                  : (np <= 128 ? k_dc_merge<false, 128> : k_dc_merge<false, kMT>);
-                const int nt = np <= 64 ? 64 : (np <= 128 ? 128 : kMT);
+                const int half = np / 2;
+                auto k = half <= 64 ? k_dc_merge<false, 64>
+                                    : (half <= 128 ? k_dc_merge<false, 128> : k_dc_merge<false, kMT>);
+                const int nt = half <= 64 ? 64 : (half <= 128 ? 128 : kMT);
                 hipLaunchKernelGGL(k, dim3(i1 - i0), dim3(nt), shmem, ctx->stream, ddesc, dnodes + i0,
                                    np, e, D, Z, W, X, rot, dgemm + 2 * i0, nullptr, nullptr);
             } else {
