@@ -6,8 +6,9 @@
 // most kDcLeaf rows.  A split at sp couples rows sp-1 and sp through beta = e[sp-1]:
 //   T = diag(T1 - |beta| e_last e_last^T, T2 - |beta| e_1 e_1^T) + |beta| u u^T,
 //   u = [e_last ; sgn(beta) e_1].
-// Leaves: implicit QL with Wilkinson-type shifts (tql2), one wavefront per leaf,
-// lane r owning row r of the leaf's eigenvector block in LDS.
+// Leaves: implicit QL with Wilkinson-type shifts (tql2), four leaves per
+// wavefront (kDcLeaf lanes each), lane r owning row r of its leaf's eigenvector
+// block in LDS.
 // Merge of (Q1,D1),(Q2,D2): z = [Q1(last,:); sgn Q2(first,:)] / sqrt 2,
 // rho = 2|beta|; sort D; deflate (small rho|z_i| or close pairs rotated together);
 // solve the secular equation 1 + rho sum w_i^2/(d_i - lam) = 0 for
@@ -26,16 +27,34 @@ namespace {
 constexpr int kMT = 256;
 constexpr double kEps = DBL_EPSILON;
 
-__global__ __launch_bounds__(64) void k_dc_leaf(const SubDesc *__restrict__ desc,
-                                                const DcNode *__restrict__ leaves,
-                                                double *__restrict__ d_all,
-                                                const double *__restrict__ e_all,
-                                                double *__restrict__ Za) {
-    __shared__ double z[kDcLeaf][kDcLeaf + 1];
-    __shared__ double dl[kDcLeaf], el[kDcLeaf];
-    const DcNode nd = leaves[blockIdx.x];
+// Leaves per wavefront: kDcLeaf lanes each.  Every lane of a group runs the
+// leaf's scalar recurrence (same values, same branches), so one wavefront
+// issues the work of four leaves; the groups diverge, so the group's LDS
+// hand-offs are ordered by wavefront-scope fences instead of barriers (the
+// lanes of a wavefront are one scope, and the group's lanes are converged).
+constexpr int kLeafPack = PTB_WAVE / kDcLeaf;
+
+__device__ __forceinline__ void group_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(PTB_WAVE) void k_dc_leaf(const SubDesc *__restrict__ desc,
+                                                      const DcNode *__restrict__ leaves, int nleaves,
+                                                      double *__restrict__ d_all,
+                                                      const double *__restrict__ e_all,
+                                                      double *__restrict__ Za) {
+    __shared__ double zs[kLeafPack][kDcLeaf][kDcLeaf + 1];
+    __shared__ double dls[kLeafPack][kDcLeaf], els[kLeafPack][kDcLeaf];
+    const int grp = threadIdx.x / kDcLeaf, lane = threadIdx.x % kDcLeaf;
+    const int li = blockIdx.x * kLeafPack + grp;
+    if (li >= nleaves) return;
+    double(*z)[kDcLeaf + 1] = zs[grp];
+    double *dl = dls[grp], *el = els[grp];
+    const DcNode nd = leaves[li];
     const SubDesc ds = desc[nd.s];
-    const int n = nd.hi - nd.lo, lane = threadIdx.x;
+    const int n = nd.hi - nd.lo;
     const double *e = e_all + ds.mapoff;
     double *d = d_all + ds.mapoff;
     if (lane < n) {
@@ -45,11 +64,10 @@ __global__ __launch_bounds__(64) void k_dc_leaf(const SubDesc *__restrict__ desc
         dl[lane] = dv;
         el[lane] = (lane < n - 1) ? e[nd.lo + lane] : 0.0;
     }
-    if (lane < kDcLeaf)
-        for (int c = 0; c < kDcLeaf; ++c) z[lane][c] = (lane == c) ? 1.0 : 0.0;
-    __syncthreads();
-    // every lane runs the scalar recurrence on the shared d/e (same values);
-    // lane r applies each rotation to row r of z
+    for (int c = 0; c < kDcLeaf; ++c) z[lane][c] = (lane == c) ? 1.0 : 0.0;
+    group_lds_sync();
+    // every lane of the group runs the scalar recurrence on the shared d/e
+    // (same values); lane r applies each rotation to row r of z
     for (int l = 0; l < n; ++l) {
         for (int iter = 0; iter < 60; ++iter) {
             int mm = l;
@@ -66,14 +84,14 @@ __global__ __launch_bounds__(64) void k_dc_leaf(const SubDesc *__restrict__ desc
             for (int i = mm - 1; i >= l; --i) {
                 const double f = s * el[i], b = c * el[i];
                 r = hypot(f, g);
-                __syncthreads();
+                group_lds_sync();
                 if (lane == 0) el[i + 1] = r;
                 if (r == 0.0) {
                     if (lane == 0) {
                         dl[i + 1] -= p;
                         el[mm] = 0.0;
                     }
-                    __syncthreads();
+                    group_lds_sync();
                     under = true;
                     break;
                 }
@@ -89,19 +107,19 @@ __global__ __launch_bounds__(64) void k_dc_leaf(const SubDesc *__restrict__ desc
                     z[lane][i + 1] = s * zi + c * zi1;
                     z[lane][i] = c * zi - s * zi1;
                 }
-                __syncthreads();
+                group_lds_sync();
             }
             if (under) continue;
-            __syncthreads();
+            group_lds_sync();
             if (lane == 0) {
                 dl[l] -= p;
                 el[l] = g;
                 el[mm] = 0.0;
             }
-            __syncthreads();
+            group_lds_sync();
         }
     }
-    __syncthreads();
+    group_lds_sync();
     if (lane < n) d[nd.lo + lane] = dl[lane];
     double *Z = Za + ds.off;
     for (int c = 0; c < n; ++c)
@@ -733,7 +751,8 @@ int dc_leaves(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dleaves, int 
               double *d, const double *e, double *Z) {
     ProfScope prof_(ctx, "dc_leaf");
     if (nleaves <= 0) return PTBGPU_OK;
-    hipLaunchKernelGGL(k_dc_leaf, dim3(nleaves), dim3(64), 0, ctx->stream, ddesc, dleaves, d, e, Z);
+    hipLaunchKernelGGL(k_dc_leaf, dim3((nleaves + kLeafPack - 1) / kLeafPack), dim3(PTB_WAVE), 0, ctx->stream,
+                       ddesc, dleaves, nleaves, d, e, Z);
     PTB_CHECK_LAUNCH(ctx);
     return PTBGPU_OK;
 }
