@@ -794,15 +794,19 @@ int dc_merge_level(ptbgpu_ctx *ctx, const SubDesc *ddesc, const DcNode *dnodes, 
             const int np = runs[r].second;
             const bool global = !in_lds(np) || (ctx->dc_global_min > 0 && np >= ctx->dc_global_min);
             if (!global) {
-                // a workgroup of min(max(np / 2, 64), 256) threads per node:
+                // a workgroup of min(max(np / 2, 64), 512) threads per node:
                 // the small nodes of the low levels leave fewer threads of a
                 // larger group idle at every barrier, and more of them fit
-                // per CU (np / 2 measured faster than np and np / 4)
+                // per CU (np / 2 measured faster than np and np / 4); nodes
+                // of 1024 rows and more, whose LDS allows two or one per CU,
+                // fill the CU's wavefront slots with 512 threads
                 const size_t shmem = (size_t)np * (6 * sizeof(double) + 4 * sizeof(int));
                 const int half = np / 2;
-                auto k = half <= 64 ? k_dc_merge<false, 64>
-                                    : (half <= 128 ? k_dc_merge<false, 128> : k_dc_merge<false, kMT>);
-                const int nt = half <= 64 ? 64 : (half <= 128 ? 128 : kMT);
+                auto k = half <= 64    ? k_dc_merge<false, 64>
+                         : half <= 128 ? k_dc_merge<false, 128>
+                         : half <= 256 ? k_dc_merge<false, 256>
+                                       : k_dc_merge<false, 512>;
+                const int nt = half <= 64 ? 64 : (half <= 128 ? 128 : (half <= 256 ? 256 : 512));
                 hipLaunchKernelGGL(k, dim3(i1 - i0), dim3(nt), shmem, ctx->stream, ddesc, dnodes + i0,
                                    np, e, D, Z, W, X, rot, dgemm + 2 * i0, nullptr, nullptr);
             } else {
